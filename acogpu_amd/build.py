@@ -1,0 +1,51 @@
+"""Build of the gfx950 shared library (HIP kernels + C ABI) and the acouav CLI.
+
+Every build compiles from source (hipcc cross-compiles gfx950 without a GPU).
+Outputs land in acogpu_amd/_build/ (git-ignored, shipped with the tree).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT, "libacogpu_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off: fp32 expressions evaluate one IEEE operation at a time
+# (bit-exact replay against the CPU oracle).  -packed-fp32-ops: no packed fp32
+# instructions are generated.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result", "-Wno-unused-value", "-Wno-unused-command-line-argument",
+]
+
+LIB_SOURCES = ["aco_kernels.hip", "aco_capi.hip"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib() -> str:
+    os.makedirs(OUT, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in LIB_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_all() -> None:
+    build_lib()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,249 @@
+"""ctypes binding of the C ABI (include/acogpu.h) of libacogpu_amd.so.
+
+This is the Python host side used by the tests, smoke() and bench.py.  It
+loads the in-tree library built by acogpu_amd/build.py and fails loudly when
+it is missing: there is no CPU fallback anywhere in the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libacogpu_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "acogpu.h")
+
+VARIANT_AS, VARIANT_RBAS = 0, 1
+MODE_CANDIDATE, MODE_FULLROW = 0, 1
+TAU0_CONST, TAU0_AS, TAU0_RBAS = 0, 1, 2
+MAX_DISTANCE = 0x7FFFFFFF
+
+
+class AcoError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("variant", ctypes.c_int32), ("mode", ctypes.c_int32), ("n", ctypes.c_int32),
+        ("m", ctypes.c_int32), ("k", ctypes.c_int32), ("w", ctypes.c_int32),
+        ("alpha", ctypes.c_float), ("beta", ctypes.c_float), ("rho", ctypes.c_float),
+        ("tau0_policy", ctypes.c_int32), ("tau0", ctypes.c_float), ("aging", ctypes.c_int32),
+        ("min_age", ctypes.c_int32), ("max_age", ctypes.c_int32), ("local_perf", ctypes.c_int32),
+        ("global_perf", ctypes.c_int32), ("min_pop", ctypes.c_int32), ("max_pop", ctypes.c_int32),
+        ("broker", ctypes.c_int32), ("ants_per_block", ctypes.c_int32), ("seed", ctypes.c_uint64),
+        ("stream_id", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+_lib = None
+
+# every entry point declared in include/acogpu.h: name -> (restype, argtypes)
+_vp, _i32, _i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+SIGNATURES = {
+    "aco_params_init": (ctypes.c_int, [_vp]),
+    "aco_create": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "aco_initialize": (ctypes.c_int, [_vp]),
+    "aco_iterate": (ctypes.c_int, [_vp, _vp]),
+    "aco_run": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_get_stats": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "aco_get_best_tour": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_tours": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_lengths": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_pheromone": (ctypes.c_int, [_vp, _vp]),
+    "aco_set_pheromone": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_choice": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_choice_nn": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_nn": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_mean": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_dist": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_eta": (ctypes.c_int, [_vp, _vp, _vp]),
+    "aco_get_aging": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "aco_set_aging": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "aco_set_best": (ctypes.c_int, [_vp, _i64, _vp]),
+    "aco_get_counters": (ctypes.c_int, [_vp, _vp]),
+    "aco_profile": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_calibrate_copy": (ctypes.c_int, [_vp, _i64, _i32, _vp]),
+    "aco_destroy": (ctypes.c_int, [_vp]),
+    "aco_last_error": (ctypes.c_char_p, [_vp]),
+    "aco_version": (ctypes.c_char_p, []),
+}
+
+
+def lib():
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AcoError(f"{LIB_PATH} is missing: run `python -m acogpu_amd.build` first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().aco_params_init(ctypes.byref(p))
+    for key, val in kw.items():
+        if not hasattr(p, key):
+            raise AttributeError(key)
+        setattr(p, key, val)
+    return p
+
+
+class Colony:
+    """One aco_ctx: the Colony/AntSystem/RankBasedAntSystem state on the GPU."""
+
+    def __init__(self, params: Params, dist=None, xy=None):
+        self._L = lib()
+        self.params = params
+        self.n, self.m = params.n, params.m
+        self._ctx = ctypes.c_void_p()
+        d = None if dist is None else np.ascontiguousarray(dist, dtype=np.int32)
+        c = None if xy is None else np.ascontiguousarray(xy, dtype=np.int32)
+        rc = self._L.aco_create(ctypes.byref(params), _p(d), _p(c), ctypes.byref(self._ctx))
+        if rc != 0:
+            raise AcoError(f"aco_create failed ({rc}): {self._L.aco_last_error(None).decode()}")
+
+    # -- lifecycle ---------------------------------------------------------
+    def _chk(self, rc):
+        if rc != 0:
+            raise AcoError(f"({rc}) {self._L.aco_last_error(self._ctx).decode()}")
+
+    def close(self):
+        if self._ctx:
+            self._L.aco_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def initialize(self):
+        self._chk(self._L.aco_initialize(self._ctx))
+
+    def iterate(self):
+        t = np.zeros(6, dtype=np.float32)
+        self._chk(self._L.aco_iterate(self._ctx, _p(t)))
+        return t
+
+    def run(self, iterations):
+        ms = ctypes.c_float()
+        self._chk(self._L.aco_run(self._ctx, int(iterations), ctypes.byref(ms)))
+        return ms.value
+
+    def profile(self, iterations):
+        out = np.zeros(4, dtype=np.float32)
+        self._chk(self._L.aco_profile(self._ctx, int(iterations), _p(out)))
+        return out
+
+    def calibrate_copy(self, nbytes, reps=20):
+        ms = ctypes.c_float()
+        self._chk(self._L.aco_calibrate_copy(self._ctx, int(nbytes), int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    # -- state -------------------------------------------------------------
+    def stats(self):
+        ib, gb, nv = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_uint32()
+        self._chk(self._L.aco_get_stats(self._ctx, ctypes.byref(ib), ctypes.byref(gb), ctypes.byref(nv)))
+        return ib.value, gb.value, nv.value
+
+    def best_tour(self):
+        t = np.zeros(self.n, dtype=np.int32)
+        self._chk(self._L.aco_get_best_tour(self._ctx, _p(t)))
+        return t
+
+    def set_best(self, glob_best, tour=None):
+        t = None if tour is None else np.ascontiguousarray(tour, dtype=np.int32)
+        self._chk(self._L.aco_set_best(self._ctx, int(glob_best), _p(t)))
+
+    def tours(self):
+        t = np.zeros((self.m, self.n), dtype=np.int32)
+        self._chk(self._L.aco_get_tours(self._ctx, _p(t)))
+        return t
+
+    def lengths(self):
+        t = np.zeros(self.m, dtype=np.int64)
+        self._chk(self._L.aco_get_lengths(self._ctx, _p(t)))
+        return t
+
+    def pheromone(self):
+        t = np.zeros((self.n, self.n), dtype=np.float32)
+        self._chk(self._L.aco_get_pheromone(self._ctx, _p(t)))
+        return t
+
+    def set_pheromone(self, tau):
+        t = np.ascontiguousarray(tau, dtype=np.float32)
+        self._chk(self._L.aco_set_pheromone(self._ctx, _p(t)))
+
+    def choice(self):
+        t = np.zeros((self.n, self.n), dtype=np.float32)
+        self._chk(self._L.aco_get_choice(self._ctx, _p(t)))
+        return t
+
+    def choice_nn(self):
+        t = np.zeros((self.n, self.params.k), dtype=np.float32)
+        self._chk(self._L.aco_get_choice_nn(self._ctx, _p(t)))
+        return t
+
+    def nn(self):
+        cols = self.params.k if self.params.mode == MODE_CANDIDATE else self.n
+        t = np.zeros((self.n, cols), dtype=np.int32)
+        self._chk(self._L.aco_get_nn(self._ctx, _p(t)))
+        return t
+
+    def mean(self):
+        t = np.zeros(self.n, dtype=np.float32)
+        self._chk(self._L.aco_get_mean(self._ctx, _p(t)))
+        return t
+
+    def dist(self):
+        t = np.zeros((self.n, self.n), dtype=np.int32)
+        self._chk(self._L.aco_get_dist(self._ctx, _p(t)))
+        return t
+
+    def eta(self):
+        ln = ctypes.c_int32()
+        self._chk(self._L.aco_get_eta(self._ctx, None, ctypes.byref(ln)))
+        t = np.zeros(ln.value, dtype=np.float32)
+        self._chk(self._L.aco_get_eta(self._ctx, _p(t), ctypes.byref(ln)))
+        return t
+
+    def aging(self):
+        age = np.zeros(self.m, dtype=np.int32)
+        valid = np.zeros(self.m, dtype=np.uint8)
+        perf = np.zeros(self.m, dtype=np.int32)
+        prev = np.zeros(self.m, dtype=np.int64)
+        sc = np.zeros(4, dtype=np.int32)
+        self._chk(self._L.aco_get_aging(self._ctx, _p(age), _p(valid), _p(perf), _p(prev), _p(sc)))
+        return {"age": age, "valid": valid, "perf": perf, "prev_len": prev, "num_valid": int(sc[0]),
+                "converged": int(sc[1]), "gperf": int(sc[2]), "iter": int(sc[3])}
+
+    def set_aging(self, age=None, valid=None, perf=None, prev_len=None, scalars=None):
+        conv = lambda a, dt: None if a is None else np.ascontiguousarray(a, dtype=dt)
+        self._chk(self._L.aco_set_aging(self._ctx, _p(conv(age, np.int32)), _p(conv(valid, np.uint8)),
+                                        _p(conv(perf, np.int32)), _p(conv(prev_len, np.int64)),
+                                        _p(conv(scalars, np.int32))))
+
+    def counters(self):
+        t = np.zeros(2, dtype=np.uint64)
+        self._chk(self._L.aco_get_counters(self._ctx, _p(t)))
+        return int(t[0]), int(t[1])

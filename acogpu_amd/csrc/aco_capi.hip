@@ -1,0 +1,671 @@
+// C ABI of the MI355X ACO solver (include/acogpu.h): context lifetime, device
+// buffers, the per-iteration kernel sequence and its hipGraph replay, parity
+// taps.  All device memory is allocated in aco_create; an iteration allocates
+// nothing and performs no host synchronisation.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/acogpu.h"
+#include "aco_kernels.h"
+
+using aco::DevState;
+
+struct aco_ctx {
+    aco_params p;
+    int32_t n = 0, m = 0, k = 0, P = 0, words = 0, apb = 2;
+    int32_t nn_cols = 0;  // k (candidate) or n (full row)
+    hipStream_t stream = nullptr;
+    int32_t* d_dist = nullptr;
+    float* d_tau[2] = {nullptr, nullptr};
+    float* d_choice = nullptr;
+    float* d_choice_nn = nullptr;
+    int32_t* d_nn = nullptr;
+    float* d_eta = nullptr;
+    int32_t eta_len = 0;
+    float* d_mean = nullptr;
+    int32_t* d_tours = nullptr;
+    int64_t* d_len = nullptr;
+    int64_t* d_prev_len = nullptr;
+    int32_t* d_age = nullptr;
+    int32_t* d_perf = nullptr;
+    uint8_t* d_valid = nullptr;
+    int32_t* d_best_tour = nullptr;
+    int32_t* d_dep_ant = nullptr;
+    float* d_dep_val = nullptr;
+    DevState* d_state = nullptr;
+    int64_t* d_scratch64 = nullptr;
+    int32_t cur = 0;           // index of the current pheromone buffer
+    bool initialized = false;
+    hipGraphExec_t graph = nullptr;  // two iterations starting at cur == 0
+    hipEvent_t ev[6] = {};
+    char err[512] = {0};
+};
+
+static thread_local char g_create_err[512] = "no error";
+
+static int fail(aco_ctx* c, int code, const char* fmt, ...) {
+    char* buf = c ? c->err : g_create_err;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((ctx), ACO_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                     \
+    } while (0)
+
+extern "C" const char* aco_version(void) { return "acogpu_amd 0.1 (gfx950)"; }
+
+extern "C" const char* aco_last_error(const aco_ctx* c) { return c ? c->err : g_create_err; }
+
+extern "C" int aco_params_init(aco_params* p) {
+    if (!p) return ACO_E_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->variant = ACO_VARIANT_AS;
+    p->mode = ACO_MODE_CANDIDATE;
+    p->m = 10;           // DEFAULT_NUM_ANTS
+    p->k = 20;
+    p->w = 5;            // DEFAULT_NUM_RANK_ANTS
+    p->alpha = 1.0f;     // DEFAULT_ALPHA
+    p->beta = 5.0f;      // DEFAULT_BETA
+    p->rho = 0.1f;       // DEFAULT_RHO
+    p->tau0_policy = ACO_TAU0_CONST;
+    p->tau0 = 1e-6f;     // DEFAULT_PHEROMONE
+    p->min_age = 30;     // DEFAULT_MIN_RITIREMENT_AGE
+    p->max_age = 50;     // DEFAULT_MAX_RITIREMENT_AGE
+    p->local_perf = 10;  // DEFAULT_LOCAL_NO_CONVERGENCE
+    p->global_perf = 10; // DEFAULT_GLOBAL_NO_CONVERGENCE
+    p->min_pop = 40;     // DEFAULT_MIN_POPULATION_LIMIT
+    p->max_pop = 100;    // DEFAULT_MAX_POPULATION_LIMIT
+    p->seed = 12345678;  // RANDOM_SEED (inc/Colony.h:15)
+    return ACO_OK;
+}
+
+static void free_all(aco_ctx* c) {
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    void* ptrs[] = {c->d_dist, c->d_tau[0], c->d_tau[1], c->d_choice, c->d_choice_nn, c->d_nn, c->d_eta,
+                    c->d_mean, c->d_tours, c->d_len, c->d_prev_len, c->d_age, c->d_perf, c->d_valid,
+                    c->d_best_tour, c->d_dep_ant, c->d_dep_val, c->d_state, c->d_scratch64};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+
+template <typename T>
+static int dalloc(aco_ctx* c, T** out, size_t count) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(out), count * sizeof(T) + 16);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        (void)hipGetLastError();
+        return fail(c, ACO_E_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return ACO_OK;
+}
+
+static int check_params(const aco_params* p) {
+    if (p->n < 2) return fail(nullptr, ACO_E_ARG, "n must be >= 2 (got %d)", p->n);
+    if (p->m < 1) return fail(nullptr, ACO_E_ARG, "m must be >= 1 (got %d)", p->m);
+    if (p->variant != ACO_VARIANT_AS && p->variant != ACO_VARIANT_RBAS)
+        return fail(nullptr, ACO_E_ARG, "unknown variant %d", p->variant);
+    if (p->mode == ACO_MODE_CANDIDATE) {
+        if (p->k < 1 || p->k > 64 || p->k > p->n - 1)
+            return fail(nullptr, ACO_E_ARG, "candidate list length k=%d must be in [1, min(64, n-1)]", p->k);
+    } else if (p->mode == ACO_MODE_FULLROW) {
+        if (p->n > 8192) return fail(nullptr, ACO_E_RANGE, "full-row mode supports n <= 8192 (got %d)", p->n);
+    } else {
+        return fail(nullptr, ACO_E_ARG, "unknown mode %d", p->mode);
+    }
+    if (!(p->rho >= 0.0f && p->rho <= 1.0f)) return fail(nullptr, ACO_E_ARG, "rho must be in [0,1]");
+    if (!(p->alpha >= 0.0f) || !(p->beta >= 0.0f)) return fail(nullptr, ACO_E_ARG, "alpha, beta must be >= 0");
+    if (p->variant == ACO_VARIANT_RBAS && p->w < 1) return fail(nullptr, ACO_E_ARG, "w must be >= 1");
+    if (p->aging && (p->min_age < 1 || p->max_age < 0))
+        return fail(nullptr, ACO_E_ARG, "aging needs min_age >= 1 and max_age >= 0");
+    if (p->ants_per_block < 0 || p->ants_per_block > 4)
+        return fail(nullptr, ACO_E_ARG, "ants_per_block must be in [0, 4]");
+    return ACO_OK;
+}
+
+extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_t* xy, aco_ctx** out) {
+    if (!p || !out) return fail(nullptr, ACO_E_ARG, "null argument");
+    *out = nullptr;
+    if ((dist == nullptr) == (xy == nullptr))
+        return fail(nullptr, ACO_E_ARG, "exactly one of dist and xy must be given");
+    int rc = check_params(p);
+    if (rc) return rc;
+
+    aco_ctx* c = new aco_ctx();
+    c->p = *p;
+    // RankBasedAntSystem::computeParameters clamps w to n (src/RankBasedAntSystem.cu:35-40).
+    if (c->p.variant == ACO_VARIANT_RBAS && c->p.w > c->p.n) c->p.w = c->p.n;
+    c->n = p->n;
+    c->m = p->m;
+    c->k = p->mode == ACO_MODE_CANDIDATE ? p->k : 0;
+    c->nn_cols = p->mode == ACO_MODE_CANDIDATE ? p->k : p->n;
+    c->P = (c->n + 63) & ~63;
+    c->words = (c->n + 31) / 32;
+    c->apb = p->ants_per_block ? p->ants_per_block : (p->mode == ACO_MODE_CANDIDATE ? 1 : 1);
+
+    auto bail = [&](int code) {
+        free_all(c);
+        if (code != ACO_OK && c->err[0]) std::snprintf(g_create_err, sizeof(g_create_err), "%s", c->err);
+        delete c;
+        return code;
+    };
+#define TRYB(expr)                     \
+    do {                               \
+        int r_ = (expr);               \
+        if (r_ != ACO_OK) return bail(r_); \
+    } while (0)
+#define HIPB(expr)                                                                                    \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return bail(fail(c, ACO_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)));          \
+    } while (0)
+
+    HIPB(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIPB(hipEventCreate(&e));
+    const size_t nP = (size_t)c->n * c->P;
+    TRYB(dalloc(c, &c->d_dist, nP));
+    TRYB(dalloc(c, &c->d_tau[0], nP));
+    TRYB(dalloc(c, &c->d_tau[1], nP));
+    TRYB(dalloc(c, &c->d_choice, nP));
+    TRYB(dalloc(c, &c->d_nn, (size_t)c->n * c->nn_cols));
+    TRYB(dalloc(c, &c->d_choice_nn, (size_t)c->n * (c->k > 0 ? c->k : 1)));
+    TRYB(dalloc(c, &c->d_mean, (size_t)c->n));
+    TRYB(dalloc(c, &c->d_tours, (size_t)c->m * c->n));
+    TRYB(dalloc(c, &c->d_len, (size_t)c->m));
+    TRYB(dalloc(c, &c->d_prev_len, (size_t)c->m));
+    TRYB(dalloc(c, &c->d_age, (size_t)c->m));
+    TRYB(dalloc(c, &c->d_perf, (size_t)c->m));
+    TRYB(dalloc(c, &c->d_valid, (size_t)c->m));
+    TRYB(dalloc(c, &c->d_best_tour, (size_t)c->n));
+    TRYB(dalloc(c, &c->d_dep_ant, (size_t)c->m + c->p.w + 2));
+    TRYB(dalloc(c, &c->d_dep_val, (size_t)c->m + c->p.w + 2));
+    TRYB(dalloc(c, &c->d_state, 1));
+    TRYB(dalloc(c, &c->d_scratch64, 4));
+    HIPB(hipMemsetAsync(c->d_dist, 0, nP * sizeof(int32_t), c->stream));
+    HIPB(hipMemsetAsync(c->d_tau[0], 0, nP * sizeof(float), c->stream));
+    HIPB(hipMemsetAsync(c->d_tau[1], 0, nP * sizeof(float), c->stream));
+    HIPB(hipMemsetAsync(c->d_choice, 0, nP * sizeof(float), c->stream));
+    HIPB(hipMemsetAsync(c->d_tours, 0, (size_t)c->m * c->n * sizeof(int32_t), c->stream));
+    HIPB(hipMemsetAsync(c->d_len, 0, (size_t)c->m * sizeof(int64_t), c->stream));
+
+    // Distance matrix.
+    if (xy) {
+        for (int32_t i = 0; i < 2 * c->n; ++i)
+            if (xy[i] >= (1 << 29) || xy[i] <= -(1 << 29))
+                return bail(fail(c, ACO_E_RANGE, "coordinate %d out of range (|x| < 2^29)", xy[i]));
+        int32_t* d_xy = nullptr;
+        TRYB(dalloc(c, &d_xy, (size_t)2 * c->n));
+        hipError_t e = hipMemcpyAsync(d_xy, xy, sizeof(int32_t) * 2 * c->n, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = aco::launch_dist_from_xy(d_xy, c->n, c->P, c->d_dist, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(d_xy);
+        HIPB(e);
+    } else {
+        // The reference passes a device pointer in its GPU build and a host
+        // pointer in its CPU build (src/Setup.cpp:54-70); accept both.
+        hipPointerAttribute_t attr;
+        hipMemcpyKind kind = hipMemcpyHostToDevice;
+        if (hipPointerGetAttributes(&attr, dist) == hipSuccess && attr.type == hipMemoryTypeDevice)
+            kind = hipMemcpyDeviceToDevice;
+        (void)hipGetLastError();
+        HIPB(hipMemcpy2DAsync(c->d_dist, (size_t)c->P * 4, dist, (size_t)c->n * 4, (size_t)c->n * 4, c->n, kind,
+                              c->stream));
+        HIPB(aco::launch_fix_diagonal(c->d_dist, c->n, c->P, c->stream));
+    }
+
+    // Heuristic table eta^beta by integer distance, fp64 pow rounded once.
+    int32_t* d_max = reinterpret_cast<int32_t*>(c->d_scratch64);
+    HIPB(hipMemsetAsync(d_max, 0, sizeof(int32_t), c->stream));
+    HIPB(aco::launch_dist_max(c->d_dist, c->n, c->P, d_max, c->stream));
+    int32_t dmax = 0;
+    HIPB(hipMemcpyAsync(&dmax, d_max, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPB(hipStreamSynchronize(c->stream));
+    if (dmax < 0 || dmax > (1 << 26))
+        return bail(fail(c, ACO_E_RANGE, "largest distance %d exceeds the heuristic table limit 2^26", dmax));
+    c->eta_len = dmax + 1;
+    std::vector<float> eta((size_t)c->eta_len);
+    eta[0] = (float)std::pow(2.0, (double)c->p.beta);  // d == 0: duplicate cities, taken as d = 1/2
+    for (int32_t d = 1; d <= dmax; ++d) eta[d] = (float)std::pow(1.0 / (double)d, (double)c->p.beta);
+    TRYB(dalloc(c, &c->d_eta, (size_t)c->eta_len));
+    HIPB(hipMemcpy(c->d_eta, eta.data(), sizeof(float) * eta.size(), hipMemcpyHostToDevice));
+    HIPB(hipDeviceSynchronize());
+    *out = c;
+    return ACO_OK;
+#undef TRYB
+#undef HIPB
+}
+
+extern "C" int aco_destroy(aco_ctx* c) {
+    if (!c) return ACO_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return ACO_OK;
+}
+
+static int reset_state(aco_ctx* c) {
+    DevState st{};
+    st.iter_best = (int64_t)0x7fffffff - 1;  // iterBestDist = MAX_DISTANCE - 1 (src/Colony.cu:38)
+    st.glob_best = (int64_t)0x7fffffff;      // globBestDist = MAX_DISTANCE (src/Colony.cu:39)
+    st.iter_best_ant = -1;
+    st.gperf = 0;  // globalperformance, uninitialised in the reference (inc/Colony.h:114)
+    st.iter = 0;
+    st.num_valid = c->m;  // intialpopulation = numAnts (src/Colony.cu:132)
+    st.converged = 0;
+    st.ndep = 0;
+    HIP_TRY(c, hipMemcpyAsync(c->d_state, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+    std::vector<uint8_t> ones((size_t)c->m, 1);
+    HIP_TRY(c, hipMemcpyAsync(c->d_valid, ones.data(), (size_t)c->m, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_age, 0, sizeof(int32_t) * c->m, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_perf, 0, sizeof(int32_t) * c->m, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_prev_len, 0, sizeof(int64_t) * c->m, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_len, 0, sizeof(int64_t) * c->m, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_best_tour, 0, sizeof(int32_t) * c->n, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+extern "C" int aco_initialize(aco_ctx* c) {
+    if (!c) return ACO_E_ARG;
+    // Colony::computeParameters -> computeInitialPheromone (src/AntSystem.cu:194-203,
+    // src/RankBasedAntSystem.cu:353-363).
+    float tau0 = c->p.tau0;
+    if (c->p.tau0_policy != ACO_TAU0_CONST) {
+        HIP_TRY(c, aco::launch_greedy(c->d_dist, c->n, c->P, 0, c->d_scratch64, c->stream));
+        int64_t cnn = 0;
+        HIP_TRY(c, hipMemcpyAsync(&cnn, c->d_scratch64, sizeof(cnn), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (cnn <= 0) return fail(c, ACO_E_STATE, "nearest-neighbour tour length is %lld", (long long)cnn);
+        if (c->p.tau0_policy == ACO_TAU0_AS)
+            tau0 = (float)((double)c->m / (double)cnn);
+        else
+            tau0 = (float)(0.5 * c->p.w * (c->p.w - 1) / ((double)c->p.rho * (double)cnn));
+        c->p.tau0 = tau0;
+    }
+    c->cur = 0;
+    HIP_TRY(c, aco::launch_fill_rows(c->d_tau[0], c->n, c->P, tau0, c->stream));
+    HIP_TRY(c, aco::launch_fill_rows(c->d_tau[1], c->n, c->P, 0.0f, c->stream));
+    HIP_TRY(c, aco::launch_nn_lists(c->d_dist, c->n, c->P, c->nn_cols, c->d_nn, c->stream));
+    int rc = reset_state(c);
+    if (rc) return rc;
+    if (c->graph) {
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    c->initialized = true;
+    return ACO_OK;
+}
+
+// The four kernels of one iteration on the ctx stream, reading pheromone
+// buffer `cur` and producing buffer 1-cur.  ev (may be null) brackets them.
+static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
+    hipError_t e;
+    float* tau = c->d_tau[cur];
+    float* tau_next = c->d_tau[1 - cur];
+    const bool cand = c->p.mode == ACO_MODE_CANDIDATE;
+    if (ev) hipEventRecord(ev[0], c->stream);
+
+    aco::SweepArgs sa{};
+    sa.tau = tau;
+    sa.dist = c->d_dist;
+    sa.eta = c->d_eta;
+    sa.eta_len = c->eta_len;
+    sa.nn = cand ? c->d_nn : nullptr;
+    sa.k = c->k;
+    sa.choice = c->d_choice;
+    sa.choice_nn = c->d_choice_nn;
+    sa.tau_next = tau_next;
+    sa.mean = c->d_mean;
+    sa.n = c->n;
+    sa.P = c->P;
+    sa.alpha = c->p.alpha;
+    sa.one_minus_rho = 1.0f - c->p.rho;
+    sa.want_mean = c->p.aging;
+    if ((e = aco::launch_sweep(sa, c->stream)) != hipSuccess) return e;
+    if (ev) hipEventRecord(ev[1], c->stream);
+
+    aco::ConstructArgs ca{};
+    ca.nn = c->d_nn;
+    ca.choice_nn = c->d_choice_nn;
+    ca.choice = c->d_choice;
+    ca.dist = c->d_dist;
+    ca.tau = tau;
+    ca.mean = c->d_mean;
+    ca.eta = c->d_eta;
+    ca.eta_len = c->eta_len;
+    ca.tours = c->d_tours;
+    ca.lengths = c->d_len;
+    ca.age = c->d_age;
+    ca.valid = c->d_valid;
+    ca.state = c->d_state;
+    ca.n = c->n;
+    ca.P = c->P;
+    ca.m = c->m;
+    ca.k = c->k;
+    ca.aging = c->p.aging;
+    ca.min_age = c->p.min_age;
+    ca.alpha = c->p.alpha;
+    ca.seed_lo = (uint32_t)c->p.seed;
+    ca.seed_hi = (uint32_t)(c->p.seed >> 32);
+    ca.stream = c->p.stream_id;
+    ca.words = c->words;
+    if ((e = aco::launch_construct(ca, c->p.mode, c->apb, c->stream)) != hipSuccess) return e;
+    if (ev) hipEventRecord(ev[2], c->stream);
+
+    aco::StatsArgs st{};
+    st.tours = c->d_tours;
+    st.lengths = c->d_len;
+    st.valid = c->d_valid;
+    st.age = c->d_age;
+    st.perf = c->d_perf;
+    st.prev_len = c->d_prev_len;
+    st.best_tour = c->d_best_tour;
+    st.dep_ant = c->d_dep_ant;
+    st.dep_val = c->d_dep_val;
+    st.state = c->d_state;
+    st.n = c->n;
+    st.m = c->m;
+    st.variant = c->p.variant;
+    st.w = c->p.w;
+    st.aging = c->p.aging;
+    st.max_age = c->p.max_age;
+    st.global_perf = c->p.global_perf;
+    st.min_pop = c->p.min_pop;
+    if ((e = aco::launch_stats(st, c->stream)) != hipSuccess) return e;
+    if (ev) hipEventRecord(ev[3], c->stream);
+
+    aco::DepositArgs da{};
+    da.tours = c->d_tours;
+    da.best_tour = c->d_best_tour;
+    da.dep_ant = c->d_dep_ant;
+    da.dep_val = c->d_dep_val;
+    da.state = c->d_state;
+    da.tau_next = tau_next;
+    da.n = c->n;
+    da.P = c->P;
+    da.dense = c->p.variant == ACO_VARIANT_AS;
+    const int32_t grid = da.dense ? c->m : c->p.w + 1;
+    if ((e = aco::launch_deposit(da, grid, c->stream)) != hipSuccess) return e;
+    if (ev) hipEventRecord(ev[4], c->stream);
+    return hipSuccess;
+}
+
+extern "C" int aco_iterate(aco_ctx* c, float* timers) {
+    if (!c) return ACO_E_ARG;
+    if (!c->initialized) return fail(c, ACO_E_STATE, "aco_iterate before aco_initialize");
+    HIP_TRY(c, enqueue_iteration(c, c->cur, c->ev));
+    c->cur ^= 1;
+    HIP_TRY(c, hipEventSynchronize(c->ev[4]));
+    if (timers) {
+        float t[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i) HIP_TRY(c, hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
+        // Placement is fused into construction and evaporation into the
+        // probability sweep: their slots report 0, the sweep reports under
+        // TIME_COMPUTE_PROBABILITIES.
+        timers[ACO_TIME_INIT_TOUR] = 0.0f;
+        timers[ACO_TIME_TOUR] = t[1];
+        timers[ACO_TIME_COMPUTE_ANT_DISTANCE] = t[2];
+        timers[ACO_TIME_EVAPORATE_PHEROMONE] = 0.0f;
+        timers[ACO_TIME_UPDATE_PHEROMONE] = t[3];
+        timers[ACO_TIME_COMPUTE_PROBABILITIES] = t[0];
+    }
+    return ACO_OK;
+}
+
+static int build_graph(aco_ctx* c) {
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_iteration(c, 0, nullptr);
+    if (e == hipSuccess) e = enqueue_iteration(c, 1, nullptr);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return fail(c, ACO_E_HIP, "kernel launch during capture failed: %s", hipGetErrorString(e));
+    }
+    HIP_TRY(c, e2);
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIP_TRY(c, e);
+    return ACO_OK;
+}
+
+extern "C" int aco_run(aco_ctx* c, int32_t iterations, float* elapsed_ms) {
+    if (!c) return ACO_E_ARG;
+    if (!c->initialized) return fail(c, ACO_E_STATE, "aco_run before aco_initialize");
+    if (iterations < 0) return fail(c, ACO_E_ARG, "negative iteration count");
+    if (!c->graph) {
+        int rc = build_graph(c);
+        if (rc) return rc;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+    int32_t left = iterations;
+    if (left > 0 && c->cur == 1) {
+        HIP_TRY(c, enqueue_iteration(c, 1, nullptr));
+        c->cur = 0;
+        --left;
+    }
+    for (; left >= 2; left -= 2) HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+    if (left == 1) {
+        HIP_TRY(c, enqueue_iteration(c, 0, nullptr));
+        c->cur = 1;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev[5], c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->ev[5]));
+    if (elapsed_ms) HIP_TRY(c, hipEventElapsedTime(elapsed_ms, c->ev[0], c->ev[5]));
+    return ACO_OK;
+}
+
+extern "C" int aco_profile(aco_ctx* c, int32_t iterations, float* kernel_ms) {
+    if (!c || !kernel_ms || iterations < 1) return ACO_E_ARG;
+    if (!c->initialized) return fail(c, ACO_E_STATE, "aco_profile before aco_initialize");
+    double acc[4] = {0, 0, 0, 0};
+    for (int32_t it = 0; it < iterations; ++it) {
+        HIP_TRY(c, enqueue_iteration(c, c->cur, c->ev));
+        c->cur ^= 1;
+        HIP_TRY(c, hipEventSynchronize(c->ev[4]));
+        for (int i = 0; i < 4; ++i) {
+            float t = 0;
+            HIP_TRY(c, hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+            acc[i] += t;
+        }
+    }
+    for (int i = 0; i < 4; ++i) kernel_ms[i] = (float)(acc[i] / iterations);
+    return ACO_OK;
+}
+
+extern "C" int aco_calibrate_copy(aco_ctx* c, int64_t bytes, int32_t reps, float* ms) {
+    if (!c || !ms || bytes <= 0 || (bytes % 16) != 0 || reps < 1) return ACO_E_ARG;
+    float4 *a = nullptr, *b = nullptr;
+    HIP_TRY(c, hipMalloc(reinterpret_cast<void**>(&a), (size_t)bytes));
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&b), (size_t)bytes);
+    if (e != hipSuccess) {
+        hipFree(a);
+        return fail(c, ACO_E_NOMEM, "calibration buffer: %s", hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(a, 0, (size_t)bytes, c->stream);
+    if (e == hipSuccess) e = aco::launch_copy(a, b, bytes / 16, c->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
+    for (int32_t r = 0; r < reps && e == hipSuccess; ++r) e = aco::launch_copy(a, b, bytes / 16, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+    hipFree(a);
+    hipFree(b);
+    HIP_TRY(c, e);
+    *ms = t / reps;
+    return ACO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Taps and state access.
+static int copy_pitched_out(aco_ctx* c, void* dst, const void* src, size_t elem) {
+    HIP_TRY(c, hipMemcpy2DAsync(dst, (size_t)c->n * elem, src, (size_t)c->P * elem, (size_t)c->n * elem, c->n,
+                                hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+static int copy_out(aco_ctx* c, void* dst, const void* src, size_t bytes) {
+    HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+static int copy_in(aco_ctx* c, void* dst, const void* src, size_t bytes) {
+    HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+static int read_state(aco_ctx* c, DevState* st) { return copy_out(c, st, c->d_state, sizeof(*st)); }
+
+extern "C" int aco_get_stats(aco_ctx* c, int64_t* iter_best, int64_t* glob_best, uint32_t* valid_ants) {
+    if (!c) return ACO_E_ARG;
+    DevState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    if (iter_best) *iter_best = st.iter_best;
+    if (glob_best) *glob_best = st.glob_best;
+    if (valid_ants) *valid_ants = c->p.aging ? (uint32_t)st.num_valid : (uint32_t)c->m;
+    return ACO_OK;
+}
+
+extern "C" int aco_get_best_tour(aco_ctx* c, int32_t* tour) {
+    if (!c || !tour) return ACO_E_ARG;
+    return copy_out(c, tour, c->d_best_tour, sizeof(int32_t) * c->n);
+}
+
+extern "C" int aco_set_best(aco_ctx* c, int64_t glob_best, const int32_t* tour) {
+    if (!c) return ACO_E_ARG;
+    DevState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    st.glob_best = glob_best;
+    rc = copy_in(c, c->d_state, &st, sizeof(st));
+    if (rc) return rc;
+    if (tour) return copy_in(c, c->d_best_tour, tour, sizeof(int32_t) * c->n);
+    return ACO_OK;
+}
+
+extern "C" int aco_get_tours(aco_ctx* c, int32_t* tours) {
+    if (!c || !tours) return ACO_E_ARG;
+    return copy_out(c, tours, c->d_tours, sizeof(int32_t) * (size_t)c->m * c->n);
+}
+
+extern "C" int aco_get_lengths(aco_ctx* c, int64_t* lengths) {
+    if (!c || !lengths) return ACO_E_ARG;
+    return copy_out(c, lengths, c->d_len, sizeof(int64_t) * c->m);
+}
+
+extern "C" int aco_get_pheromone(aco_ctx* c, float* tau) {
+    if (!c || !tau) return ACO_E_ARG;
+    return copy_pitched_out(c, tau, c->d_tau[c->cur], sizeof(float));
+}
+
+extern "C" int aco_set_pheromone(aco_ctx* c, const float* tau) {
+    if (!c || !tau) return ACO_E_ARG;
+    HIP_TRY(c, hipMemcpy2DAsync(c->d_tau[c->cur], (size_t)c->P * 4, tau, (size_t)c->n * 4, (size_t)c->n * 4, c->n,
+                                hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ACO_OK;
+}
+
+extern "C" int aco_get_choice(aco_ctx* c, float* choice) {
+    if (!c || !choice) return ACO_E_ARG;
+    return copy_pitched_out(c, choice, c->d_choice, sizeof(float));
+}
+
+extern "C" int aco_get_choice_nn(aco_ctx* c, float* choice_nn) {
+    if (!c || !choice_nn) return ACO_E_ARG;
+    if (c->k == 0) return fail(c, ACO_E_STATE, "no candidate lists in full-row mode");
+    return copy_out(c, choice_nn, c->d_choice_nn, sizeof(float) * (size_t)c->n * c->k);
+}
+
+extern "C" int aco_get_nn(aco_ctx* c, int32_t* nn) {
+    if (!c || !nn) return ACO_E_ARG;
+    return copy_out(c, nn, c->d_nn, sizeof(int32_t) * (size_t)c->n * c->nn_cols);
+}
+
+extern "C" int aco_get_mean(aco_ctx* c, float* mean) {
+    if (!c || !mean) return ACO_E_ARG;
+    return copy_out(c, mean, c->d_mean, sizeof(float) * c->n);
+}
+
+extern "C" int aco_get_dist(aco_ctx* c, int32_t* dist) {
+    if (!c || !dist) return ACO_E_ARG;
+    return copy_pitched_out(c, dist, c->d_dist, sizeof(int32_t));
+}
+
+extern "C" int aco_get_eta(aco_ctx* c, float* eta, int32_t* eta_len) {
+    if (!c || !eta_len) return ACO_E_ARG;
+    *eta_len = c->eta_len;
+    if (!eta) return ACO_OK;
+    return copy_out(c, eta, c->d_eta, sizeof(float) * c->eta_len);
+}
+
+extern "C" int aco_get_aging(aco_ctx* c, int32_t* age, uint8_t* valid, int32_t* perf, int64_t* prev_len,
+                             int32_t* scalars) {
+    if (!c) return ACO_E_ARG;
+    int rc;
+    if (age && (rc = copy_out(c, age, c->d_age, sizeof(int32_t) * c->m))) return rc;
+    if (valid && (rc = copy_out(c, valid, c->d_valid, (size_t)c->m))) return rc;
+    if (perf && (rc = copy_out(c, perf, c->d_perf, sizeof(int32_t) * c->m))) return rc;
+    if (prev_len && (rc = copy_out(c, prev_len, c->d_prev_len, sizeof(int64_t) * c->m))) return rc;
+    if (scalars) {
+        DevState st;
+        if ((rc = read_state(c, &st))) return rc;
+        scalars[0] = st.num_valid;
+        scalars[1] = st.converged;
+        scalars[2] = st.gperf;
+        scalars[3] = (int32_t)st.iter;
+    }
+    return ACO_OK;
+}
+
+extern "C" int aco_set_aging(aco_ctx* c, const int32_t* age, const uint8_t* valid, const int32_t* perf,
+                             const int64_t* prev_len, const int32_t* scalars) {
+    if (!c) return ACO_E_ARG;
+    int rc;
+    if (age && (rc = copy_in(c, c->d_age, age, sizeof(int32_t) * c->m))) return rc;
+    if (valid && (rc = copy_in(c, c->d_valid, valid, (size_t)c->m))) return rc;
+    if (perf && (rc = copy_in(c, c->d_perf, perf, sizeof(int32_t) * c->m))) return rc;
+    if (prev_len && (rc = copy_in(c, c->d_prev_len, prev_len, sizeof(int64_t) * c->m))) return rc;
+    if (scalars) {
+        DevState st;
+        if ((rc = read_state(c, &st))) return rc;
+        st.num_valid = scalars[0];
+        st.converged = scalars[1];
+        st.gperf = scalars[2];
+        st.iter = (uint32_t)scalars[3];
+        if ((rc = copy_in(c, c->d_state, &st, sizeof(st)))) return rc;
+    }
+    return ACO_OK;
+}
+
+extern "C" int aco_get_counters(aco_ctx* c, uint64_t* counters) {
+    if (!c || !counters) return ACO_E_ARG;
+    DevState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    counters[0] = st.fb_steps;
+    counters[1] = st.fb_scanned;
+    return ACO_OK;
+}

@@ -1,0 +1,100 @@
+// Internal launch API of the gfx950 kernels (aco_kernels.hip).  Not part of
+// the public ABI; aco_capi.hip is the only caller.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace aco {
+
+// Device-resident iteration state: everything the host used to read back
+// every iteration in the reference (src/Colony.cu:918-925,985, 447-467) stays
+// here so an iteration is a fixed sequence of kernels (graph-capturable).
+struct DevState {
+    int64_t iter_best;       // Colony::iterBestDist
+    int64_t glob_best;       // Colony::globBestDist
+    int32_t iter_best_ant;
+    int32_t gperf;           // Colony::globalperformance (aging)
+    uint32_t iter;           // iteration counter (Philox counter word 2)
+    int32_t num_valid;       // numValidAnts (inc/Colony.cuh:10)
+    int32_t converged;       // isconverged (inc/Colony.cuh:11)
+    int32_t ndep;            // entries in the deposit list
+    unsigned long long fb_steps;    // fallback steps taken
+    unsigned long long fb_scanned;  // unvisited cities read by fallback steps
+};
+
+struct SweepArgs {
+    const float* tau;        // current pheromone (pitched n x P)
+    const int32_t* dist;     // pitched n x P
+    const float* eta;        // heuristic table by distance
+    int32_t eta_len;
+    const int32_t* nn;       // n x k candidate lists (may be null)
+    int32_t k;
+    float* choice;           // pitched n x P
+    float* choice_nn;        // n x k
+    float* tau_next;         // evaporated pheromone (pitched)
+    float* mean;             // n row means (aging)
+    int32_t n, P;
+    float alpha, one_minus_rho;
+    int32_t want_mean;
+};
+
+struct ConstructArgs {
+    const int32_t* nn;       // n x k (candidate) or n x n (full row)
+    const float* choice_nn;  // n x k
+    const float* choice;     // pitched
+    const int32_t* dist;     // pitched
+    const float* tau;        // pitched, current pheromone (aging young ants)
+    const float* mean;       // n
+    const float* eta;
+    int32_t eta_len;
+    int32_t* tours;          // m x n
+    int64_t* lengths;        // m
+    int32_t* age;            // m (aging)
+    const uint8_t* valid;    // m (aging)
+    DevState* state;
+    int32_t n, P, m, k;
+    int32_t aging, min_age;
+    float alpha;
+    uint32_t seed_lo, seed_hi, stream;
+    int32_t words;           // visited-bitmap words per ant
+};
+
+struct StatsArgs {
+    const int32_t* tours;
+    const int64_t* lengths;
+    uint8_t* valid;
+    int32_t* age;
+    int32_t* perf;
+    int64_t* prev_len;
+    int32_t* best_tour;
+    int32_t* dep_ant;
+    float* dep_val;
+    DevState* state;
+    int32_t n, m, variant, w;
+    int32_t aging, max_age, global_perf, min_pop;
+};
+
+struct DepositArgs {
+    const int32_t* tours;
+    const int32_t* best_tour;
+    const int32_t* dep_ant;
+    const float* dep_val;
+    const DevState* state;
+    float* tau_next;
+    int32_t n, P, dense;
+};
+
+hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s);
+hipError_t launch_fix_diagonal(int32_t* dist, int32_t n, int32_t P, hipStream_t s);
+hipError_t launch_dist_max(const int32_t* dist, int32_t n, int32_t P, int32_t* out, hipStream_t s);
+hipError_t launch_nn_lists(const int32_t* dist, int32_t n, int32_t P, int32_t k, int32_t* nn, hipStream_t s);
+hipError_t launch_fill_rows(float* a, int32_t n, int32_t P, float v, hipStream_t s);
+hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, hipStream_t s);
+hipError_t launch_sweep(const SweepArgs& a, hipStream_t s);
+hipError_t launch_construct(const ConstructArgs& a, int32_t mode, int32_t ants_per_block, hipStream_t s);
+hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
+hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s);
+hipError_t launch_copy(const float4* src, float4* dst, int64_t count, hipStream_t s);
+
+}  // namespace aco

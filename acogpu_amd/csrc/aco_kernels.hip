@@ -1,0 +1,697 @@
+// gfx950 kernels of the Ant-System hot path.
+//
+// Memory layout in HBM (one colony):
+//   dist, tau[2], choice   n rows x P columns (P = n rounded up to 64), int32 /
+//                          fp32; padding columns hold 0 (tau, choice).
+//   nn, choice_nn          n x k, row-major (candidate lists and their weights).
+//   tours                  m x n int32; lengths m int64.
+// Every kernel is written for wave64: one wave per matrix row in the dense
+// sweeps, one wave per ant in construction.
+//
+// Bit-exactness: this file is compiled with -ffp-contract=off, so every fp32
+// expression is evaluated as written, one IEEE operation at a time, matching
+// the CPU oracle.
+
+#include "aco_device.h"
+#include "aco_kernels.h"
+
+namespace aco {
+
+__device__ __forceinline__ float apply_alpha(float p, float alpha) {
+    return alpha == 1.0f ? p : (float)pow((double)p, (double)alpha);
+}
+
+// ---------------------------------------------------------------------------
+// EUC_2D, exact integer form of (int)(sqrt(dx^2+dy^2)+0.5) (Utils::
+// round_distance src/Utils.cu:212-218): the r with (2r-1)^2 <= 4D < (2r+1)^2.
+__device__ __forceinline__ int32_t euc2d_exact(int64_t dx, int64_t dy) {
+    const int64_t D = dx * dx + dy * dy;
+    int64_t r = (int64_t)(sqrt((double)D) + 0.5);
+    while (r > 0 && (2 * r - 1) * (2 * r - 1) > 4 * D) --r;
+    while ((2 * r + 1) * (2 * r + 1) <= 4 * D) ++r;
+    return (int32_t)r;
+}
+
+__global__ __launch_bounds__(256) void k_dist_from_xy(const int32_t* __restrict__ xy, int32_t n, int32_t P,
+                                                      int32_t* __restrict__ dist) {
+    const int32_t i = blockIdx.y;
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= P) return;
+    int32_t d = ACO_MAX_DISTANCE;
+    if (j < n && j != i)
+        d = euc2d_exact((int64_t)xy[2 * i] - xy[2 * j], (int64_t)xy[2 * i + 1] - xy[2 * j + 1]);
+    dist[(size_t)i * P + j] = d;
+}
+
+__global__ __launch_bounds__(256) void k_fix_diagonal(int32_t* __restrict__ dist, int32_t n, int32_t P) {
+    const int32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dist[(size_t)i * P + i] = ACO_MAX_DISTANCE;
+}
+
+__global__ __launch_bounds__(256) void k_dist_max(const int32_t* __restrict__ dist, int32_t n, int32_t P,
+                                                  int32_t* __restrict__ out) {
+    const int32_t i = blockIdx.y;
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    int32_t v = 0;
+    if (j < n && j != i) {
+        const int32_t d = dist[(size_t)i * P + j];
+        if (d != ACO_MAX_DISTANCE) v = d;
+    }
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, v);
+}
+
+// ---------------------------------------------------------------------------
+// Nearest-neighbour lists: one wave per row, k rounds of a wave-wide minimum
+// over the (d, j) keys strictly above the previous one -> the stable (d, j)
+// order of dev_compute_nnlist (inc/Colony.cuh:101-123).  k < n excludes the
+// city itself; k == n is the full row (self last, d = MAX_DISTANCE).
+__global__ __launch_bounds__(256) void k_nn_lists(const int32_t* __restrict__ dist, int32_t n, int32_t P,
+                                                  int32_t k, int32_t* __restrict__ nn) {
+    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const int32_t* row = dist + (size_t)i * P;
+    const bool include_self = (k == n);
+    uint64_t last = 0;
+    for (int32_t r = 0; r < k; ++r) {
+        uint64_t best = ~0ull;
+        for (int32_t j = lane; j < n; j += 64) {
+            if (j == i && !include_self) continue;
+            const uint64_t key = ((uint64_t)(uint32_t)row[j] << 32) | (uint32_t)j;
+            if ((r == 0 || key > last) && key < best) best = key;
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const uint32_t lo = __shfl_xor((uint32_t)best, off);
+            const uint32_t hi = __shfl_xor((uint32_t)(best >> 32), off);
+            const uint64_t o = ((uint64_t)hi << 32) | lo;
+            best = o < best ? o : best;
+        }
+        last = best;
+        if (lane == 0) nn[(size_t)i * k + r] = (int32_t)(uint32_t)best;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_rows(float* __restrict__ a, int32_t n, int32_t P, float v) {
+    const int32_t i = blockIdx.y;
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j < P) a[(size_t)i * P + j] = (j < n) ? v : 0.0f;
+}
+
+// Nearest-neighbour tour from `start` (ties to the lowest index): C_nn of the
+// initial-pheromone formulas (Colony::greedyDistance, a stub in the
+// reference, src/Colony.cu:1053-1055).  One block; the visited bitmap in LDS.
+__global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dist, int32_t n, int32_t P,
+                                                 int32_t start, int64_t* __restrict__ out) {
+    extern __shared__ uint32_t sh[];
+    uint32_t* vis = sh;
+    const int32_t words = (n + 31) / 32;
+    uint64_t* red = reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1));
+    __shared__ int32_t s_next;
+    for (int32_t w = threadIdx.x; w < words; w += blockDim.x) vis[w] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) vis[start >> 5] |= 1u << (start & 31);
+    __syncthreads();
+    int64_t len = 0;
+    int32_t c = start;
+    for (int32_t s = 1; s < n; ++s) {
+        uint64_t best = ~0ull;
+        for (int32_t j = threadIdx.x; j < n; j += blockDim.x) {
+            if ((vis[j >> 5] >> (j & 31)) & 1u) continue;
+            const uint64_t key = ((uint64_t)(uint32_t)dist[(size_t)c * P + j] << 32) | (uint32_t)j;
+            best = key < best ? key : best;
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const uint32_t lo = __shfl_xor((uint32_t)best, off);
+            const uint32_t hi = __shfl_xor((uint32_t)(best >> 32), off);
+            const uint64_t o = ((uint64_t)hi << 32) | lo;
+            best = o < best ? o : best;
+        }
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t b = red[0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = red[w] < b ? red[w] : b;
+            s_next = (int32_t)(uint32_t)b;
+            len += (int64_t)(b >> 32);
+            vis[s_next >> 5] |= 1u << (s_next & 31);
+        }
+        __syncthreads();
+        c = s_next;
+    }
+    if (threadIdx.x == 0) *out = len + dist[(size_t)c * P + start];
+}
+
+// ---------------------------------------------------------------------------
+// Fused dense sweep, one wave per row (4 rows per 256-thread block), 16-byte
+// coalesced loads and stores.  Per element:
+//   choice   = tau^alpha * eta(d)          (dev_compute_selection, inc/Colony.cuh:91-99)
+//   tau_next = max((1-rho) tau, 1e-18)     (dev_evaporate_pheromones, src/AntSystem.cu:26-32)
+// and per row the mean of tau (dev_compute_pheromone_mean, inc/Colony.cuh:164-190),
+// lane l summing columns 256s + 4l + q in order, lanes combined by the DPP scan.
+// Algorithmic traffic: 16 bytes per element (read tau, dist; write choice, tau_next).
+__global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.n) return;
+    const size_t row = (size_t)i * a.P;
+    const float* __restrict__ trow = a.tau + row;
+    const int32_t* __restrict__ drow = a.dist + row;
+    float acc = 0.0f;
+    for (int32_t j0 = 4 * lane; j0 < a.n; j0 += 256) {
+        const float4 t = *reinterpret_cast<const float4*>(trow + j0);
+        const int4 d = *reinterpret_cast<const int4*>(drow + j0);
+        const float tv[4] = {t.x, t.y, t.z, t.w};
+        const int32_t dv[4] = {d.x, d.y, d.z, d.w};
+        float cv[4], ev[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool in = (j0 + q) < a.n;
+            const float x = in ? tv[q] : 0.0f;
+            acc = acc + x;
+            cv[q] = in ? apply_alpha(x, a.alpha) * eta_of(a.eta, a.eta_len, dv[q]) : 0.0f;
+            ev[q] = in ? fmaxf(a.one_minus_rho * x, ACO_MIN_PHEROMONE) : 0.0f;
+        }
+        *reinterpret_cast<float4*>(a.choice + row + j0) = make_float4(cv[0], cv[1], cv[2], cv[3]);
+        *reinterpret_cast<float4*>(a.tau_next + row + j0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
+    }
+    if (a.want_mean) {
+        const float total = readlane_f(wave_inclusive_scan(acc), 63);
+        if (lane == 0) a.mean[i] = total / (float)a.n;
+    }
+    if (a.nn != nullptr) {
+        for (int32_t r = lane; r < a.k; r += 64) {
+            const int32_t j = a.nn[(size_t)i * a.k + r];
+            a.choice_nn[(size_t)i * a.k + r] = apply_alpha(trow[j], a.alpha) * eta_of(a.eta, a.eta_len, drow[j]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Construction helpers.
+struct AntCtx {
+    bool young;
+    float ratio;
+};
+
+// compute_fittness (inc/Colony.cuh:14-31) for aging ants younger than
+// min_age after convergence: tau pulled toward the row mean.
+__device__ __forceinline__ float young_weight(const ConstructArgs& a, int32_t c, int32_t j, float ratio) {
+    const size_t xy = (size_t)c * a.P + j;
+    const float mc = a.mean[c];
+    const float p = ((a.tau[xy] - mc) * ratio) + mc;
+    return apply_alpha(p, a.alpha) * eta_of(a.eta, a.eta_len, a.dist[xy]);
+}
+
+__device__ __forceinline__ bool is_visited(const uint32_t* vis, int32_t j) {
+    return (vis[j >> 5] >> (j & 31)) & 1u;
+}
+
+// Tour staging: lane (s & 63) keeps the city of step s; a full chunk of 64
+// steps leaves as one coalesced store.
+__device__ __forceinline__ void stage_city(int32_t* tour, int32_t& tbuf, int32_t s, int32_t city, int32_t n,
+                                           int lane) {
+    if (lane == (s & 63)) tbuf = city;
+    if ((s & 63) == 63 || s == n - 1) {
+        const int32_t base = s & ~63;
+        if (lane <= (s & 63)) tour[base + lane] = tbuf;
+    }
+}
+
+// choose_best_next: argmax of the selection weight over every unvisited city,
+// ties to the lowest index.  Lanes stride the row; exact comparisons make the
+// result independent of the reduction order.
+__device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const uint32_t* vis, int32_t c,
+                                                   const AntCtx& ant, int lane, unsigned long long& scanned) {
+    float best = -1.0f;
+    int32_t bj = 0x7fffffff;
+    const float* crow = a.choice + (size_t)c * a.P;
+    for (int32_t j = lane; j < a.n; j += 64) {
+        if (is_visited(vis, j)) continue;
+        ++scanned;
+        const float x = ant.young ? young_weight(a, c, j, ant.ratio) : crow[j];
+        if (x > best) {
+            best = x;
+            bj = j;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float ob = __shfl_xor(best, off);
+        const int32_t oj = __shfl_xor(bj, off);
+        if (ob > best || (ob == best && oj < bj)) {
+            best = ob;
+            bj = oj;
+        }
+    }
+    int32_t next = uniform_i(bj);
+    if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
+        for (int32_t j0 = 0; j0 < a.n; j0 += 64) {
+            const int32_t j = j0 + lane;
+            const unsigned long long m = __ballot(j < a.n && !is_visited(vis, j));
+            if (m) {
+                next = j0 + __ffsll((long long)m) - 1;
+                break;
+            }
+        }
+    }
+    return next;
+}
+
+// ---------------------------------------------------------------------------
+// Tour construction, candidate-list mode.  One ant per wave64 for all n-1
+// steps (the reference launches 3(n-1) kernels per iteration,
+// src/Colony.cu:713-755).  Per step: lane r < k holds candidate r of the
+// current city, its weight (0 if visited, per the LDS bitmap), the DPP
+// inclusive scan gives the roulette prefix, a ballot picks the first lane
+// whose prefix exceeds u * total.  Zero total -> argmax fallback over the
+// unvisited cities.  Tour length accumulated exactly in int64.
+__global__ __launch_bounds__(256) void k_construct_cand(const ConstructArgs a) {
+    extern __shared__ uint32_t lds[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int32_t ant = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (ant >= a.m) return;
+    uint32_t* vis = lds + (size_t)wave * a.words;
+    const uint32_t iter = a.state->iter;
+
+    AntCtx ac{false, 0.0f};
+    if (a.aging) {
+        const int32_t age = a.age[ant] + 1;  // updateAntAges, every ant
+        if (lane == 0) a.age[ant] = age;
+        if (!a.valid[ant]) return;
+        ac.young = a.state->converged && age < a.min_age;
+        ac.ratio = ac.young ? (float)age / (float)a.min_age : 0.0f;
+    }
+    for (int32_t w = lane; w < a.words; w += 64) vis[w] = 0u;
+    wave_lds_sync();
+
+    Philox4 rb = philox4x32_10(0u, (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
+    const int32_t start = word_to_city(rb.x0, a.n);
+    if (lane == 0) vis[start >> 5] |= 1u << (start & 31);
+    wave_lds_sync();
+
+    int32_t* tour = a.tours + (size_t)ant * a.n;
+    int32_t tbuf = start;
+    int64_t len = 0;
+    int32_t c = start;
+    unsigned long long fb = 0, scanned = 0;
+    for (int32_t s = 1; s < a.n; ++s) {
+        if ((s & 3) == 0) rb = philox4x32_10((uint32_t)(s >> 2), (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
+        const float u = word_to_uniform(philox_word(rb, (uint32_t)(s & 3)));
+        int32_t j = 0;
+        float w = 0.0f;
+        if (lane < a.k) {
+            j = a.nn[(size_t)c * a.k + lane];
+            const float cw = ac.young ? young_weight(a, c, j, ac.ratio) : a.choice_nn[(size_t)c * a.k + lane];
+            w = is_visited(vis, j) ? 0.0f : cw;
+        }
+        const float p = wave_inclusive_scan(w);
+        const float total = readlane_f(p, 63);
+        int32_t next;
+        if (total > 0.0f) {
+            const float rr = u * total;
+            const unsigned long long hit = __ballot(p > rr && w > 0.0f);
+            int sel;
+            if (hit) {
+                sel = __ffsll((long long)hit) - 1;
+            } else {
+                const unsigned long long pos = __ballot(w > 0.0f);
+                sel = 63 - __clzll((long long)pos);
+            }
+            next = readlane_i(j, sel);
+        } else {
+            ++fb;
+            next = fallback_argmax(a, vis, c, ac, lane, scanned);
+        }
+        if (lane == 0) {
+            vis[next >> 5] |= 1u << (next & 31);
+            len += a.dist[(size_t)c * a.P + next];
+        }
+        wave_lds_sync();
+        stage_city(tour, tbuf, s, next, a.n, lane);
+        c = next;
+    }
+    for (int off = 32; off >= 1; off >>= 1) scanned += __shfl_xor(scanned, off);
+    if (lane == 0) {
+        if (a.n == 1) tour[0] = start;
+        len += a.dist[(size_t)c * a.P + start];
+        a.lengths[ant] = len;
+        if (fb) atomicAdd(&a.state->fb_steps, fb);
+        if (scanned) atomicAdd(&a.state->fb_scanned, scanned);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tour construction, full-row mode: the reference's candidate set
+// (calculateProbabilityOfSelection src/Colony.cu:795-818: every city in the
+// row's NN order, visited ones weighted 0).  The row is processed in chunks
+// of 64 lanes; chunk prefixes come from the DPP scan and are offset by the
+// running chunk total T (sequential over chunks).  Fallback: the farthest
+// unvisited city (select_local_best inc/Colony.cuh:33-43).  LDS per ant:
+// bitmap, weights and prefixes of the whole row, chunk offsets.
+__global__ __launch_bounds__(256) void k_construct_full(const ConstructArgs a) {
+    extern __shared__ uint32_t lds[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int32_t ant = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (ant >= a.m) return;
+    const int32_t nchunk = (a.n + 63) / 64;
+    const size_t per_ant = (size_t)a.words + 2 * (size_t)nchunk * 64 + nchunk;
+    uint32_t* vis = lds + (size_t)wave * per_ant;
+    float* wbuf = reinterpret_cast<float*>(vis + a.words);
+    float* pbuf = wbuf + (size_t)nchunk * 64;
+    float* tp = pbuf + (size_t)nchunk * 64;
+    const uint32_t iter = a.state->iter;
+
+    AntCtx ac{false, 0.0f};
+    if (a.aging) {
+        const int32_t age = a.age[ant] + 1;
+        if (lane == 0) a.age[ant] = age;
+        if (!a.valid[ant]) return;
+        ac.young = a.state->converged && age < a.min_age;
+        ac.ratio = ac.young ? (float)age / (float)a.min_age : 0.0f;
+    }
+    for (int32_t w = lane; w < a.words; w += 64) vis[w] = 0u;
+    wave_lds_sync();
+    Philox4 rb = philox4x32_10(0u, (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
+    const int32_t start = word_to_city(rb.x0, a.n);
+    if (lane == 0) vis[start >> 5] |= 1u << (start & 31);
+    wave_lds_sync();
+
+    int32_t* tour = a.tours + (size_t)ant * a.n;
+    int32_t tbuf = start;
+    int64_t len = 0;
+    int32_t c = start;
+    unsigned long long fb = 0;
+    for (int32_t s = 1; s < a.n; ++s) {
+        if ((s & 3) == 0) rb = philox4x32_10((uint32_t)(s >> 2), (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
+        const float u = word_to_uniform(philox_word(rb, (uint32_t)(s & 3)));
+        const int32_t* nrow = a.nn + (size_t)c * a.n;
+        const float* crow = a.choice + (size_t)c * a.P;
+        float T = 0.0f;
+        for (int32_t q = 0; q < nchunk; ++q) {
+            const int32_t r = q * 64 + lane;
+            float w = 0.0f;
+            if (r < a.n) {
+                const int32_t j = nrow[r];
+                if (!is_visited(vis, j)) w = ac.young ? young_weight(a, c, j, ac.ratio) : crow[j];
+            }
+            const float p = wave_inclusive_scan(w);
+            wbuf[r] = w;
+            pbuf[r] = p;
+            if (lane == 0) tp[q] = T;
+            T = T + readlane_f(p, 63);
+        }
+        wave_lds_sync();
+        int32_t next = -1;
+        if (T > 0.0f) {
+            const float rr = u * T;
+            for (int32_t q = 0; q < nchunk; ++q) {
+                const int32_t r = q * 64 + lane;
+                const float P = tp[q] + pbuf[r];
+                const unsigned long long hit = __ballot(P > rr && wbuf[r] > 0.0f);
+                if (hit) {
+                    next = nrow[q * 64 + __ffsll((long long)hit) - 1];
+                    break;
+                }
+            }
+        }
+        if (next < 0) {
+            ++fb;
+            for (int32_t q = nchunk - 1; q >= 0; --q) {
+                const int32_t r = q * 64 + lane;
+                const unsigned long long m = __ballot(r < a.n && !is_visited(vis, nrow[r < a.n ? r : 0]));
+                if (m) {
+                    next = nrow[q * 64 + 63 - __clzll((long long)m)];
+                    break;
+                }
+            }
+        }
+        next = uniform_i(next);
+        if (lane == 0) {
+            vis[next >> 5] |= 1u << (next & 31);
+            len += a.dist[(size_t)c * a.P + next];
+        }
+        wave_lds_sync();
+        stage_city(tour, tbuf, s, next, a.n, lane);
+        c = next;
+    }
+    if (lane == 0) {
+        if (a.n == 1) tour[0] = start;
+        len += a.dist[(size_t)c * a.P + start];
+        a.lengths[ant] = len;
+        if (fb) atomicAdd(&a.state->fb_steps, fb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Iteration bookkeeping in one 1024-thread block (computeAntDistances
+// src/Colony.cu:869-1047, CPU semantics): aging performance counters,
+// iteration best (first strict minimum), best-so-far update and tour copy,
+// deposit list (AS: every valid ant, dense by ant index; RBAS: the w-1 best
+// ranked ants plus the best-so-far tour), then aging retire/recruit
+// (src/Colony.cu:366-423).
+struct LenKey {
+    int64_t len;
+    int32_t ant;
+};
+
+__device__ __forceinline__ bool key_less(const LenKey& x, const LenKey& y) {
+    return x.len < y.len || (x.len == y.len && x.ant < y.ant);
+}
+
+__device__ LenKey block_min_key(LenKey v, LenKey* red) {
+    for (int off = 32; off >= 1; off >>= 1) {
+        LenKey o;
+        const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)v.len, off);
+        const uint32_t hi = __shfl_xor((uint32_t)((uint64_t)v.len >> 32), off);
+        o.len = (int64_t)(((uint64_t)hi << 32) | lo);
+        o.ant = __shfl_xor(v.ant, off);
+        if (key_less(o, v)) v = o;
+    }
+    const int nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    LenKey b = red[0];
+    for (int w = 1; w < nw; ++w)
+        if (key_less(red[w], b)) b = red[w];
+    return b;
+}
+
+__device__ int32_t block_min_i32(int32_t v, int32_t* red) {
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int32_t b = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = min(b, red[w]);
+    return b;
+}
+
+__global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
+    __shared__ LenKey red[16];
+    __shared__ int32_t redi[16];
+    __shared__ int32_t s_improved;
+    extern __shared__ uint32_t taken[];  // RBAS: ants already ranked
+    DevState* st = a.state;
+    const LenKey none{(int64_t)ACO_MAX_DISTANCE - 1, 0x7fffffff};
+
+    LenKey mine = none;
+    for (int32_t k = threadIdx.x; k < a.m; k += blockDim.x) {
+        if (a.aging && !a.valid[k]) continue;
+        const int64_t L = a.lengths[k];
+        if (a.aging) {
+            a.perf[k] = (L == a.prev_len[k]) ? a.perf[k] + 1 : 0;
+            a.prev_len[k] = L;
+        }
+        const LenKey x{L, k};
+        if (key_less(x, mine)) mine = x;
+    }
+    const LenKey ib = block_min_key(mine, red);
+    if (threadIdx.x == 0) {
+        // iterBestDist starts at MAX_DISTANCE - 1 and only a strictly smaller
+        // length is taken (src/Colony.cu:872,1015).
+        const bool found = ib.len < (int64_t)ACO_MAX_DISTANCE - 1;
+        st->iter_best = found ? ib.len : (int64_t)ACO_MAX_DISTANCE - 1;
+        st->iter_best_ant = found ? ib.ant : -1;
+        const bool improved = found && ib.len < st->glob_best;
+        if (improved) {
+            st->glob_best = ib.len;
+            st->gperf = 0;
+        } else {
+            st->gperf += 1;
+        }
+        s_improved = improved ? ib.ant : -1;
+    }
+    __syncthreads();
+    const int32_t imp = s_improved;
+    if (imp >= 0)
+        for (int32_t i = threadIdx.x; i < a.n; i += blockDim.x) a.best_tour[i] = a.tours[(size_t)imp * a.n + i];
+
+    if (a.variant == 0) {  // AS: dense, one entry per ant
+        for (int32_t k = threadIdx.x; k < a.m; k += blockDim.x) {
+            const bool v = !a.aging || a.valid[k];
+            a.dep_val[k] = v ? (float)(1.0 / (double)a.lengths[k]) : 0.0f;
+        }
+        if (threadIdx.x == 0) st->ndep = a.m;
+    } else {  // RBAS
+        const int32_t words = (a.m + 31) / 32;
+        for (int32_t w = threadIdx.x; w < words; w += blockDim.x) taken[w] = 0u;
+        __syncthreads();
+        int32_t nd = 0;
+        for (int32_t r = 1; r <= a.w - 1; ++r) {
+            LenKey mk = none;
+            for (int32_t k = threadIdx.x; k < a.m; k += blockDim.x) {
+                if (a.aging && !a.valid[k]) continue;
+                if ((taken[k >> 5] >> (k & 31)) & 1u) continue;
+                const LenKey x{a.lengths[k], k};
+                if (key_less(x, mk)) mk = x;
+            }
+            // `none` has len MAX-1: a valid ant of that length cannot exist.
+            const LenKey b = block_min_key(mk, red);
+            if (b.ant == 0x7fffffff) break;
+            if (threadIdx.x == 0) {
+                taken[b.ant >> 5] |= 1u << (b.ant & 31);
+                a.dep_ant[nd] = b.ant;
+                a.dep_val[nd] = (float)((double)(a.w - r) / (double)b.len);
+            }
+            ++nd;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (st->glob_best < (int64_t)ACO_MAX_DISTANCE) {
+                a.dep_ant[nd] = -1;
+                a.dep_val[nd] = (float)((double)a.w / (double)st->glob_best);
+                ++nd;
+            }
+            st->ndep = nd;
+        }
+    }
+
+    if (a.aging) {
+        // retireAnts: first valid ant among the first m/2 at max_age.
+        int32_t cand = 0x7fffffff;
+        for (int32_t k = threadIdx.x; k < a.m / 2; k += blockDim.x)
+            if (a.valid[k] && a.age[k] >= a.max_age) cand = min(cand, k);
+        const int32_t ret = block_min_i32(cand, redi);
+        if (threadIdx.x == 0 && ret != 0x7fffffff) {
+            a.valid[ret] = 0;
+            st->num_valid -= 1;
+            st->converged = 1;
+        }
+        // recruiteAnts: first invalid ant, when the colony stagnates or is small.
+        __shared__ int32_t s_recruit;
+        if (threadIdx.x == 0) s_recruit = (st->gperf >= a.global_perf || st->num_valid < a.min_pop);
+        __syncthreads();
+        if (s_recruit) {
+            int32_t inv = 0x7fffffff;
+            for (int32_t k = threadIdx.x; k < a.m; k += blockDim.x)
+                if (!a.valid[k]) inv = min(inv, k);
+            const int32_t rc = block_min_i32(inv, redi);
+            if (threadIdx.x == 0 && rc != 0x7fffffff) {
+                a.valid[rc] = 1;
+                a.age[rc] = 0;
+                st->num_valid += 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) st->iter += 1;
+}
+
+// ---------------------------------------------------------------------------
+// Deposit: one block per list entry, fp32 atomics on both (i,j) and (j,i) of
+// every tour edge (dev_update_pheromones src/AntSystem.cu:105-128, which only
+// updated one direction; the CPU branch src/AntSystem.cu:296-316 is symmetric).
+__global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
+    const int32_t e = blockIdx.x;
+    if (e >= a.state->ndep) return;
+    const int32_t ant = a.dense ? e : a.dep_ant[e];
+    const float d = a.dep_val[e];
+    if (d == 0.0f) return;
+    const int32_t* t = ant >= 0 ? a.tours + (size_t)ant * a.n : a.best_tour;
+    for (int32_t i = threadIdx.x; i < a.n; i += blockDim.x) {
+        const int32_t x = t[i];
+        const int32_t y = t[(i + 1 == a.n) ? 0 : i + 1];
+        unsafeAtomicAdd(a.tau_next + (size_t)x * a.P + y, d);
+        unsafeAtomicAdd(a.tau_next + (size_t)y * a.P + x, d);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s) {
+    hipLaunchKernelGGL(k_dist_from_xy, dim3(cdiv(P, 256), n), dim3(256), 0, s, xy, n, P, dist);
+    return hipGetLastError();
+}
+
+hipError_t launch_fix_diagonal(int32_t* dist, int32_t n, int32_t P, hipStream_t s) {
+    hipLaunchKernelGGL(k_fix_diagonal, dim3(cdiv(n, 256)), dim3(256), 0, s, dist, n, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_dist_max(const int32_t* dist, int32_t n, int32_t P, int32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_dist_max, dim3(cdiv(n, 256), n), dim3(256), 0, s, dist, n, P, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_nn_lists(const int32_t* dist, int32_t n, int32_t P, int32_t k, int32_t* nn, hipStream_t s) {
+    hipLaunchKernelGGL(k_nn_lists, dim3(cdiv(n, 4)), dim3(256), 0, s, dist, n, P, k, nn);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_rows(float* a, int32_t n, int32_t P, float v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_rows, dim3(cdiv(P, 256), n), dim3(256), 0, s, a, n, P, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, hipStream_t s) {
+    const int32_t words = (n + 31) / 32;
+    const size_t lds = (size_t)((words + 1) & ~1) * 4 + 16 * sizeof(uint64_t);
+    hipLaunchKernelGGL(k_greedy, dim3(1), dim3(1024), lds, s, dist, n, P, start, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_sweep, dim3(cdiv(a.n, 4)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_construct(const ConstructArgs& a, int32_t mode, int32_t ants_per_block, hipStream_t s) {
+    const int W = ants_per_block;
+    if (mode == 0) {
+        const size_t lds = (size_t)W * a.words * sizeof(uint32_t);
+        hipLaunchKernelGGL(k_construct_cand, dim3(cdiv(a.m, W)), dim3(64 * W), lds, s, a);
+    } else {
+        const int32_t nchunk = (a.n + 63) / 64;
+        const size_t per_ant = (size_t)a.words + 2 * (size_t)nchunk * 64 + nchunk;
+        hipLaunchKernelGGL(k_construct_full, dim3(cdiv(a.m, W)), dim3(64 * W), (size_t)W * per_ant * 4, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stats(const StatsArgs& a, hipStream_t s) {
+    const size_t lds = (size_t)((a.m + 31) / 32) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_stats, dim3(1), dim3(1024), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const float4* src, float4* dst, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, src, dst, count);
+    return hipGetLastError();
+}
+
+}  // namespace aco

@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark: tours/s of the ACO hot path on one MI355X.
+
+Workload (default): rank-based Ant System with ant aging, n = 1000 cities
+from 10 Gaussian clusters (centres uniform in [0,10000]^2, sigma 400, clipped,
+rounded), EUC_2D integer distances, m = 1000 ants, candidate lists k = 20,
+w = 6 ranked ants, alpha 1, beta 3, rho 0.1, fp32 pheromone / choice_info.
+One step = one ACO iteration = m tours (sweep, construction, statistics,
+deposit), all on the device; the timed loop replays a hipGraph of the
+iteration with the inputs resident in HBM.
+
+Prints one JSON line.  `--gpus N` with N > 1 is refused: this build measures
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from acogpu_amd import capi, instances  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def c2_params(n=1000, m=1000):
+    return capi.default_params(
+        variant=capi.VARIANT_RBAS, mode=capi.MODE_CANDIDATE, n=n, m=m, k=20, w=6,
+        alpha=1.0, beta=3.0, rho=0.1, tau0_policy=capi.TAU0_RBAS, aging=1,
+        min_age=30, max_age=50, local_perf=10, global_perf=10, min_pop=40, max_pop=100,
+        seed=20241203)
+
+
+def construct_bytes_per_launch(n, k, tours, scanned):
+    """Algorithmic bytes of one construction launch: per tour n steps of
+    k candidate indices + k weights (8k B) and one tour write (4 B), one
+    distance read per edge (4 B), plus 4 B per unvisited city scanned by
+    fallback steps."""
+    return tours * (n * (8 * k + 4) + 4 * n) + 4 * scanned
+
+
+def cpu_baseline(xy, iterations):
+    """The oracle's reference-semantics restatement (every city in NN order,
+    sequential fp32 sums, farthest-unvisited fallback, aging, rank-based
+    deposit), one thread, on the same instance."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    dist = orc.dist_from_int_coords(xy)
+    n = dist.shape[0]
+    cnn = orc.greedy_length(dist, 0)
+    tau0 = float(np.float32(0.5 * 6 * 5 / (np.float64(np.float32(0.1)) * cnn)))
+    secs, gb, ib, _ = orc.run(dist, 1000, variant=orc.RBAS, mode=orc.MODE_REFSEQ, k=20, w=6, aging=True,
+                              min_age=30, max_age=50, global_perf=10, min_pop=40,
+                              iterations=iterations, alpha=1.0, beta=3.0, rho=0.1, tau0=tau0, seed=7)
+    return 1000 * iterations / secs, secs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ants-per-block", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(instances and ants shard without exchange; see the sharding notes)", file=sys.stderr)
+        return 2
+
+    n, m, k = 1000, 1000, 20
+    xy = instances.clustered(n, clusters=10, side=10000, sigma=400.0, seed=1)
+    p = c2_params(n, m)
+    p.ants_per_block = args.ants_per_block
+    log(f"bench: creating colony n={n} m={m} k={k} RBAS w=6 aging")
+    col = capi.Colony(p, xy=xy)
+    col.initialize()
+    log(f"bench: warmup {args.warmup} iterations")
+    col.run(args.warmup)
+    log(f"bench: timed {args.steps} iterations (hipGraph replay)")
+    t0 = time.perf_counter()
+    ev_ms = col.run(args.steps)
+    wall = time.perf_counter() - t0
+    ib, gb, nv = col.stats()
+    tours_per_s = m * args.steps / wall
+    log(f"bench: {wall * 1e3 / args.steps:.3f} ms/iteration (events {ev_ms / args.steps:.3f}), "
+        f"{tours_per_s:.4g} tours/s, best {gb}")
+
+    # Per-kernel durations with HIP events on the ctx stream.
+    fb0 = col.counters()
+    kms = col.profile(args.profile_iters)
+    fb1 = col.counters()
+    scanned_per_launch = (fb1[1] - fb0[1]) / args.profile_iters
+    nvalid = col.stats()[2]
+    bytes_launch = construct_bytes_per_launch(n, k, nvalid, scanned_per_launch)
+    achieved = bytes_launch / (kms[1] * 1e-3) / 1e9
+    log(f"bench: kernel ms sweep {kms[0]:.4f} construct {kms[1]:.4f} stats {kms[2]:.4f} "
+        f"deposit {kms[3]:.4f}; fallback cities scanned/launch {scanned_per_launch:.0f}")
+
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "construct_c2_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"bench: CPU baseline, {args.cpu_iters} iterations x {m} ants, reference semantics, 1 thread")
+        rate, secs = cpu_baseline(xy, args.cpu_iters)
+        cpu = {"value": rate, "unit": "tours/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iters} iterations x {m} ants of the same n=1000 instance, "
+                         f"reference full-row semantics (oracle refseq mode), {secs:.1f} s"}
+        log(f"bench: CPU {rate:.4g} tours/s ({secs:.1f} s)")
+
+    line = {
+        "metric": "tours/sec at n=1000",
+        "value": tours_per_s,
+        "unit": "tours/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "rank-based AS + aging, n=1000 clustered (10 Gaussian clusters, "
+                               "sigma 400), m=1000, k=20, w=6, alpha 1, beta 3, rho 0.1",
+                   "n": n, "m": m, "k": k, "iterations_timed": args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": "k_construct_cand", "kernel_ms": float(kms[1]),
+                     "bytes_per_launch": bytes_launch},
+        "kernel_ms": {"sweep": float(kms[0]), "construct": float(kms[1]), "stats": float(kms[2]),
+                      "deposit": float(kms[3])},
+        "cpu_baseline": cpu,
+        "best_length": gb,
+    }
+    print(json.dumps(line), flush=True)
+    col.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,158 @@
+/*
+ * acogpu.h -- C ABI of the MI355X-native Ant Colony Optimization solver.
+ *
+ * The reference (observit17/acogpu) has no C or FFI interface: its boundary
+ * is the C++ class Colony (inc/Colony.h:24-155) with subclasses AntSystem
+ * (inc/AntSystem.h:7-23) and RankBasedAntSystem (inc/RankBasedAntSystem.h:
+ * 12-28), called by main (src/Setup.cpp:73-129) and TSP::execute
+ * (src/TSP.cpp:23-79).  Each entry point below names the member function it
+ * replaces.  The C++ classes of acogpu_amd/csrc/colony/ are thin wrappers over
+ * this ABI; any other host language binds it directly (ctypes, cgo, JNI).
+ *
+ * Conventions: every call returns 0 on success or a negative ACO_E* code and
+ * never exits the process; aco_last_error() holds the message.  No process
+ * globals: one aco_ctx per colony, calls on one ctx are not thread-safe,
+ * calls on different ctxs are independent.  Host arrays are plain row-major
+ * C arrays; device memory never crosses the ABI except the optional device
+ * distance matrix accepted by aco_create (the reference's GPU convention,
+ * src/Setup.cpp:60-66).
+ */
+#ifndef ACOGPU_H
+#define ACOGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACO_OK 0
+#define ACO_E_ARG (-1)      /* invalid argument / parameter combination */
+#define ACO_E_HIP (-2)      /* HIP runtime error (message has the HIP string) */
+#define ACO_E_NOMEM (-3)    /* device allocation failed */
+#define ACO_E_STATE (-4)    /* call out of order (e.g. iterate before initialize) */
+#define ACO_E_RANGE (-5)    /* input outside the supported range */
+
+#define ACO_VARIANT_AS 0    /* AntSystem (src/AntSystem.cu)                   */
+#define ACO_VARIANT_RBAS 1  /* RankBasedAntSystem (src/RankBasedAntSystem.cu) */
+
+#define ACO_MODE_CANDIDATE 0 /* k-nearest candidate lists + argmax fallback   */
+#define ACO_MODE_FULLROW 1   /* every city in NN order, as the reference does */
+
+#define ACO_TAU0_CONST 0    /* tau0 field (reference: 1e-6, inc/default.h:16) */
+#define ACO_TAU0_AS 1       /* m / C_nn                                       */
+#define ACO_TAU0_RBAS 2     /* 0.5 w (w-1) / (rho C_nn) (src/RankBasedAntSystem.cu:44) */
+
+/* timers[] slots, same order as enum StatsTimer (inc/Statistics.h:7-15) */
+#define ACO_TIME_INIT_TOUR 0
+#define ACO_TIME_TOUR 1
+#define ACO_TIME_COMPUTE_ANT_DISTANCE 2
+#define ACO_TIME_EVAPORATE_PHEROMONE 3
+#define ACO_TIME_UPDATE_PHEROMONE 4
+#define ACO_TIME_COMPUTE_PROBABILITIES 5
+#define ACO_TOTAL_TIMERS 6
+
+typedef struct aco_ctx aco_ctx;
+
+/* Colony + AntSystem + aging constructor arguments (src/Colony.cu:25-137,
+ * src/AntSystem.cu:160-181, inc/Colony.h:118-124) and CLI flags
+ * (src/Commandline.cpp:98-231). */
+typedef struct aco_params {
+    int32_t variant;        /* ACO_VARIANT_*                                  */
+    int32_t mode;           /* ACO_MODE_*                                     */
+    int32_t n;              /* cities                                         */
+    int32_t m;              /* ants (-m)                                      */
+    int32_t k;              /* candidate list length, 1..64 (candidate mode)  */
+    int32_t w;              /* RBAS ranked ants (RankBasedAntSystem::setW)    */
+    float alpha;            /* -a                                             */
+    float beta;             /* -b                                             */
+    float rho;              /* -r                                             */
+    int32_t tau0_policy;    /* ACO_TAU0_*                                     */
+    float tau0;             /* used by ACO_TAU0_CONST                         */
+    int32_t aging;          /* 0/1: the "aging" build variant                 */
+    int32_t min_age;        /* -j minRetirementAge                            */
+    int32_t max_age;        /* -k maxRetirementAge                            */
+    int32_t local_perf;     /* -l localPeformance (unused by the reference)   */
+    int32_t global_perf;    /* -g globalPerformance                           */
+    int32_t min_pop;        /* -x minPopulationLimit                          */
+    int32_t max_pop;        /* -y maxPopulationLimit                          */
+    int32_t broker;         /* "broker" build variant: accepted, no behaviour */
+    int32_t ants_per_block; /* construction launch shape, 0 = autotuned       */
+    uint64_t seed;          /* Philox key (reference: RANDOM_SEED, inc/Colony.h:15) */
+    uint32_t stream_id;     /* Philox counter word 3 (instance id)            */
+    int32_t reserved[7];
+} aco_params;
+
+/* Reference defaults (inc/default.h:8-39, src/Colony.cu:32-36). */
+int aco_params_init(aco_params *p);
+
+/* Colony::Colony + AntSystem::AntSystem / RankBasedAntSystem::
+ * RankBasedAntSystem.  Exactly one of dist (n*n int32, host or device
+ * pointer; diagonal ignored) or xy (2n int32 host coordinates, interleaved,
+ * |coordinate| < 2^29) is non-NULL.  Allocates every device buffer once. */
+int aco_create(const aco_params *p, const int32_t *dist, const int32_t *xy, aco_ctx **out);
+
+/* Colony::initialize (src/Colony.cu:185-214): pheromone fill, heuristic
+ * table, NN lists, aging arrays, best-so-far reset. */
+int aco_initialize(aco_ctx *c);
+
+/* Colony::constructSolution (src/Colony.cu:537-594): one iteration, phase
+ * times in milliseconds (timers may be NULL). */
+int aco_iterate(aco_ctx *c, float *timers);
+
+/* TSP::execute's loop body for `iterations` iterations with no host
+ * synchronisation inside (hipGraph replay).  elapsed_ms (may be NULL) gets the
+ * HIP-event time of the whole loop. */
+int aco_run(aco_ctx *c, int32_t iterations, float *elapsed_ms);
+
+/* getIterBestDist / getGlobBestDist / getNumValidAnts (src/Colony.cu:
+ * 1096-1104, 447-467). */
+int aco_get_stats(aco_ctx *c, int64_t *iter_best, int64_t *glob_best, uint32_t *valid_ants);
+
+/* Colony::getTour (src/Colony.cu:1173-1180) as 0-based city indices (n). */
+int aco_get_best_tour(aco_ctx *c, int32_t *tour);
+
+/* Parity taps. */
+int aco_get_tours(aco_ctx *c, int32_t *tours);            /* m*n          */
+int aco_get_lengths(aco_ctx *c, int64_t *lengths);        /* m            */
+int aco_get_pheromone(aco_ctx *c, float *tau);            /* n*n, current */
+int aco_set_pheromone(aco_ctx *c, const float *tau);      /* n*n          */
+int aco_get_choice(aco_ctx *c, float *choice);            /* n*n, last sweep */
+int aco_get_choice_nn(aco_ctx *c, float *choice_nn);      /* n*k          */
+int aco_get_nn(aco_ctx *c, int32_t *nn);                  /* n*k (or n*n full-row) */
+int aco_get_mean(aco_ctx *c, float *mean);                /* n            */
+int aco_get_dist(aco_ctx *c, int32_t *dist);              /* n*n          */
+int aco_get_eta(aco_ctx *c, float *eta, int32_t *eta_len);/* eta may be NULL to query */
+/* aging state: age[m], valid[m], perf[m], prev_len[m];
+ * scalars = {num_valid, converged, gperf, iteration} */
+int aco_get_aging(aco_ctx *c, int32_t *age, uint8_t *valid, int32_t *perf, int64_t *prev_len,
+                  int32_t *scalars);
+int aco_set_aging(aco_ctx *c, const int32_t *age, const uint8_t *valid, const int32_t *perf,
+                  const int64_t *prev_len, const int32_t *scalars);
+/* best-so-far length/tour overwrite (parity replays) */
+int aco_set_best(aco_ctx *c, int64_t glob_best, const int32_t *tour);
+/* counters = {fallback steps, unvisited cities scanned by fallback steps} since create */
+int aco_get_counters(aco_ctx *c, uint64_t *counters);
+
+/* Runs `iterations` iterations with HIP events around every kernel on the
+ * ctx stream; kernel_ms[0..3] = average ms of {sweep, construct, stats,
+ * deposit}. */
+int aco_profile(aco_ctx *c, int32_t iterations, float *kernel_ms);
+
+/* Memory-bandwidth calibration: copies `bytes` (multiple of 16) device to
+ * device with a plain streaming kernel `reps` times; returns ms per copy. */
+int aco_calibrate_copy(aco_ctx *c, int64_t bytes, int32_t reps, float *ms);
+
+/* Colony::~Colony */
+int aco_destroy(aco_ctx *c);
+
+/* Message of the last failing call on c (NULL: last aco_create failure on
+ * this thread). Never NULL. */
+const char *aco_last_error(const aco_ctx *c);
+
+const char *aco_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,211 @@
+"""GPU parity: every kernel of the hot path against the CPU oracle.
+
+Each test prints the case it runs (flushed) before touching the GPU.  All
+calls go through the C ABI (include/acogpu.h) via ctypes.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from acogpu_amd import capi, instances
+from replay import check_iteration, replay, snapshot
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def make(params, xy=None, dist=None):
+    col = capi.Colony(params, dist=dist, xy=xy)
+    col.initialize()
+    return col
+
+
+def random_tau(rng, n, lo=1e-6, hi=1e-2):
+    t = np.exp(rng.uniform(np.log(lo), np.log(hi), size=(n, n))).astype(np.float32)
+    t = np.triu(t, 1)
+    return (t + t.T).astype(np.float32)
+
+
+def test_inputs_dist_nn_eta():
+    say("case: inputs n=300 uniform seed=3 k=20 beta=3")
+    xy = instances.uniform(300, 1000, seed=3)
+    p = capi.default_params(n=300, m=8, k=20, beta=3.0)
+    with make(p, xy=xy) as col:
+        ref = orc.dist_from_int_coords(xy)
+        np.testing.assert_array_equal(col.dist(), ref)
+        np.testing.assert_array_equal(col.nn(), orc.nn_lists(ref, 20))
+        np.testing.assert_array_equal(col.eta(), orc.eta_table(3.0, orc.max_offdiag(ref)))
+
+
+def test_inputs_host_distance_matrix():
+    say("case: host dist n=130 duplicate city, diagonal forced")
+    xy = instances.uniform(130, 500, seed=4)
+    xy[7] = xy[3]  # duplicate point: d = 0
+    d = orc.dist_from_int_coords(xy)
+    d2 = d.copy()
+    np.fill_diagonal(d2, 0)  # the diagonal is ignored by the ABI
+    p = capi.default_params(n=130, m=4, k=10, beta=2.0)
+    with make(p, dist=d2) as col:
+        np.testing.assert_array_equal(col.dist(), d)
+        np.testing.assert_array_equal(col.nn(), orc.nn_lists(d, 10))
+
+
+@pytest.mark.parametrize("n,m,k,alpha,beta,rho,seed", [
+    (64, 32, 16, 1.0, 2.0, 0.5, 1),
+    (200, 64, 20, 1.0, 3.0, 0.1, 2),
+    (333, 100, 32, 1.0, 2.0, 0.2, 3),
+    (150, 40, 64, 1.0, 5.0, 0.1, 4),
+])
+def test_as_iteration_replay(n, m, k, alpha, beta, rho, seed):
+    say(f"case: AS candidate n={n} m={m} k={k} beta={beta} rho={rho} seed={seed}")
+    rng = np.random.default_rng(seed)
+    xy = instances.uniform(n, 1000, seed=seed)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=k, alpha=alpha, beta=beta, rho=rho, seed=1000 + seed)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, n))
+        for it in range(3):
+            snap = snapshot(col)
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+
+
+def test_as_iteration_replay_fallback_heavy():
+    say("case: AS candidate n=400 m=48 k=4 (fallback heavy)")
+    xy = instances.uniform(400, 2000, seed=9)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=400, m=48, k=4, beta=2.0, rho=0.3, seed=77)
+    with make(p, xy=xy) as col:
+        snap = snapshot(col)
+        col.iterate()
+        ref = replay(p, dist, snap)
+        check_iteration(col, p, dist, snap, ref)
+        steps, scanned = col.counters()
+        assert steps == ref["fallback"][0] and scanned == ref["fallback"][1]
+        assert steps > 0
+
+
+def test_rbas_aging_replay_c2_shape():
+    say("case: RBAS+aging clustered n=500 m=160 k=20 w=6 beta=3 rho=0.1 (C2 shape, small)")
+    xy = instances.clustered(500, seed=11)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=500, m=160, k=20, w=6, beta=3.0, rho=0.1, variant=capi.VARIANT_RBAS,
+                            aging=1, min_age=4, max_age=6, global_perf=3, min_pop=150,
+                            tau0_policy=capi.TAU0_RBAS, seed=5)
+    with make(p, xy=xy) as col:
+        checked_young = False
+        for it in range(12):
+            snap = snapshot(col)
+            col.iterate()
+            ref = replay(p, dist, snap)
+            check_iteration(col, p, dist, snap, ref)
+            if snap["converged"] and np.any((snap["age"] + 1 < p.min_age) & (snap["valid"] > 0)):
+                checked_young = True
+        assert checked_young, "no young ant after convergence was exercised"
+
+
+def test_rbas_replay_forced_young_state():
+    say("case: RBAS aging forced state n=256 m=64 k=24")
+    rng = np.random.default_rng(21)
+    xy = instances.uniform(256, 5000, seed=21)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=256, m=64, k=24, w=4, beta=2.0, rho=0.25, variant=capi.VARIANT_RBAS,
+                            aging=1, min_age=10, max_age=12, global_perf=5, min_pop=60, seed=99)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, 256))
+        age = rng.integers(0, 14, size=64).astype(np.int32)
+        valid = (rng.random(64) < 0.8).astype(np.uint8)
+        col.set_aging(age=age, valid=valid, perf=np.zeros(64, np.int32), prev_len=np.zeros(64, np.int64),
+                      scalars=np.array([int(valid.sum()), 1, 4, 17], np.int32))
+        col.set_best(10 ** 9, np.arange(256, dtype=np.int32))
+        snap = snapshot(col)
+        col.iterate()
+        check_iteration(col, p, dist, snap, replay(p, dist, snap))
+
+
+@pytest.mark.parametrize("n,m,seed", [(70, 20, 1), (300, 24, 2)])
+def test_fullrow_iteration_replay(n, m, seed):
+    say(f"case: full-row AS n={n} m={m} seed={seed}")
+    rng = np.random.default_rng(seed)
+    xy = instances.uniform(n, 1000, seed=40 + seed)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, mode=capi.MODE_FULLROW, beta=2.0, rho=0.5, seed=seed)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, n))
+        for it in range(2):
+            snap = snapshot(col)
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+
+
+@pytest.mark.parametrize("mode", [capi.MODE_CANDIDATE, capi.MODE_FULLROW])
+def test_deterministic_run_matches_oracle_run(mode):
+    """RBAS with w=1 deposits one tour, so every pheromone cell receives at
+    most one fp32 add per iteration: the whole multi-iteration run is
+    deterministic and must equal the oracle's fp32 run bit for bit, through
+    the hipGraph replay path."""
+    say(f"case: w=1 RBAS graph run 20 iterations mode={mode}")
+    xy = instances.uniform(120, 1000, seed=5)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=120, m=30, k=12, w=1, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS,
+                            mode=mode, seed=3, tau0=0.01)
+    with make(p, xy=xy) as col:
+        col.run(20)
+        _, gb, _ = col.stats()
+        secs, ogb, oib, obest = orc.run(dist, 30, variant=orc.RBAS,
+                                        mode=orc.MODE_CANDIDATE if mode == capi.MODE_CANDIDATE else orc.MODE_FULLROW,
+                                        k=12, w=1, iterations=20, alpha=1.0, beta=2.0, rho=0.2,
+                                        tau0=0.01, seed=3)
+        assert gb == ogb
+        np.testing.assert_array_equal(col.best_tour(), obest)
+        tau = col.pheromone()
+        assert np.isfinite(tau).all()
+
+
+def test_graph_and_eager_paths_agree():
+    say("case: graph vs eager, AS m=1, 6 iterations")
+    xy = instances.uniform(90, 1000, seed=8)
+    p = capi.default_params(n=90, m=1, k=10, beta=2.0, rho=0.3, seed=4)
+    with make(p, xy=xy) as a, make(p, xy=xy) as b:
+        a.run(6)
+        for _ in range(6):
+            b.iterate()
+        np.testing.assert_array_equal(a.tours(), b.tours())
+        np.testing.assert_array_equal(a.pheromone(), b.pheromone())
+        assert a.stats() == b.stats()
+
+
+def test_tau0_policies():
+    say("case: tau0 policies n=100")
+    xy = instances.uniform(100, 1000, seed=12)
+    dist = orc.dist_from_int_coords(xy)
+    cnn = orc.greedy_length(dist, 0)
+    p = capi.default_params(n=100, m=50, k=10, tau0_policy=capi.TAU0_AS)
+    with make(p, xy=xy) as col:
+        assert col.pheromone()[0, 1] == np.float32(50.0 / cnn)
+    p = capi.default_params(n=100, m=50, k=10, w=6, rho=0.1, variant=capi.VARIANT_RBAS,
+                            tau0_policy=capi.TAU0_RBAS)
+    with make(p, xy=xy) as col:
+        assert col.pheromone()[0, 1] == np.float32(0.5 * 6 * 5 / (np.float64(np.float32(0.1)) * cnn))
+
+
+def test_c2_full_size_properties():
+    say("case: C2 full size n=1000 m=1000 k=20 RBAS+aging, 30 iterations")
+    xy = instances.clustered(1000, seed=2024)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=1000, m=1000, k=20, w=6, beta=3.0, rho=0.1, variant=capi.VARIANT_RBAS,
+                            aging=1, tau0_policy=capi.TAU0_RBAS, seed=1)
+    with make(p, xy=xy) as col:
+        col.run(30)
+        ib, gb, nv = col.stats()
+        best = col.best_tour()
+        assert sorted(best.tolist()) == list(range(1000))
+        assert instances.tour_length(dist, best) == gb
+        assert gb <= ib
+        snap = snapshot(col)
+        col.iterate()
+        ref = replay(p, dist, snap)
+        check_iteration(col, p, dist, snap, ref)
