@@ -34,7 +34,8 @@ class Params(ctypes.Structure):
         ("min_age", ctypes.c_int32), ("max_age", ctypes.c_int32), ("local_perf", ctypes.c_int32),
         ("global_perf", ctypes.c_int32), ("min_pop", ctypes.c_int32), ("max_pop", ctypes.c_int32),
         ("broker", ctypes.c_int32), ("ants_per_block", ctypes.c_int32), ("seed", ctypes.c_uint64),
-        ("stream_id", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 7),
+        ("stream_id", ctypes.c_uint32), ("construct_path", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -64,6 +65,7 @@ SIGNATURES = {
     "aco_set_aging": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "aco_set_best": (ctypes.c_int, [_vp, _i64, _vp]),
     "aco_get_counters": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_config": (ctypes.c_int, [_vp, _vp]),
     "aco_profile": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_calibrate_copy": (ctypes.c_int, [_vp, _i64, _i32, _vp]),
     "aco_destroy": (ctypes.c_int, [_vp]),
@@ -242,6 +244,12 @@ class Colony:
         self._chk(self._L.aco_set_aging(self._ctx, _p(conv(age, np.int32)), _p(conv(valid, np.uint8)),
                                         _p(conv(perf, np.int32)), _p(conv(prev_len, np.int64)),
                                         _p(conv(scalars, np.int32))))
+
+    def config(self):
+        t = np.zeros(4, dtype=np.int32)
+        self._chk(self._L.aco_get_config(self._ctx, _p(t)))
+        return {"path": int(t[0]), "ants_per_block": int(t[1]), "young_blocks": int(t[2]),
+                "max_lds": int(t[3])}
 
     def counters(self):
         t = np.zeros(2, dtype=np.uint64)

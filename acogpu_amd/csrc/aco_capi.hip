@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/acogpu.h"
@@ -18,14 +19,20 @@ using aco::DevState;
 
 struct aco_ctx {
     aco_params p;
-    int32_t n = 0, m = 0, k = 0, P = 0, words = 0, apb = 2;
+    int32_t n = 0, m = 0, k = 0, P = 0, words = 0, apb = 1;
+    int32_t path = 0;          // construction kernel: 0 global, 1 full row, 2 LDS-staged
+    int32_t young_blocks = 0;  // staged path: blocks reserved for young aging ants
+    int32_t max_lds = 0;       // LDS bytes per block on this device
     int32_t nn_cols = 0;  // k (candidate) or n (full row)
     hipStream_t stream = nullptr;
     int32_t* d_dist = nullptr;
     float* d_tau[2] = {nullptr, nullptr};
     float* d_choice = nullptr;
     float* d_choice_nn = nullptr;
+    float* d_tau_nn = nullptr;
+    float* d_eta_nn = nullptr;
     int32_t* d_nn = nullptr;
+    uint16_t* d_nn16 = nullptr;
     float* d_eta = nullptr;
     int32_t eta_len = 0;
     float* d_mean = nullptr;
@@ -96,6 +103,7 @@ extern "C" int aco_params_init(aco_params* p) {
 static void free_all(aco_ctx* c) {
     if (c->graph) hipGraphExecDestroy(c->graph);
     void* ptrs[] = {c->d_dist, c->d_tau[0], c->d_tau[1], c->d_choice, c->d_choice_nn, c->d_nn, c->d_eta,
+                    c->d_tau_nn, c->d_eta_nn, c->d_nn16,
                     c->d_mean, c->d_tours, c->d_len, c->d_prev_len, c->d_age, c->d_perf, c->d_valid,
                     c->d_best_tour, c->d_dep_ant, c->d_dep_val, c->d_state, c->d_scratch64};
     for (void* q : ptrs)
@@ -136,6 +144,8 @@ static int check_params(const aco_params* p) {
         return fail(nullptr, ACO_E_ARG, "aging needs min_age >= 1 and max_age >= 0");
     if (p->ants_per_block < 0 || p->ants_per_block > 4)
         return fail(nullptr, ACO_E_ARG, "ants_per_block must be in [0, 4]");
+    if (p->construct_path < 0 || p->construct_path > 2)
+        return fail(nullptr, ACO_E_ARG, "construct_path must be 0 (auto), 1 (global) or 2 (LDS-staged)");
     return ACO_OK;
 }
 
@@ -157,7 +167,47 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     c->nn_cols = p->mode == ACO_MODE_CANDIDATE ? p->k : p->n;
     c->P = (c->n + 63) & ~63;
     c->words = (c->n + 31) / 32;
-    c->apb = p->ants_per_block ? p->ants_per_block : (p->mode == ACO_MODE_CANDIDATE ? 1 : 1);
+    {
+        int dev = 0, max_lds = 65536;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            max_lds = 65536;
+        (void)hipGetLastError();
+        c->max_lds = max_lds;
+        max_lds -= 256;  // static LDS of the kernels
+        if (p->mode == ACO_MODE_FULLROW) {
+            c->path = 1;
+            c->apb = p->ants_per_block ? p->ants_per_block : 1;
+            const int32_t nchunk = (c->n + 63) / 64;
+            const size_t per_ant = ((size_t)c->words + 2 * (size_t)nchunk * 64 + nchunk) * 4;
+            while (c->apb > 1 && per_ant * c->apb > (size_t)max_lds) --c->apb;
+            if (per_ant > (size_t)max_lds) {
+                int rc2 = fail(nullptr, ACO_E_RANGE, "full-row mode needs %zu bytes of LDS per ant (limit %d)", per_ant,
+                               max_lds);
+                delete c;
+                return rc2;
+            }
+        } else {
+            const int32_t want = p->ants_per_block ? p->ants_per_block : 4;
+            int32_t W = want;
+            while (W > 1 && aco::staged_lds_bytes(c->n, c->k, c->words, W) > (size_t)max_lds) --W;
+            const bool fits = c->n <= 65535 && aco::staged_lds_bytes(c->n, c->k, c->words, W) <= (size_t)max_lds;
+            if (p->construct_path == 2 && !fits) {
+                int rc2 = fail(nullptr, ACO_E_RANGE, "LDS-staged construction needs %zu bytes of LDS (limit %d)",
+                               aco::staged_lds_bytes(c->n, c->k, c->words, 1), max_lds);
+                delete c;
+                return rc2;
+            }
+            if (fits && p->construct_path != 1) {
+                c->path = 2;
+                c->apb = W;
+                c->young_blocks = p->aging ? std::min(c->m, std::max(p->min_age, 1)) : 0;
+            } else {
+                c->path = 0;
+                c->apb = p->ants_per_block ? p->ants_per_block : 1;
+            }
+        }
+    }
 
     auto bail = [&](int code) {
         free_all(c);
@@ -186,6 +236,9 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     TRYB(dalloc(c, &c->d_choice, nP));
     TRYB(dalloc(c, &c->d_nn, (size_t)c->n * c->nn_cols));
     TRYB(dalloc(c, &c->d_choice_nn, (size_t)c->n * (c->k > 0 ? c->k : 1)));
+    TRYB(dalloc(c, &c->d_tau_nn, (size_t)c->n * (c->k > 0 ? c->k : 1)));
+    TRYB(dalloc(c, &c->d_eta_nn, (size_t)c->n * (c->k > 0 ? c->k : 1)));
+    TRYB(dalloc(c, &c->d_nn16, (size_t)c->n * (c->k > 0 ? c->k : 1)));
     TRYB(dalloc(c, &c->d_mean, (size_t)c->n));
     TRYB(dalloc(c, &c->d_tours, (size_t)c->m * c->n));
     TRYB(dalloc(c, &c->d_len, (size_t)c->m));
@@ -303,6 +356,9 @@ extern "C" int aco_initialize(aco_ctx* c) {
     HIP_TRY(c, aco::launch_fill_rows(c->d_tau[0], c->n, c->P, tau0, c->stream));
     HIP_TRY(c, aco::launch_fill_rows(c->d_tau[1], c->n, c->P, 0.0f, c->stream));
     HIP_TRY(c, aco::launch_nn_lists(c->d_dist, c->n, c->P, c->nn_cols, c->d_nn, c->stream));
+    if (c->k > 0)
+        HIP_TRY(c, aco::launch_nn_aux(c->d_nn, c->d_dist, c->d_eta, c->eta_len, c->n, c->P, c->k, c->d_nn16,
+                                      c->d_eta_nn, c->stream));
     int rc = reset_state(c);
     if (rc) return rc;
     if (c->graph) {
@@ -331,8 +387,11 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     sa.k = c->k;
     sa.choice = c->d_choice;
     sa.choice_nn = c->d_choice_nn;
+    sa.tau_nn = (cand && c->p.aging) ? c->d_tau_nn : nullptr;
     sa.tau_next = tau_next;
     sa.mean = c->d_mean;
+    sa.age = c->p.aging ? c->d_age : nullptr;
+    sa.m = c->m;
     sa.n = c->n;
     sa.P = c->P;
     sa.alpha = c->p.alpha;
@@ -343,7 +402,10 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
 
     aco::ConstructArgs ca{};
     ca.nn = c->d_nn;
+    ca.nn16 = c->d_nn16;
     ca.choice_nn = c->d_choice_nn;
+    ca.tau_nn = c->d_tau_nn;
+    ca.eta_nn = c->d_eta_nn;
     ca.choice = c->d_choice;
     ca.dist = c->d_dist;
     ca.tau = tau;
@@ -366,7 +428,8 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     ca.seed_hi = (uint32_t)(c->p.seed >> 32);
     ca.stream = c->p.stream_id;
     ca.words = c->words;
-    if ((e = aco::launch_construct(ca, c->p.mode, c->apb, c->stream)) != hipSuccess) return e;
+    ca.young_blocks = c->young_blocks;
+    if ((e = aco::launch_construct(ca, c->path, c->apb, c->stream)) != hipSuccess) return e;
     if (ev) hipEventRecord(ev[2], c->stream);
 
     aco::StatsArgs st{};
@@ -667,5 +730,14 @@ extern "C" int aco_get_counters(aco_ctx* c, uint64_t* counters) {
     if (rc) return rc;
     counters[0] = st.fb_steps;
     counters[1] = st.fb_scanned;
+    return ACO_OK;
+}
+
+extern "C" int aco_get_config(aco_ctx* c, int32_t* config) {
+    if (!c || !config) return ACO_E_ARG;
+    config[0] = c->path;
+    config[1] = c->apb;
+    config[2] = c->young_blocks;
+    config[3] = c->max_lds;
     return ACO_OK;
 }

@@ -32,8 +32,11 @@ struct SweepArgs {
     int32_t k;
     float* choice;           // pitched n x P
     float* choice_nn;        // n x k
+    float* tau_nn;           // n x k pheromone on candidate edges (aging)
     float* tau_next;         // evaporated pheromone (pitched)
     float* mean;             // n row means (aging)
+    int32_t* age;            // m ant ages: incremented here (aging)
+    int32_t m;
     int32_t n, P;
     float alpha, one_minus_rho;
     int32_t want_mean;
@@ -41,7 +44,10 @@ struct SweepArgs {
 
 struct ConstructArgs {
     const int32_t* nn;       // n x k (candidate) or n x n (full row)
+    const uint16_t* nn16;    // n x k, 16-bit copy for LDS staging
     const float* choice_nn;  // n x k
+    const float* tau_nn;     // n x k (aging young ants)
+    const float* eta_nn;     // n x k heuristic on candidate edges
     const float* choice;     // pitched
     const int32_t* dist;     // pitched
     const float* tau;        // pitched, current pheromone (aging young ants)
@@ -58,6 +64,7 @@ struct ConstructArgs {
     float alpha;
     uint32_t seed_lo, seed_hi, stream;
     int32_t words;           // visited-bitmap words per ant
+    int32_t young_blocks;    // trailing blocks reserved for young aging ants (staged path)
 };
 
 struct StatsArgs {
@@ -92,7 +99,12 @@ hipError_t launch_nn_lists(const int32_t* dist, int32_t n, int32_t P, int32_t k,
 hipError_t launch_fill_rows(float* a, int32_t n, int32_t P, float v, hipStream_t s);
 hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, hipStream_t s);
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t s);
-hipError_t launch_construct(const ConstructArgs& a, int32_t mode, int32_t ants_per_block, hipStream_t s);
+// Construction paths: 0 = candidate lists with global reads, 1 = full row,
+// 2 = candidate lists staged in LDS (n*k*6 bytes + per-ant state per block).
+size_t staged_lds_bytes(int32_t n, int32_t k, int32_t words, int32_t ants_per_block);
+hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_per_block, hipStream_t s);
+hipError_t launch_nn_aux(const int32_t* nn, const int32_t* dist, const float* eta, int32_t eta_len, int32_t n,
+                         int32_t P, int32_t k, uint16_t* nn16, float* eta_nn, hipStream_t s);
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
 hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s);
 hipError_t launch_copy(const float4* src, float4* dst, int64_t count, hipStream_t s);

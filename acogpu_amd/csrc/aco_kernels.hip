@@ -152,6 +152,10 @@ __global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dis
 // Algorithmic traffic: 16 bytes per element (read tau, dist; write choice, tau_next).
 __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
     const int lane = threadIdx.x & 63;
+    // updateAntAges (src/Colony.cu:353-364): every ant ages by one at the
+    // start of an iteration, before construction reads the ages.
+    if (a.age != nullptr)
+        for (int32_t k = blockIdx.x * 256 + threadIdx.x; k < a.m; k += gridDim.x * 256) a.age[k] += 1;
     const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= a.n) return;
     const size_t row = (size_t)i * a.P;
@@ -182,7 +186,9 @@ __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
     if (a.nn != nullptr) {
         for (int32_t r = lane; r < a.k; r += 64) {
             const int32_t j = a.nn[(size_t)i * a.k + r];
-            a.choice_nn[(size_t)i * a.k + r] = apply_alpha(trow[j], a.alpha) * eta_of(a.eta, a.eta_len, drow[j]);
+            const float t = trow[j];
+            a.choice_nn[(size_t)i * a.k + r] = apply_alpha(t, a.alpha) * eta_of(a.eta, a.eta_len, drow[j]);
+            if (a.tau_nn != nullptr) a.tau_nn[(size_t)i * a.k + r] = t;
         }
     }
 }
@@ -195,20 +201,30 @@ struct AntCtx {
 };
 
 // compute_fittness (inc/Colony.cuh:14-31) for aging ants younger than
-// min_age after convergence: tau pulled toward the row mean.
+// min_age after convergence: tau pulled toward the row mean,
+// ((tau - mean) * (age/min_age)) + mean, times eta.
+__device__ __forceinline__ float young_formula(float tau, float mc, float ratio, float eta, float alpha) {
+    const float p = ((tau - mc) * ratio) + mc;
+    return apply_alpha(p, alpha) * eta;
+}
+
 __device__ __forceinline__ float young_weight(const ConstructArgs& a, int32_t c, int32_t j, float ratio) {
     const size_t xy = (size_t)c * a.P + j;
-    const float mc = a.mean[c];
-    const float p = ((a.tau[xy] - mc) * ratio) + mc;
-    return apply_alpha(p, a.alpha) * eta_of(a.eta, a.eta_len, a.dist[xy]);
+    return young_formula(a.tau[xy], a.mean[c], ratio, eta_of(a.eta, a.eta_len, a.dist[xy]), a.alpha);
 }
 
 __device__ __forceinline__ bool is_visited(const uint32_t* vis, int32_t j) {
     return (vis[j >> 5] >> (j & 31)) & 1u;
 }
 
-// Tour staging: lane (s & 63) keeps the city of step s; a full chunk of 64
-// steps leaves as one coalesced store.
+// Marks city j visited: a fire-and-forget LDS OR (ds_or_b32) by one lane;
+// the wave's later LDS reads are ordered after it.
+__device__ __forceinline__ void mark_visited(uint32_t* vis, int32_t j, int lane) {
+    if (lane == 0) atomicOr(vis + (j >> 5), 1u << (j & 31));
+}
+
+// Tour staging in VGPRs: lane (s & 63) keeps the city of step s; a full chunk
+// of 64 steps leaves as one coalesced store.
 __device__ __forceinline__ void stage_city(int32_t* tour, int32_t& tbuf, int32_t s, int32_t city, int32_t n,
                                            int lane) {
     if (lane == (s & 63)) tbuf = city;
@@ -218,21 +234,81 @@ __device__ __forceinline__ void stage_city(int32_t* tour, int32_t& tbuf, int32_t
     }
 }
 
+// Random words of one ant, 256 steps per refill: lane l holds the Philox
+// block (base + l), i.e. the words of steps 4(base+l) .. 4(base+l)+3.
+struct AntRng {
+    uint32_t w0, w1, w2, w3;
+    int32_t base;
+};
+
+__device__ __forceinline__ void rng_refill(AntRng& r, int32_t s, uint32_t ant, uint32_t iter, const ConstructArgs& a,
+                                           int lane) {
+    r.base = s >> 2;
+    const Philox4 q = philox4x32_10((uint32_t)(r.base + lane), ant, iter, a.stream, a.seed_lo, a.seed_hi);
+    r.w0 = q.x0;
+    r.w1 = q.x1;
+    r.w2 = q.x2;
+    r.w3 = q.x3;
+}
+
+__device__ __forceinline__ uint32_t rng_word(const AntRng& r, int32_t s) {
+    const int L = (s >> 2) - r.base;
+    const int q = s & 3;
+    const uint32_t v = q == 0 ? r.w0 : (q == 1 ? r.w1 : (q == 2 ? r.w2 : r.w3));
+    return (uint32_t)readlane_i((int)v, L);
+}
+
+// Roulette over one wave of candidate weights: inclusive DPP scan, r = u *
+// total, first lane with prefix > r and positive weight (else the last
+// positive lane).  Returns the lane, or -1 when the total is not positive.
+__device__ __forceinline__ int select_lane(float w, float u) {
+    const float p = wave_inclusive_scan(w);
+    const float total = readlane_f(p, 63);
+    if (!(total > 0.0f)) return -1;
+    const float rr = u * total;
+    const unsigned long long hit = __ballot(p > rr && w > 0.0f);
+    if (hit) return __ffsll((long long)hit) - 1;
+    const unsigned long long pos = __ballot(w > 0.0f);
+    return 63 - __clzll((long long)pos);
+}
+
 // choose_best_next: argmax of the selection weight over every unvisited city,
-// ties to the lowest index.  Lanes stride the row; exact comparisons make the
-// result independent of the reduction order.
-__device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const uint32_t* vis, int32_t c,
-                                                   const AntCtx& ant, int lane, unsigned long long& scanned) {
+// ties to the lowest index.  Lanes stride the row four loads at a time; exact
+// comparisons make the result independent of the reduction order.
+__device__ __noinline__ int32_t fallback_argmax(const ConstructArgs& a, const uint32_t* vis, int32_t c,
+                                                bool young, float ratio, int lane, unsigned long long& scanned) {
     float best = -1.0f;
     int32_t bj = 0x7fffffff;
     const float* crow = a.choice + (size_t)c * a.P;
-    for (int32_t j = lane; j < a.n; j += 64) {
-        if (is_visited(vis, j)) continue;
-        ++scanned;
-        const float x = ant.young ? young_weight(a, c, j, ant.ratio) : crow[j];
-        if (x > best) {
-            best = x;
-            bj = j;
+    if (!young) {
+        for (int32_t j0 = 0; j0 < a.n; j0 += 256) {
+            float x[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int32_t j = j0 + q * 64 + lane;
+                x[q] = j < a.n ? crow[j] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int32_t j = j0 + q * 64 + lane;
+                if (j < a.n && !is_visited(vis, j)) {
+                    ++scanned;
+                    if (x[q] > best) {
+                        best = x[q];
+                        bj = j;
+                    }
+                }
+            }
+        }
+    } else {
+        for (int32_t j = lane; j < a.n; j += 64) {
+            if (is_visited(vis, j)) continue;
+            ++scanned;
+            const float x = young_weight(a, c, j, ratio);
+            if (x > best) {
+                best = x;
+                bj = j;
+            }
         }
     }
     for (int off = 32; off >= 1; off >>= 1) {
@@ -257,88 +333,217 @@ __device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const
     return next;
 }
 
+__device__ __forceinline__ void finish_counters(const ConstructArgs& a, unsigned long long fb,
+                                                unsigned long long scanned, int lane) {
+    for (int off = 32; off >= 1; off >>= 1) scanned += __shfl_xor(scanned, off);
+    if (lane == 0) {
+        if (fb) atomicAdd(&a.state->fb_steps, fb);
+        if (scanned) atomicAdd(&a.state->fb_scanned, scanned);
+    }
+}
+
+__device__ __forceinline__ bool ant_is_young(const ConstructArgs& a, int32_t ant, int32_t converged) {
+    return a.aging && converged && a.valid[ant] && a.age[ant] < a.min_age;
+}
+
 // ---------------------------------------------------------------------------
-// Tour construction, candidate-list mode.  One ant per wave64 for all n-1
-// steps (the reference launches 3(n-1) kernels per iteration,
-// src/Colony.cu:713-755).  Per step: lane r < k holds candidate r of the
-// current city, its weight (0 if visited, per the LDS bitmap), the DPP
-// inclusive scan gives the roulette prefix, a ballot picks the first lane
-// whose prefix exceeds u * total.  Zero total -> argmax fallback over the
-// unvisited cities.  Tour length accumulated exactly in int64.
+// Tour construction, candidate-list mode, global-memory variant (any n).
+// One ant per wave64 for all n-1 steps (the reference launches 3(n-1)
+// kernels per iteration, src/Colony.cu:713-755).  Per step: lane r < k holds
+// candidate r of the current city and its weight (0 if visited, per the LDS
+// bitmap); the DPP scan gives the roulette prefix and a ballot the pick; a
+// zero total falls back to the argmax over the unvisited cities.  The tour
+// length is accumulated exactly in int64, one step behind (the distance load
+// completes under the next step's candidate loads).
 __global__ __launch_bounds__(256) void k_construct_cand(const ConstructArgs a) {
     extern __shared__ uint32_t lds[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int32_t ant = blockIdx.x * (blockDim.x >> 6) + wave;
     if (ant >= a.m) return;
+    if (a.aging && !a.valid[ant]) return;
     uint32_t* vis = lds + (size_t)wave * a.words;
     const uint32_t iter = a.state->iter;
+    const bool young = ant_is_young(a, ant, a.state->converged);
+    const float ratio = young ? (float)a.age[ant] / (float)a.min_age : 0.0f;
 
-    AntCtx ac{false, 0.0f};
-    if (a.aging) {
-        const int32_t age = a.age[ant] + 1;  // updateAntAges, every ant
-        if (lane == 0) a.age[ant] = age;
-        if (!a.valid[ant]) return;
-        ac.young = a.state->converged && age < a.min_age;
-        ac.ratio = ac.young ? (float)age / (float)a.min_age : 0.0f;
-    }
     for (int32_t w = lane; w < a.words; w += 64) vis[w] = 0u;
-    wave_lds_sync();
-
-    Philox4 rb = philox4x32_10(0u, (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
-    const int32_t start = word_to_city(rb.x0, a.n);
-    if (lane == 0) vis[start >> 5] |= 1u << (start & 31);
-    wave_lds_sync();
+    AntRng rng;
+    rng_refill(rng, 0, (uint32_t)ant, iter, a, lane);
+    const int32_t start = word_to_city(rng_word(rng, 0), a.n);
+    mark_visited(vis, start, lane);
 
     int32_t* tour = a.tours + (size_t)ant * a.n;
     int32_t tbuf = start;
     int64_t len = 0;
+    int32_t pend = 0;
     int32_t c = start;
     unsigned long long fb = 0, scanned = 0;
     for (int32_t s = 1; s < a.n; ++s) {
-        if ((s & 3) == 0) rb = philox4x32_10((uint32_t)(s >> 2), (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
-        const float u = word_to_uniform(philox_word(rb, (uint32_t)(s & 3)));
+        if ((s & 255) == 0) rng_refill(rng, s, (uint32_t)ant, iter, a, lane);
+        const float u = word_to_uniform(rng_word(rng, s));
         int32_t j = 0;
         float w = 0.0f;
         if (lane < a.k) {
-            j = a.nn[(size_t)c * a.k + lane];
-            const float cw = ac.young ? young_weight(a, c, j, ac.ratio) : a.choice_nn[(size_t)c * a.k + lane];
+            const size_t e = (size_t)c * a.k + lane;
+            j = a.nn[e];
+            const float cw = young ? young_formula(a.tau_nn[e], a.mean[c], ratio, a.eta_nn[e], a.alpha)
+                                   : a.choice_nn[e];
             w = is_visited(vis, j) ? 0.0f : cw;
         }
-        const float p = wave_inclusive_scan(w);
-        const float total = readlane_f(p, 63);
+        len += pend;
+        const int sel = select_lane(w, u);
         int32_t next;
-        if (total > 0.0f) {
-            const float rr = u * total;
-            const unsigned long long hit = __ballot(p > rr && w > 0.0f);
-            int sel;
-            if (hit) {
-                sel = __ffsll((long long)hit) - 1;
-            } else {
-                const unsigned long long pos = __ballot(w > 0.0f);
-                sel = 63 - __clzll((long long)pos);
-            }
+        if (sel >= 0) {
             next = readlane_i(j, sel);
         } else {
             ++fb;
-            next = fallback_argmax(a, vis, c, ac, lane, scanned);
+            next = fallback_argmax(a, vis, c, young, ratio, lane, scanned);
         }
-        if (lane == 0) {
-            vis[next >> 5] |= 1u << (next & 31);
-            len += a.dist[(size_t)c * a.P + next];
-        }
-        wave_lds_sync();
+        mark_visited(vis, next, lane);
+        pend = a.dist[(size_t)c * a.P + next];
         stage_city(tour, tbuf, s, next, a.n, lane);
         c = next;
     }
-    for (int off = 32; off >= 1; off >>= 1) scanned += __shfl_xor(scanned, off);
+    len += pend;
+    finish_counters(a, fb, scanned, lane);
     if (lane == 0) {
-        if (a.n == 1) tour[0] = start;
         len += a.dist[(size_t)c * a.P + start];
         a.lengths[ant] = len;
-        if (fb) atomicAdd(&a.state->fb_steps, fb);
-        if (scanned) atomicAdd(&a.state->fb_scanned, scanned);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Tour construction, candidate-list mode, LDS-staged variant (n*k*6 bytes fit
+// in LDS).  Every block first stages the candidate lists (16-bit) and their
+// weights for the whole instance, so each step reads only LDS.  Normal
+// blocks stage choice_nn and run W ants each; the trailing young_blocks
+// blocks each take one aging ant younger than min_age after convergence and
+// stage that ant's pulled-toward-the-mean weights instead (they differ per
+// age), so young ants also run at LDS speed.  Tours are kept in LDS (16-bit)
+// and written, with their exact lengths, after the last step.
+__device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int32_t ant, bool young, float ratio,
+                                                     const float* wl, const uint16_t* nl, uint32_t* vis,
+                                                     uint16_t* t16, uint32_t iter, int lane) {
+    for (int32_t w = lane; w < a.words; w += 64) vis[w] = 0u;
+    AntRng rng;
+    rng_refill(rng, 0, (uint32_t)ant, iter, a, lane);
+    const int32_t start = word_to_city(rng_word(rng, 0), a.n);
+    mark_visited(vis, start, lane);
+    if (lane == 0) t16[0] = (uint16_t)start;
+    int32_t c = start;
+    unsigned long long fb = 0, scanned = 0;
+    for (int32_t s = 1; s < a.n; ++s) {
+        if ((s & 255) == 0) rng_refill(rng, s, (uint32_t)ant, iter, a, lane);
+        const float u = word_to_uniform(rng_word(rng, s));
+        int32_t j = 0;
+        float w = 0.0f;
+        if (lane < a.k) {
+            const int32_t e = c * a.k + lane;
+            j = nl[e];
+            const float cw = wl[e];
+            w = is_visited(vis, j) ? 0.0f : cw;
+        }
+        const int sel = select_lane(w, u);
+        int32_t next;
+        if (sel >= 0) {
+            next = readlane_i(j, sel);
+        } else {
+            ++fb;
+            next = fallback_argmax(a, vis, c, young, ratio, lane, scanned);
+        }
+        mark_visited(vis, next, lane);
+        if (lane == 0) t16[s] = (uint16_t)next;
+        c = next;
+    }
+    finish_counters(a, fb, scanned, lane);
+    // tour out (coalesced) and exact length, edge i -> i+1 per lane
+    int32_t* tour = a.tours + (size_t)ant * a.n;
+    int64_t len = 0;
+    for (int32_t i = lane; i < a.n; i += 64) {
+        const int32_t x = t16[i];
+        const int32_t y = t16[(i + 1 == a.n) ? 0 : i + 1];
+        tour[i] = x;
+        len += a.dist[(size_t)x * a.P + y];
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)len, off);
+        const uint32_t hi = __shfl_xor((uint32_t)((uint64_t)len >> 32), off);
+        len += (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    if (lane == 0) a.lengths[ant] = len;
+}
+
+__global__ __launch_bounds__(256) void k_construct_staged(const ConstructArgs a) {
+    extern __shared__ uint32_t lds[];
+    __shared__ int32_t s_ant;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int32_t nk = a.n * a.k;
+    float* wl = reinterpret_cast<float*>(lds);
+    uint16_t* nl = reinterpret_cast<uint16_t*>(wl + nk);
+    uint32_t* per = reinterpret_cast<uint32_t*>(nl + ((nk + 1) & ~1));
+    const int32_t per_words = a.words + (a.n + 1) / 2;
+    uint32_t* vis = per + (size_t)wave * per_words;
+    uint16_t* t16 = reinterpret_cast<uint16_t*>(vis + a.words);
+    const uint32_t iter = a.state->iter;
+    const int32_t converged = a.state->converged;
+    const int32_t normal_blocks = gridDim.x - a.young_blocks;
+
+    for (int32_t x = threadIdx.x; x < nk; x += blockDim.x) nl[x] = a.nn16[x];
+
+    if ((int32_t)blockIdx.x < normal_blocks) {
+        const int32_t ant = blockIdx.x * nwaves + wave;
+        for (int32_t x = threadIdx.x; x < nk; x += blockDim.x) wl[x] = a.choice_nn[x];
+        __syncthreads();
+        if (ant >= a.m) return;
+        if (a.aging && (!a.valid[ant] || ant_is_young(a, ant, converged))) return;
+        construct_staged_ant(a, ant, false, 0.0f, wl, nl, vis, t16, iter, lane);
+        return;
+    }
+    if (!a.aging || !converged) return;
+    // Young blocks: block y takes the young ants y, y + young_blocks, ...
+    for (int32_t y = blockIdx.x - normal_blocks;; y += a.young_blocks) {
+        if (wave == 0) {
+            int32_t found = -1, seen = 0;
+            for (int32_t a0 = 0; a0 < a.m && found < 0; a0 += 64) {
+                const int32_t k = a0 + lane;
+                unsigned long long mk = __ballot(k < a.m && ant_is_young(a, k, converged));
+                const int cnt = __popcll(mk);
+                if (seen + cnt > y) {
+                    for (int t = 0; t < y - seen; ++t) mk &= mk - 1;
+                    found = a0 + __ffsll((long long)mk) - 1;
+                }
+                seen += cnt;
+            }
+            if (lane == 0) s_ant = found;
+        }
+        __syncthreads();
+        const int32_t ant = s_ant;
+        if (ant < 0) return;
+        const float ratio = (float)a.age[ant] / (float)a.min_age;
+        for (int32_t x = threadIdx.x; x < nk; x += blockDim.x) {
+            const int32_t row = x / a.k;
+            wl[x] = young_formula(a.tau_nn[x], a.mean[row], ratio, a.eta_nn[x], a.alpha);
+        }
+        __syncthreads();
+        if (wave == 0) construct_staged_ant(a, ant, true, ratio, wl, nl, vis, t16, iter, lane);
+        __syncthreads();
+    }
+}
+
+// 16-bit candidate lists and the heuristic on candidate edges (static; built
+// once after the NN lists).
+__global__ __launch_bounds__(256) void k_nn_aux(const int32_t* __restrict__ nn, const int32_t* __restrict__ dist,
+                                                const float* __restrict__ eta, int32_t eta_len, int32_t n, int32_t P,
+                                                int32_t k, uint16_t* __restrict__ nn16, float* __restrict__ eta_nn) {
+    const int32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= n * k) return;
+    const int32_t i = x / k;
+    const int32_t j = nn[x];
+    if (nn16) nn16[x] = (uint16_t)j;
+    eta_nn[x] = eta_of(eta, eta_len, dist[(size_t)i * P + j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -363,20 +568,15 @@ __global__ __launch_bounds__(256) void k_construct_full(const ConstructArgs a) {
     float* tp = pbuf + (size_t)nchunk * 64;
     const uint32_t iter = a.state->iter;
 
+    if (a.aging && !a.valid[ant]) return;
     AntCtx ac{false, 0.0f};
-    if (a.aging) {
-        const int32_t age = a.age[ant] + 1;
-        if (lane == 0) a.age[ant] = age;
-        if (!a.valid[ant]) return;
-        ac.young = a.state->converged && age < a.min_age;
-        ac.ratio = ac.young ? (float)age / (float)a.min_age : 0.0f;
-    }
+    ac.young = ant_is_young(a, ant, a.state->converged);
+    ac.ratio = ac.young ? (float)a.age[ant] / (float)a.min_age : 0.0f;
     for (int32_t w = lane; w < a.words; w += 64) vis[w] = 0u;
-    wave_lds_sync();
-    Philox4 rb = philox4x32_10(0u, (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
-    const int32_t start = word_to_city(rb.x0, a.n);
-    if (lane == 0) vis[start >> 5] |= 1u << (start & 31);
-    wave_lds_sync();
+    AntRng rng;
+    rng_refill(rng, 0, (uint32_t)ant, iter, a, lane);
+    const int32_t start = word_to_city(rng_word(rng, 0), a.n);
+    mark_visited(vis, start, lane);
 
     int32_t* tour = a.tours + (size_t)ant * a.n;
     int32_t tbuf = start;
@@ -384,8 +584,8 @@ __global__ __launch_bounds__(256) void k_construct_full(const ConstructArgs a) {
     int32_t c = start;
     unsigned long long fb = 0;
     for (int32_t s = 1; s < a.n; ++s) {
-        if ((s & 3) == 0) rb = philox4x32_10((uint32_t)(s >> 2), (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
-        const float u = word_to_uniform(philox_word(rb, (uint32_t)(s & 3)));
+        if ((s & 255) == 0) rng_refill(rng, s, (uint32_t)ant, iter, a, lane);
+        const float u = word_to_uniform(rng_word(rng, s));
         const int32_t* nrow = a.nn + (size_t)c * a.n;
         const float* crow = a.choice + (size_t)c * a.P;
         float T = 0.0f;
@@ -402,7 +602,6 @@ __global__ __launch_bounds__(256) void k_construct_full(const ConstructArgs a) {
             if (lane == 0) tp[q] = T;
             T = T + readlane_f(p, 63);
         }
-        wave_lds_sync();
         int32_t next = -1;
         if (T > 0.0f) {
             const float rr = u * T;
@@ -428,11 +627,8 @@ __global__ __launch_bounds__(256) void k_construct_full(const ConstructArgs a) {
             }
         }
         next = uniform_i(next);
-        if (lane == 0) {
-            vis[next >> 5] |= 1u << (next & 31);
-            len += a.dist[(size_t)c * a.P + next];
-        }
-        wave_lds_sync();
+        mark_visited(vis, next, lane);
+        if (lane == 0) len += a.dist[(size_t)c * a.P + next];
         stage_city(tour, tbuf, s, next, a.n, lane);
         c = next;
     }
@@ -665,16 +861,31 @@ hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_construct(const ConstructArgs& a, int32_t mode, int32_t ants_per_block, hipStream_t s) {
+size_t staged_lds_bytes(int32_t n, int32_t k, int32_t words, int32_t ants_per_block) {
+    const size_t nk = (size_t)n * k;
+    return nk * 4 + ((nk + 1) & ~(size_t)1) * 2 + (size_t)ants_per_block * (words + (n + 1) / 2) * 4;
+}
+
+hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_per_block, hipStream_t s) {
     const int W = ants_per_block;
-    if (mode == 0) {
+    if (path == 0) {
         const size_t lds = (size_t)W * a.words * sizeof(uint32_t);
         hipLaunchKernelGGL(k_construct_cand, dim3(cdiv(a.m, W)), dim3(64 * W), lds, s, a);
+    } else if (path == 2) {
+        const size_t lds = staged_lds_bytes(a.n, a.k, a.words, W);
+        hipLaunchKernelGGL(k_construct_staged, dim3(cdiv(a.m, W) + a.young_blocks), dim3(64 * W), lds, s, a);
     } else {
         const int32_t nchunk = (a.n + 63) / 64;
         const size_t per_ant = (size_t)a.words + 2 * (size_t)nchunk * 64 + nchunk;
         hipLaunchKernelGGL(k_construct_full, dim3(cdiv(a.m, W)), dim3(64 * W), (size_t)W * per_ant * 4, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_nn_aux(const int32_t* nn, const int32_t* dist, const float* eta, int32_t eta_len, int32_t n,
+                         int32_t P, int32_t k, uint16_t* nn16, float* eta_nn, hipStream_t s) {
+    hipLaunchKernelGGL(k_nn_aux, dim3(cdiv((int64_t)n * k, 256)), dim3(256), 0, s, nn, dist, eta, eta_len, n, P, k,
+                       nn16, eta_nn);
     return hipGetLastError();
 }
 

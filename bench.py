@@ -89,6 +89,8 @@ def main():
     log(f"bench: creating colony n={n} m={m} k={k} RBAS w=6 aging")
     col = capi.Colony(p, xy=xy)
     col.initialize()
+    cfg = col.config()
+    log(f"bench: launch config {cfg}")
     log(f"bench: warmup {args.warmup} iterations")
     col.run(args.warmup)
     log(f"bench: timed {args.steps} iterations (hipGraph replay)")
@@ -102,12 +104,12 @@ def main():
 
     # Per-kernel durations with HIP events on the ctx stream.
     fb0 = col.counters()
-    kms = col.profile(args.profile_iters)
+    kms = [float(x) for x in col.profile(args.profile_iters)]
     fb1 = col.counters()
-    scanned_per_launch = (fb1[1] - fb0[1]) / args.profile_iters
+    scanned_per_launch = float(fb1[1] - fb0[1]) / args.profile_iters
     nvalid = col.stats()[2]
     bytes_launch = construct_bytes_per_launch(n, k, nvalid, scanned_per_launch)
-    achieved = bytes_launch / (kms[1] * 1e-3) / 1e9
+    achieved = float(bytes_launch / (kms[1] * 1e-3) / 1e9)
     log(f"bench: kernel ms sweep {kms[0]:.4f} construct {kms[1]:.4f} stats {kms[2]:.4f} "
         f"deposit {kms[3]:.4f}; fallback cities scanned/launch {scanned_per_launch:.0f}")
 
@@ -141,7 +143,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "rank-based AS + aging, n=1000 clustered (10 Gaussian clusters, "
                                "sigma 400), m=1000, k=20, w=6, alpha 1, beta 3, rho 0.1",
-                   "n": n, "m": m, "k": k, "iterations_timed": args.steps},
+                   "n": n, "m": m, "k": k, "iterations_timed": args.steps,
+                   "construct_path": cfg["path"], "ants_per_block": cfg["ants_per_block"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "k_construct_cand", "kernel_ms": float(kms[1]),

@@ -80,7 +80,8 @@ typedef struct aco_params {
     int32_t ants_per_block; /* construction launch shape, 0 = autotuned       */
     uint64_t seed;          /* Philox key (reference: RANDOM_SEED, inc/Colony.h:15) */
     uint32_t stream_id;     /* Philox counter word 3 (instance id)            */
-    int32_t reserved[7];
+    int32_t construct_path; /* candidate mode: 0 auto, 1 global memory, 2 LDS-staged */
+    int32_t reserved[6];
 } aco_params;
 
 /* Reference defaults (inc/default.h:8-39, src/Colony.cu:32-36). */
@@ -133,6 +134,11 @@ int aco_set_aging(aco_ctx *c, const int32_t *age, const uint8_t *valid, const in
 int aco_set_best(aco_ctx *c, int64_t glob_best, const int32_t *tour);
 /* counters = {fallback steps, unvisited cities scanned by fallback steps} since create */
 int aco_get_counters(aco_ctx *c, uint64_t *counters);
+
+/* Launch configuration chosen at create: config = {construction path
+ * (0 global memory, 1 full row, 2 LDS-staged), ants per block, blocks
+ * reserved for young aging ants, LDS bytes per block available}. */
+int aco_get_config(aco_ctx *c, int32_t *config);
 
 /* Runs `iterations` iterations with HIP events around every kernel on the
  * ctx stream; kernel_ms[0..3] = average ms of {sweep, construct, stats,

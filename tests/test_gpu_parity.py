@@ -53,19 +53,26 @@ def test_inputs_host_distance_matrix():
         np.testing.assert_array_equal(col.nn(), orc.nn_lists(d, 10))
 
 
+PATHS = [1, 2]  # construction in global memory / staged in LDS
+
+
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("n,m,k,alpha,beta,rho,seed", [
     (64, 32, 16, 1.0, 2.0, 0.5, 1),
     (200, 64, 20, 1.0, 3.0, 0.1, 2),
     (333, 100, 32, 1.0, 2.0, 0.2, 3),
     (150, 40, 64, 1.0, 5.0, 0.1, 4),
+    (257, 70, 20, 1.5, 2.0, 0.3, 5),
 ])
-def test_as_iteration_replay(n, m, k, alpha, beta, rho, seed):
-    say(f"case: AS candidate n={n} m={m} k={k} beta={beta} rho={rho} seed={seed}")
+def test_as_iteration_replay(n, m, k, alpha, beta, rho, seed, path):
+    say(f"case: AS candidate n={n} m={m} k={k} alpha={alpha} beta={beta} rho={rho} seed={seed} path={path}")
     rng = np.random.default_rng(seed)
     xy = instances.uniform(n, 1000, seed=seed)
     dist = orc.dist_from_int_coords(xy)
-    p = capi.default_params(n=n, m=m, k=k, alpha=alpha, beta=beta, rho=rho, seed=1000 + seed)
+    p = capi.default_params(n=n, m=m, k=k, alpha=alpha, beta=beta, rho=rho, seed=1000 + seed,
+                            construct_path=path)
     with make(p, xy=xy) as col:
+        assert col.config()["path"] == (0 if path == 1 else 2)
         col.set_pheromone(random_tau(rng, n))
         for it in range(3):
             snap = snapshot(col)
@@ -73,11 +80,12 @@ def test_as_iteration_replay(n, m, k, alpha, beta, rho, seed):
             check_iteration(col, p, dist, snap, replay(p, dist, snap))
 
 
-def test_as_iteration_replay_fallback_heavy():
-    say("case: AS candidate n=400 m=48 k=4 (fallback heavy)")
+@pytest.mark.parametrize("path", PATHS)
+def test_as_iteration_replay_fallback_heavy(path):
+    say(f"case: AS candidate n=400 m=48 k=4 (fallback heavy) path={path}")
     xy = instances.uniform(400, 2000, seed=9)
     dist = orc.dist_from_int_coords(xy)
-    p = capi.default_params(n=400, m=48, k=4, beta=2.0, rho=0.3, seed=77)
+    p = capi.default_params(n=400, m=48, k=4, beta=2.0, rho=0.3, seed=77, construct_path=path)
     with make(p, xy=xy) as col:
         snap = snapshot(col)
         col.iterate()
@@ -88,16 +96,19 @@ def test_as_iteration_replay_fallback_heavy():
         assert steps > 0
 
 
-def test_rbas_aging_replay_c2_shape():
-    say("case: RBAS+aging clustered n=500 m=160 k=20 w=6 beta=3 rho=0.1 (C2 shape, small)")
+@pytest.mark.parametrize("path", PATHS)
+def test_rbas_aging_replay_c2_shape(path):
+    say(f"case: RBAS+aging clustered n=500 m=160 k=20 w=6 beta=3 rho=0.1 (C2 shape, small) path={path}")
     xy = instances.clustered(500, seed=11)
     dist = orc.dist_from_int_coords(xy)
+    # max_age 6: retirements from the 6th iteration on; min_pop 159 makes every
+    # retirement after the first trigger a recruitment, so young ants appear.
     p = capi.default_params(n=500, m=160, k=20, w=6, beta=3.0, rho=0.1, variant=capi.VARIANT_RBAS,
-                            aging=1, min_age=4, max_age=6, global_perf=3, min_pop=150,
-                            tau0_policy=capi.TAU0_RBAS, seed=5)
+                            aging=1, min_age=4, max_age=6, global_perf=3, min_pop=159,
+                            tau0_policy=capi.TAU0_RBAS, seed=5, construct_path=path)
     with make(p, xy=xy) as col:
         checked_young = False
-        for it in range(12):
+        for it in range(14):
             snap = snapshot(col)
             col.iterate()
             ref = replay(p, dist, snap)
@@ -107,13 +118,15 @@ def test_rbas_aging_replay_c2_shape():
         assert checked_young, "no young ant after convergence was exercised"
 
 
-def test_rbas_replay_forced_young_state():
-    say("case: RBAS aging forced state n=256 m=64 k=24")
+@pytest.mark.parametrize("path", PATHS)
+def test_rbas_replay_forced_young_state(path):
+    say(f"case: RBAS aging forced state n=256 m=64 k=24 path={path}")
     rng = np.random.default_rng(21)
     xy = instances.uniform(256, 5000, seed=21)
     dist = orc.dist_from_int_coords(xy)
     p = capi.default_params(n=256, m=64, k=24, w=4, beta=2.0, rho=0.25, variant=capi.VARIANT_RBAS,
-                            aging=1, min_age=10, max_age=12, global_perf=5, min_pop=60, seed=99)
+                            aging=1, min_age=10, max_age=12, global_perf=5, min_pop=60, seed=99,
+                            construct_path=path)
     with make(p, xy=xy) as col:
         col.set_pheromone(random_tau(rng, 256))
         age = rng.integers(0, 14, size=64).astype(np.int32)
@@ -141,17 +154,18 @@ def test_fullrow_iteration_replay(n, m, seed):
             check_iteration(col, p, dist, snap, replay(p, dist, snap))
 
 
-@pytest.mark.parametrize("mode", [capi.MODE_CANDIDATE, capi.MODE_FULLROW])
-def test_deterministic_run_matches_oracle_run(mode):
+@pytest.mark.parametrize("mode,path", [(capi.MODE_CANDIDATE, 1), (capi.MODE_CANDIDATE, 2),
+                                       (capi.MODE_FULLROW, 0)])
+def test_deterministic_run_matches_oracle_run(mode, path):
     """RBAS with w=1 deposits one tour, so every pheromone cell receives at
     most one fp32 add per iteration: the whole multi-iteration run is
     deterministic and must equal the oracle's fp32 run bit for bit, through
     the hipGraph replay path."""
-    say(f"case: w=1 RBAS graph run 20 iterations mode={mode}")
+    say(f"case: w=1 RBAS graph run 20 iterations mode={mode} path={path}")
     xy = instances.uniform(120, 1000, seed=5)
     dist = orc.dist_from_int_coords(xy)
     p = capi.default_params(n=120, m=30, k=12, w=1, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS,
-                            mode=mode, seed=3, tau0=0.01)
+                            mode=mode, seed=3, tau0=0.01, construct_path=path)
     with make(p, xy=xy) as col:
         col.run(20)
         _, gb, _ = col.stats()
@@ -193,13 +207,14 @@ def test_tau0_policies():
 
 
 def test_c2_full_size_properties():
-    say("case: C2 full size n=1000 m=1000 k=20 RBAS+aging, 30 iterations")
+    say("case: C2 full size n=1000 m=1000 k=20 RBAS+aging, 80 iterations (past convergence)")
     xy = instances.clustered(1000, seed=2024)
     dist = orc.dist_from_int_coords(xy)
     p = capi.default_params(n=1000, m=1000, k=20, w=6, beta=3.0, rho=0.1, variant=capi.VARIANT_RBAS,
                             aging=1, tau0_policy=capi.TAU0_RBAS, seed=1)
     with make(p, xy=xy) as col:
-        col.run(30)
+        say(f"config {col.config()}")
+        col.run(80)
         ib, gb, nv = col.stats()
         best = col.best_tour()
         assert sorted(best.tolist()) == list(range(1000))
