@@ -42,8 +42,32 @@ def build_lib() -> str:
     return LIB
 
 
+APP_SOURCES = ["colony/Colony.cpp", "colony/AntSystem.cpp", "colony/RankBasedAntSystem.cpp",
+               "app/Commandline.cpp", "app/TSPReader.cpp", "app/Writer.cpp", "app/TSP.cpp", "app/main.cpp"]
+# program names of the reference's build variants (release/makefile:52-59)
+APP_VARIANTS = ["acouav_gpu_aging_broker", "acouav_gpu_aging_nobroker", "acouav_gpu_noaging_broker",
+                "acouav_gpu_noaging_nobroker"]
+
+
+def build_app() -> str:
+    """acouav: the C++ Colony classes and command line over the C ABI."""
+    exe = os.path.join(OUT, "acouav")
+    srcs = [os.path.join(CSRC, s) for s in APP_SOURCES]
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(CSRC, "colony"), "-I", os.path.join(CSRC, "app"), *srcs,
+          "-L", OUT, "-lacogpu_amd", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
+    for v in APP_VARIANTS:
+        link = os.path.join(OUT, v)
+        if os.path.lexists(link):
+            os.remove(link)
+        os.symlink("acouav", link)
+    return exe
+
+
 def build_all() -> None:
     build_lib()
+    build_app()
 
 
 if __name__ == "__main__":

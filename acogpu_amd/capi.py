@@ -49,6 +49,7 @@ SIGNATURES = {
     "aco_initialize": (ctypes.c_int, [_vp]),
     "aco_iterate": (ctypes.c_int, [_vp, _vp]),
     "aco_run": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_phase": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_get_stats": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "aco_get_best_tour": (ctypes.c_int, [_vp, _vp]),
     "aco_get_tours": (ctypes.c_int, [_vp, _vp]),
@@ -146,6 +147,11 @@ class Colony:
         t = np.zeros(6, dtype=np.float32)
         self._chk(self._L.aco_iterate(self._ctx, _p(t)))
         return t
+
+    def phase(self, phase):
+        ms = ctypes.c_float()
+        self._chk(self._L.aco_phase(self._ctx, int(phase), ctypes.byref(ms)))
+        return ms.value
 
     def run(self, iterations):
         ms = ctypes.c_float()

@@ -48,6 +48,7 @@ struct aco_ctx {
     DevState* d_state = nullptr;
     int64_t* d_scratch64 = nullptr;
     int32_t cur = 0;           // index of the current pheromone buffer
+    int32_t next_phase = 0;    // aco_phase ordering
     bool initialized = false;
     hipGraphExec_t graph = nullptr;  // two iterations starting at cur == 0
     hipEvent_t ev[6] = {};
@@ -353,6 +354,7 @@ extern "C" int aco_initialize(aco_ctx* c) {
         c->p.tau0 = tau0;
     }
     c->cur = 0;
+    c->next_phase = ACO_PHASE_PROBABILITIES;
     HIP_TRY(c, aco::launch_fill_rows(c->d_tau[0], c->n, c->P, tau0, c->stream));
     HIP_TRY(c, aco::launch_fill_rows(c->d_tau[1], c->n, c->P, 0.0f, c->stream));
     HIP_TRY(c, aco::launch_nn_lists(c->d_dist, c->n, c->P, c->nn_cols, c->d_nn, c->stream));
@@ -369,15 +371,16 @@ extern "C" int aco_initialize(aco_ctx* c) {
     return ACO_OK;
 }
 
-// The four kernels of one iteration on the ctx stream, reading pheromone
-// buffer `cur` and producing buffer 1-cur.  ev (may be null) brackets them.
-static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
+// The kernels of one iteration, one function per phase, on the ctx stream;
+// they read pheromone buffer `cur` and produce buffer 1-cur.
+static hipError_t enqueue_sweep(aco_ctx* c, int cur) {
     hipError_t e;
     float* tau = c->d_tau[cur];
     float* tau_next = c->d_tau[1 - cur];
     const bool cand = c->p.mode == ACO_MODE_CANDIDATE;
-    if (ev) hipEventRecord(ev[0], c->stream);
-
+    (void)tau;
+    (void)tau_next;
+    (void)cand;
     aco::SweepArgs sa{};
     sa.tau = tau;
     sa.dist = c->d_dist;
@@ -398,8 +401,17 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     sa.one_minus_rho = 1.0f - c->p.rho;
     sa.want_mean = c->p.aging;
     if ((e = aco::launch_sweep(sa, c->stream)) != hipSuccess) return e;
-    if (ev) hipEventRecord(ev[1], c->stream);
+    return hipSuccess;
+}
 
+static hipError_t enqueue_construct(aco_ctx* c, int cur) {
+    hipError_t e;
+    float* tau = c->d_tau[cur];
+    float* tau_next = c->d_tau[1 - cur];
+    const bool cand = c->p.mode == ACO_MODE_CANDIDATE;
+    (void)tau;
+    (void)tau_next;
+    (void)cand;
     aco::ConstructArgs ca{};
     ca.nn = c->d_nn;
     ca.nn16 = c->d_nn16;
@@ -430,8 +442,17 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     ca.words = c->words;
     ca.young_blocks = c->young_blocks;
     if ((e = aco::launch_construct(ca, c->path, c->apb, c->stream)) != hipSuccess) return e;
-    if (ev) hipEventRecord(ev[2], c->stream);
+    return hipSuccess;
+}
 
+static hipError_t enqueue_stats(aco_ctx* c, int cur) {
+    hipError_t e;
+    float* tau = c->d_tau[cur];
+    float* tau_next = c->d_tau[1 - cur];
+    const bool cand = c->p.mode == ACO_MODE_CANDIDATE;
+    (void)tau;
+    (void)tau_next;
+    (void)cand;
     aco::StatsArgs st{};
     st.tours = c->d_tours;
     st.lengths = c->d_len;
@@ -452,8 +473,17 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     st.global_perf = c->p.global_perf;
     st.min_pop = c->p.min_pop;
     if ((e = aco::launch_stats(st, c->stream)) != hipSuccess) return e;
-    if (ev) hipEventRecord(ev[3], c->stream);
+    return hipSuccess;
+}
 
+static hipError_t enqueue_deposit(aco_ctx* c, int cur) {
+    hipError_t e;
+    float* tau = c->d_tau[cur];
+    float* tau_next = c->d_tau[1 - cur];
+    const bool cand = c->p.mode == ACO_MODE_CANDIDATE;
+    (void)tau;
+    (void)tau_next;
+    (void)cand;
     aco::DepositArgs da{};
     da.tours = c->d_tours;
     da.best_tour = c->d_best_tour;
@@ -466,13 +496,36 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     da.dense = c->p.variant == ACO_VARIANT_AS;
     const int32_t grid = da.dense ? c->m : c->p.w + 1;
     if ((e = aco::launch_deposit(da, grid, c->stream)) != hipSuccess) return e;
-    if (ev) hipEventRecord(ev[4], c->stream);
+    return hipSuccess;
+}
+
+static hipError_t enqueue_phase(aco_ctx* c, int cur, int phase) {
+    switch (phase) {
+        case ACO_PHASE_PROBABILITIES: return enqueue_sweep(c, cur);
+        case ACO_PHASE_TOUR: return enqueue_construct(c, cur);
+        case ACO_PHASE_ANT_DISTANCE: return enqueue_stats(c, cur);
+        case ACO_PHASE_UPDATE_PHEROMONE: return enqueue_deposit(c, cur);
+        default: return hipSuccess;  // evaporation is part of the probability sweep
+    }
+}
+
+// A whole iteration; ev (may be null) brackets the four kernels.
+static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
+    static const int order[4] = {ACO_PHASE_PROBABILITIES, ACO_PHASE_TOUR, ACO_PHASE_ANT_DISTANCE,
+                                 ACO_PHASE_UPDATE_PHEROMONE};
+    hipError_t e;
+    if (ev && (e = hipEventRecord(ev[0], c->stream)) != hipSuccess) return e;
+    for (int i = 0; i < 4; ++i) {
+        if ((e = enqueue_phase(c, cur, order[i])) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[i + 1], c->stream)) != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 
 extern "C" int aco_iterate(aco_ctx* c, float* timers) {
     if (!c) return ACO_E_ARG;
     if (!c->initialized) return fail(c, ACO_E_STATE, "aco_iterate before aco_initialize");
+    if (c->next_phase != ACO_PHASE_PROBABILITIES) return fail(c, ACO_E_STATE, "aco_iterate inside a phased iteration");
     HIP_TRY(c, enqueue_iteration(c, c->cur, c->ev));
     c->cur ^= 1;
     HIP_TRY(c, hipEventSynchronize(c->ev[4]));
@@ -513,6 +566,7 @@ extern "C" int aco_run(aco_ctx* c, int32_t iterations, float* elapsed_ms) {
     if (!c) return ACO_E_ARG;
     if (!c->initialized) return fail(c, ACO_E_STATE, "aco_run before aco_initialize");
     if (iterations < 0) return fail(c, ACO_E_ARG, "negative iteration count");
+    if (c->next_phase != ACO_PHASE_PROBABILITIES) return fail(c, ACO_E_STATE, "aco_run inside a phased iteration");
     if (!c->graph) {
         int rc = build_graph(c);
         if (rc) return rc;
@@ -739,5 +793,30 @@ extern "C" int aco_get_config(aco_ctx* c, int32_t* config) {
     config[1] = c->apb;
     config[2] = c->young_blocks;
     config[3] = c->max_lds;
+    return ACO_OK;
+}
+
+// Phase-by-phase iteration for hosts that drive Colony::constructSolution's
+// virtual steps themselves (src/Colony.cu:537-594).  Phases must come in the
+// order probabilities, tour, ant distances, [evaporate], update; `update`
+// completes the iteration (pheromone buffers swap).
+extern "C" int aco_phase(aco_ctx* c, int32_t phase, float* ms) {
+    if (!c) return ACO_E_ARG;
+    if (!c->initialized) return fail(c, ACO_E_STATE, "aco_phase before aco_initialize");
+    static const int next_of[5] = {ACO_PHASE_TOUR, ACO_PHASE_ANT_DISTANCE, ACO_PHASE_EVAPORATE,
+                                   ACO_PHASE_UPDATE_PHEROMONE, ACO_PHASE_PROBABILITIES};
+    if (phase < 0 || phase > 4) return fail(c, ACO_E_ARG, "unknown phase %d", phase);
+    if (phase != c->next_phase) {
+        // evaporation may be skipped: it is already folded into the sweep
+        if (!(c->next_phase == ACO_PHASE_EVAPORATE && phase == ACO_PHASE_UPDATE_PHEROMONE))
+            return fail(c, ACO_E_STATE, "phase %d out of order (expected %d)", phase, c->next_phase);
+    }
+    HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+    HIP_TRY(c, enqueue_phase(c, c->cur, phase));
+    HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->ev[1]));
+    if (ms) HIP_TRY(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+    c->next_phase = next_of[phase];
+    if (phase == ACO_PHASE_UPDATE_PHEROMONE) c->cur ^= 1;
     return ACO_OK;
 }

@@ -1,0 +1,156 @@
+// Colony over the C ABI.  The iteration order of constructSolution follows
+// src/Colony.cu:537-594; each step is one device phase (aco_phase).
+#include "Colony.h"
+
+#include <cstdio>
+#include <cstdlib>
+
+static const long kMaxDistance = 0x7fffffff;
+
+Colony::Colony() { aco_params_init(&params); }
+
+Colony::Colony(int* d, int cities, int ants) : distances(d), numCities(cities), numAnts(ants) {
+    aco_params_init(&params);
+    params.n = cities;
+    params.m = ants;
+    params.mode = ACO_MODE_FULLROW;  // the reference's candidate set: every city in NN order
+    if (d == nullptr || cities <= 0 || ants <= 0) {
+        std::fprintf(stderr, "Colony: invalid arguments (distances=%p, cities=%d, ants=%d)\n", (void*)d, cities,
+                     ants);
+        std::exit(EXIT_FAILURE);
+    }
+}
+
+Colony::~Colony() {
+    if (ctx) aco_destroy(ctx);
+}
+
+void Colony::check(int rc) const {
+    if (rc != ACO_OK) {
+        std::fprintf(stderr, "acogpu error %d: %s\n", rc, aco_last_error(ctx));
+        std::exit(EXIT_FAILURE);
+    }
+}
+
+void Colony::initialize() {
+    computeParameters();
+    params.tau0 = initialPheromone;
+    if (ctx) {
+        aco_destroy(ctx);
+        ctx = nullptr;
+    }
+    if (params.mode == ACO_MODE_FULLROW && params.n > 8192) {
+        params.mode = ACO_MODE_CANDIDATE;  // the full row does not fit the device path
+        if (params.k <= 0 || params.k > 64) params.k = 32;
+    }
+    int rc = aco_create(&params, distances, nullptr, &ctx);
+    if (rc != ACO_OK) {
+        std::fprintf(stderr, "acogpu error %d: %s\n", rc, aco_last_error(nullptr));
+        std::exit(EXIT_FAILURE);
+    }
+    check(aco_initialize(ctx));
+}
+
+float Colony::runPhase(int phase) {
+    if (!ctx) {
+        std::fprintf(stderr, "Colony: iteration before initialize()\n");
+        std::exit(EXIT_FAILURE);
+    }
+    float ms = 0.0f;
+    check(aco_phase(ctx, phase, &ms));
+    lastPhaseMs = ms;
+    return ms;
+}
+
+void Colony::constructSolution(float timers[]) {
+    float t[TOTAL_TIMERS] = {0, 0, 0, 0, 0, 0};
+    // initTour (placement) and updateAntAges run inside the tour kernel and
+    // the probability sweep; their slot stays 0.
+    t[TIME_COMPUTE_PROBABILITIES] = runPhase(ACO_PHASE_PROBABILITIES);
+    t[TIME_TOUR] = runPhase(ACO_PHASE_TOUR);
+    t[TIME_COMPUTE_ANT_DISTANCE] = runPhase(ACO_PHASE_ANT_DISTANCE);
+    // virtual steps of the subclasses (AntSystem / RankBasedAntSystem)
+    evaporatePheromones();
+    t[TIME_EVAPORATE_PHEROMONE] = lastPhaseMs;
+    updatePheromones();
+    t[TIME_UPDATE_PHEROMONE] = lastPhaseMs;
+    if (timers)
+        for (int i = 0; i < TOTAL_TIMERS; ++i) timers[i] = t[i];
+}
+
+void Colony::computeAntDistances() { runPhase(ACO_PHASE_ANT_DISTANCE); }
+
+void Colony::computeProbabilities() { runPhase(ACO_PHASE_PROBABILITIES); }
+
+void Colony::setAlpha(float a) { params.alpha = a; }
+void Colony::setRho(float r) { params.rho = r; }
+void Colony::setBeta(float b) { params.beta = b; }
+float Colony::getAlpha() { return params.alpha; }
+float Colony::getRho() { return params.rho; }
+float Colony::getBeta() { return params.beta; }
+int Colony::getNumAnts() { return numAnts; }
+int Colony::getNumCities() { return numCities; }
+int Colony::getReps() { return reps; }
+
+long Colony::getIterBestDist() {
+    if (!ctx) return kMaxDistance - 1;
+    int64_t ib = 0, gb = 0;
+    check(aco_get_stats(ctx, &ib, &gb, nullptr));
+    return (long)ib;
+}
+
+long Colony::getGlobBestDist() {
+    if (!ctx) return kMaxDistance;
+    int64_t ib = 0, gb = 0;
+    check(aco_get_stats(ctx, &ib, &gb, nullptr));
+    return (long)gb;
+}
+
+unsigned int Colony::getNumValidAnts() {
+    if (!ctx) return (unsigned)numAnts;
+    uint32_t nv = 0;
+    check(aco_get_stats(ctx, nullptr, nullptr, &nv));
+    return nv;
+}
+
+std::string Colony::getTour() {
+    std::string out;
+    if (!ctx) return out;
+    int32_t* t = new int32_t[numCities];
+    check(aco_get_best_tour(ctx, t));
+    for (int i = 0; i < numCities; ++i) {
+        out += std::to_string(t[i] + 1);
+        out += ',';
+    }
+    delete[] t;
+    return out;
+}
+
+float Colony::greedyDistance() { return 0.0f; }  // C_nn is computed on the device by the tau0 policies
+
+int Colony::getMinRetirementAge() { return params.min_age; }
+void Colony::setMinRetirementAge(int v) { params.min_age = v; }
+int Colony::getMaxRetirementAge() { return params.max_age; }
+void Colony::setMaxRetirementAge(int v) { params.max_age = v; }
+int Colony::getLocalPeformance() { return params.local_perf; }
+void Colony::setLocalPeformance(int v) { params.local_perf = v; }
+int Colony::getGlobalPerformance() { return params.global_perf; }
+void Colony::setGlobalPerformance(int v) { params.global_perf = v; }
+int Colony::getMinPopulationLimit() { return params.min_pop; }
+void Colony::setMinPopulationLimit(int v) { params.min_pop = v; }
+int Colony::getMaxPopulationLimit() { return params.max_pop; }
+void Colony::setMaxPopulationLimit(int v) { params.max_pop = v; }
+
+void Colony::setAging(bool on) { params.aging = on ? 1 : 0; }
+bool Colony::getAging() { return params.aging != 0; }
+void Colony::setBroker(bool on) { params.broker = on ? 1 : 0; }
+void Colony::setSeed(uint64_t seed) { params.seed = seed; }
+
+void Colony::setCandidateListLength(int k) {
+    if (k <= 0) {
+        params.mode = ACO_MODE_FULLROW;
+    } else {
+        params.mode = ACO_MODE_CANDIDATE;
+        params.k = k;
+    }
+}

@@ -101,6 +101,21 @@ int aco_initialize(aco_ctx *c);
  * times in milliseconds (timers may be NULL). */
 int aco_iterate(aco_ctx *c, float *timers);
 
+/* One phase of Colony::constructSolution (src/Colony.cu:537-594), for hosts
+ * that drive the virtual steps themselves.  Order per iteration:
+ * PROBABILITIES (computeProbabilities: choice_info, row means, and the
+ * evaporated pheromone of AntSystem::evaporatePheromones written to the next
+ * buffer), TOUR (initTour + constructPath), ANT_DISTANCE
+ * (computeAntDistances), EVAPORATE (no kernel: folded into PROBABILITIES;
+ * may be skipped), UPDATE_PHEROMONE (updatePheromones + aging
+ * retire/recruit).  ms (may be NULL) gets the phase's HIP-event time. */
+#define ACO_PHASE_PROBABILITIES 0
+#define ACO_PHASE_TOUR 1
+#define ACO_PHASE_ANT_DISTANCE 2
+#define ACO_PHASE_EVAPORATE 3
+#define ACO_PHASE_UPDATE_PHEROMONE 4
+int aco_phase(aco_ctx *c, int32_t phase, float *ms);
+
 /* TSP::execute's loop body for `iterations` iterations with no host
  * synchronisation inside (hipGraph replay).  elapsed_ms (may be NULL) gets the
  * HIP-event time of the whole loop. */
