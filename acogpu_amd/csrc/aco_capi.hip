@@ -143,8 +143,8 @@ static int check_params(const aco_params* p) {
     if (p->variant == ACO_VARIANT_RBAS && p->w < 1) return fail(nullptr, ACO_E_ARG, "w must be >= 1");
     if (p->aging && (p->min_age < 1 || p->max_age < 0))
         return fail(nullptr, ACO_E_ARG, "aging needs min_age >= 1 and max_age >= 0");
-    if (p->ants_per_block < 0 || p->ants_per_block > 4)
-        return fail(nullptr, ACO_E_ARG, "ants_per_block must be in [0, 4]");
+    if (p->ants_per_block < 0 || p->ants_per_block > 8)
+        return fail(nullptr, ACO_E_ARG, "ants_per_block must be in [0, 8]");
     if (p->construct_path < 0 || p->construct_path > 2)
         return fail(nullptr, ACO_E_ARG, "construct_path must be 0 (auto), 1 (global) or 2 (LDS-staged)");
     return ACO_OK;
@@ -189,7 +189,20 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
                 return rc2;
             }
         } else {
-            const int32_t want = p->ants_per_block ? p->ants_per_block : 4;
+            // Staged path: one block per CU (the instance fills most of the
+            // LDS); choose the ants per block so that the normal blocks plus
+            // the blocks reserved for young aging ants fit the CUs in one pass.
+            int cus = 256;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+            (void)hipGetLastError();
+            const int32_t young = p->aging ? std::min(c->m, std::max(p->min_age, 1)) : 0;
+            // Measured on MI355X (n=1000, m=1000, k=20): 8 ants per block is
+            // fastest; fewer only when the colony is too small to fill the CUs.
+            int32_t want = p->ants_per_block;
+            if (!want) {
+                want = 8;
+                while (want > 1 && (c->m + want - 1) / want + young < cus / 2) --want;
+            }
             int32_t W = want;
             while (W > 1 && aco::staged_lds_bytes(c->n, c->k, c->words, W) > (size_t)max_lds) --W;
             const bool fits = c->n <= 65535 && aco::staged_lds_bytes(c->n, c->k, c->words, W) <= (size_t)max_lds;
@@ -414,7 +427,7 @@ static hipError_t enqueue_construct(aco_ctx* c, int cur) {
     (void)cand;
     aco::ConstructArgs ca{};
     ca.nn = c->d_nn;
-    ca.nn16 = c->d_nn16;
+    ca.nn16 = c->n <= 65535 ? c->d_nn16 : nullptr;
     ca.choice_nn = c->d_choice_nn;
     ca.tau_nn = c->d_tau_nn;
     ca.eta_nn = c->d_eta_nn;

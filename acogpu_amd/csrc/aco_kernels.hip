@@ -17,9 +17,11 @@
 
 namespace aco {
 
-__device__ __forceinline__ float apply_alpha(float p, float alpha) {
-    return alpha == 1.0f ? p : (float)pow((double)p, (double)alpha);
-}
+// tau^alpha rounded once from fp64 (the oracle rounds the same fp64 power);
+// out of line so that the alpha == 1 fast path stays small.
+__device__ __noinline__ float pow_alpha(float p, float alpha) { return (float)pow((double)p, (double)alpha); }
+
+__device__ __forceinline__ float apply_alpha(float p, float alpha) { return alpha == 1.0f ? p : pow_alpha(p, alpha); }
 
 // ---------------------------------------------------------------------------
 // EUC_2D, exact integer form of (int)(sqrt(dx^2+dy^2)+0.5) (Utils::
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(256) void k_fill_rows(float* __restrict__ a, int32_
 // reference, src/Colony.cu:1053-1055).  One block; the visited bitmap in LDS.
 __global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dist, int32_t n, int32_t P,
                                                  int32_t start, int64_t* __restrict__ out) {
-    extern __shared__ uint32_t sh[];
+    extern __shared__ __align__(16) uint32_t sh[];
     uint32_t* vis = sh;
     const int32_t words = (n + 31) / 32;
     uint64_t* red = reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1));
@@ -234,28 +236,40 @@ __device__ __forceinline__ void stage_city(int32_t* tour, int32_t& tbuf, int32_t
     }
 }
 
-// Random words of one ant, 256 steps per refill: lane l holds the Philox
-// block (base + l), i.e. the words of steps 4(base+l) .. 4(base+l)+3.
+// Random words of one ant.  Lane l holds the four Philox words of block
+// 64*(s/256) + l; step s uses word (s/64)%4 of lane s%64, so a step costs one
+// lane read, the word switches every 64 steps and a refill every 256.
 struct AntRng {
     uint32_t w0, w1, w2, w3;
-    int32_t base;
+    uint32_t cur;
 };
 
 __device__ __forceinline__ void rng_refill(AntRng& r, int32_t s, uint32_t ant, uint32_t iter, const ConstructArgs& a,
                                            int lane) {
-    r.base = s >> 2;
-    const Philox4 q = philox4x32_10((uint32_t)(r.base + lane), ant, iter, a.stream, a.seed_lo, a.seed_hi);
+    const uint32_t blk = (uint32_t)((s >> 8) << 6) + (uint32_t)lane;
+    const Philox4 q = philox4x32_10(blk, ant, iter, a.stream, a.seed_lo, a.seed_hi);
     r.w0 = q.x0;
     r.w1 = q.x1;
     r.w2 = q.x2;
     r.w3 = q.x3;
+    r.cur = q.x0;
+}
+
+// Before using step s >= 1 (uniform branch, taken every 64 steps).
+__device__ __forceinline__ void rng_step(AntRng& r, int32_t s, uint32_t ant, uint32_t iter, const ConstructArgs& a,
+                                         int lane) {
+    if ((s & 63) != 0) return;
+    if ((s & 255) == 0) {
+        rng_refill(r, s, ant, iter, a, lane);
+    } else {  // words rotate: w1 serves steps 64..127 of the chunk, w2 the next 64, ...
+        r.cur = r.w1;
+        r.w1 = r.w2;
+        r.w2 = r.w3;
+    }
 }
 
 __device__ __forceinline__ uint32_t rng_word(const AntRng& r, int32_t s) {
-    const int L = (s >> 2) - r.base;
-    const int q = s & 3;
-    const uint32_t v = q == 0 ? r.w0 : (q == 1 ? r.w1 : (q == 2 ? r.w2 : r.w3));
-    return (uint32_t)readlane_i((int)v, L);
+    return (uint32_t)readlane_i((int)r.cur, s & 63);
 }
 
 // Roulette over one wave of candidate weights: inclusive DPP scan, r = u *
@@ -275,27 +289,36 @@ __device__ __forceinline__ int select_lane(float w, float u) {
 // choose_best_next: argmax of the selection weight over every unvisited city,
 // ties to the lowest index.  Lanes stride the row four loads at a time; exact
 // comparisons make the result independent of the reduction order.
-__device__ __noinline__ int32_t fallback_argmax(const ConstructArgs& a, const uint32_t* vis, int32_t c,
+__device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const uint32_t* vis, int32_t c,
                                                 bool young, float ratio, int lane, unsigned long long& scanned) {
     float best = -1.0f;
     int32_t bj = 0x7fffffff;
     const float* crow = a.choice + (size_t)c * a.P;
     if (!young) {
-        for (int32_t j0 = 0; j0 < a.n; j0 += 256) {
-            float x[4];
+        // 16-byte coalesced loads, 8 per lane in flight (2048 cities per
+        // batch); the row is padded to a multiple of 64 floats.
+        for (int32_t j0 = 0; j0 < a.n; j0 += 2048) {
+            float4 x[8];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int32_t j = j0 + q * 64 + lane;
-                x[q] = j < a.n ? crow[j] : 0.0f;
+            for (int q = 0; q < 8; ++q) {
+                const int32_t j = j0 + 256 * q + 4 * lane;
+                x[q] = j < a.n ? *reinterpret_cast<const float4*>(crow + j) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int32_t j = j0 + q * 64 + lane;
-                if (j < a.n && !is_visited(vis, j)) {
-                    ++scanned;
-                    if (x[q] > best) {
-                        best = x[q];
-                        bj = j;
+            for (int q = 0; q < 8; ++q) {
+                const int32_t j = j0 + 256 * q + 4 * lane;
+                if (j < a.n) {
+                    const uint32_t word = vis[j >> 5] >> (j & 31);  // 4 cities in one word
+                    const float xv[4] = {x[q].x, x[q].y, x[q].z, x[q].w};
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        if (j + t < a.n && !((word >> t) & 1u)) {
+                            ++scanned;
+                            if (xv[t] > best) {
+                                best = xv[t];
+                                bj = j + t;
+                            }
+                        }
                     }
                 }
             }
@@ -355,8 +378,8 @@ __device__ __forceinline__ bool ant_is_young(const ConstructArgs& a, int32_t ant
 // zero total falls back to the argmax over the unvisited cities.  The tour
 // length is accumulated exactly in int64, one step behind (the distance load
 // completes under the next step's candidate loads).
-__global__ __launch_bounds__(256) void k_construct_cand(const ConstructArgs a) {
-    extern __shared__ uint32_t lds[];
+__global__ __launch_bounds__(512) void k_construct_cand(const ConstructArgs a) {
+    extern __shared__ __align__(16) uint32_t lds[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int32_t ant = blockIdx.x * (blockDim.x >> 6) + wave;
@@ -380,13 +403,13 @@ __global__ __launch_bounds__(256) void k_construct_cand(const ConstructArgs a) {
     int32_t c = start;
     unsigned long long fb = 0, scanned = 0;
     for (int32_t s = 1; s < a.n; ++s) {
-        if ((s & 255) == 0) rng_refill(rng, s, (uint32_t)ant, iter, a, lane);
+        rng_step(rng, s, (uint32_t)ant, iter, a, lane);
         const float u = word_to_uniform(rng_word(rng, s));
         int32_t j = 0;
         float w = 0.0f;
         if (lane < a.k) {
             const size_t e = (size_t)c * a.k + lane;
-            j = a.nn[e];
+            j = a.nn16 != nullptr ? (int32_t)a.nn16[e] : a.nn[e];
             const float cw = young ? young_formula(a.tau_nn[e], a.mean[c], ratio, a.eta_nn[e], a.alpha)
                                    : a.choice_nn[e];
             w = is_visited(vis, j) ? 0.0f : cw;
@@ -422,38 +445,167 @@ __global__ __launch_bounds__(256) void k_construct_cand(const ConstructArgs a) {
 // stage that ant's pulled-toward-the-mean weights instead (they differ per
 // age), so young ants also run at LDS speed.  Tours are kept in LDS (16-bit)
 // and written, with their exact lengths, after the last step.
+// Wave-64 prefix sum restricted to 32 lanes: the first five steps of
+// wave_inclusive_scan.  When lanes 32..63 hold zeros it yields the same bits
+// in lanes 0..31 as the full scan, and lane 31 equals the full scan's lane 63.
+__device__ __forceinline__ float wave_inclusive_scan32(float v) {
+    v = v + dpp_src<0x111, 0xf>(v);
+    v = v + dpp_src<0x112, 0xf>(v);
+    v = v + dpp_src<0x114, 0xf>(v);
+    v = v + dpp_src<0x118, 0xf>(v);
+    v = v + dpp_src<0x142, 0xa>(v);
+    return v;
+}
+
+// Argmax fallback for the register bitmap layout: lane l owns cities
+// 32l .. 32l+31 (its bitmap word), reads them as eight 16-byte loads and
+// keeps the largest weight (lowest index on ties).
+__device__ __forceinline__ int32_t fallback_argmax_vreg(const ConstructArgs& a, uint32_t vword, int32_t c, bool young,
+                                                        float ratio, int lane, unsigned long long& scanned) {
+    float best = -1.0f;
+    int32_t bj = 0x7fffffff;
+    const int32_t j0 = 32 * lane;
+    if (j0 < a.n) {
+        uint32_t free_bits = ~vword;
+        if (a.n - j0 < 32) free_bits &= (1u << (a.n - j0)) - 1u;
+        scanned += __popc(free_bits);
+        const size_t row = (size_t)c * a.P + j0;
+        if (!young) {
+            float x[32];
+            const float4* src = reinterpret_cast<const float4*>(a.choice + row);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float4 v = src[q];
+                x[4 * q] = v.x;
+                x[4 * q + 1] = v.y;
+                x[4 * q + 2] = v.z;
+                x[4 * q + 3] = v.w;
+            }
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                if (((free_bits >> q) & 1u) && x[q] > best) {
+                    best = x[q];
+                    bj = j0 + q;
+                }
+            }
+        } else {
+            // all loads first (16-byte tau and dist, then the heuristic
+            // gathers), then the pulled-toward-the-mean weights
+            const float mc = a.mean[c];
+            float t[32];
+            int32_t d[32];
+            const float4* t4 = reinterpret_cast<const float4*>(a.tau + row);
+            const int4* d4 = reinterpret_cast<const int4*>(a.dist + row);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float4 tv = t4[q];
+                const int4 dv = d4[q];
+                t[4 * q] = tv.x;
+                t[4 * q + 1] = tv.y;
+                t[4 * q + 2] = tv.z;
+                t[4 * q + 3] = tv.w;
+                d[4 * q] = dv.x;
+                d[4 * q + 1] = dv.y;
+                d[4 * q + 2] = dv.z;
+                d[4 * q + 3] = dv.w;
+            }
+            float e[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) e[q] = eta_of(a.eta, a.eta_len, d[q]);
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                if ((free_bits >> q) & 1u) {
+                    const float x = young_formula(t[q], mc, ratio, e[q], a.alpha);
+                    if (x > best) {
+                        best = x;
+                        bj = j0 + q;
+                    }
+                }
+            }
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float ob = __shfl_xor(best, off);
+        const int32_t oj = __shfl_xor(bj, off);
+        if (ob > best || (ob == best && oj < bj)) {
+            best = ob;
+            bj = oj;
+        }
+    }
+    int32_t next = uniform_i(bj);
+    if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
+        uint32_t free_bits = (32 * lane < a.n) ? ~vword : 0u;
+        if (32 * lane < a.n && a.n - 32 * lane < 32) free_bits &= (1u << (a.n - 32 * lane)) - 1u;
+        const unsigned long long m = __ballot(free_bits != 0u);
+        const int l = __ffsll((long long)m) - 1;
+        next = 32 * l + __ffs(readlane_i((int)free_bits, l)) - 1;
+    }
+    return next;
+}
+
+// One ant on LDS-staged candidate data.  VREG_VIS: the visited set lives in
+// one VGPR per lane (lane l holds cities 32l..32l+31; n <= 2048) and is read
+// across lanes with ds_bpermute; otherwise an LDS bitmap.  K32: k <= 32, the
+// roulette scan stops at 32 lanes.  The common step is: two LDS reads, the
+// visited test, the scan, one ballot and one lane read.
+template <bool VREG_VIS, bool K32>
 __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int32_t ant, bool young, float ratio,
-                                                     const float* wl, const uint16_t* nl, uint32_t* vis,
+                                                     const float* wl, const uint16_t* nl, uint32_t* vis_lds,
                                                      uint16_t* t16, uint32_t iter, int lane) {
-    for (int32_t w = lane; w < a.words; w += 64) vis[w] = 0u;
+    uint32_t vreg = 0u;
+    if (!VREG_VIS)
+        for (int32_t w = lane; w < a.words; w += 64) vis_lds[w] = 0u;
     AntRng rng;
     rng_refill(rng, 0, (uint32_t)ant, iter, a, lane);
     const int32_t start = word_to_city(rng_word(rng, 0), a.n);
-    mark_visited(vis, start, lane);
-    if (lane == 0) t16[0] = (uint16_t)start;
+    if (VREG_VIS) {
+        vreg |= (lane == (start >> 5)) ? (1u << (start & 31)) : 0u;
+    } else {
+        mark_visited(vis_lds, start, lane);
+    }
+    const bool klane = lane < a.k;
+    const int32_t rl = klane ? lane : a.k - 1;
+    int32_t tbuf = start;
     int32_t c = start;
     unsigned long long fb = 0, scanned = 0;
     for (int32_t s = 1; s < a.n; ++s) {
-        if ((s & 255) == 0) rng_refill(rng, s, (uint32_t)ant, iter, a, lane);
+        rng_step(rng, s, (uint32_t)ant, iter, a, lane);
         const float u = word_to_uniform(rng_word(rng, s));
-        int32_t j = 0;
-        float w = 0.0f;
-        if (lane < a.k) {
-            const int32_t e = c * a.k + lane;
-            j = nl[e];
-            const float cw = wl[e];
-            w = is_visited(vis, j) ? 0.0f : cw;
+        const int32_t e = c * a.k + rl;
+        const int32_t j = nl[e];
+        const float cw = wl[e];
+        bool vj;
+        if (VREG_VIS) {
+            const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((j >> 5) << 2, (int)vreg);
+            vj = (word >> (j & 31)) & 1u;
+        } else {
+            vj = is_visited(vis_lds, j);
         }
-        const int sel = select_lane(w, u);
+        const float w = (klane && !vj) ? cw : 0.0f;
+        const float p = K32 ? wave_inclusive_scan32(w) : wave_inclusive_scan(w);
+        const float total = readlane_f(p, K32 ? 31 : 63);
+        const float rr = u * total;
+        const unsigned long long hit = __ballot(p > rr && w > 0.0f);
         int32_t next;
-        if (sel >= 0) {
-            next = readlane_i(j, sel);
+        if (hit) {
+            next = readlane_i(j, __ffsll((long long)hit) - 1);
+        } else if (total > 0.0f) {
+            const unsigned long long pos = __ballot(w > 0.0f);
+            next = readlane_i(j, 63 - __clzll((long long)pos));
         } else {
             ++fb;
-            next = fallback_argmax(a, vis, c, young, ratio, lane, scanned);
+            next = VREG_VIS ? fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned)
+                            : fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned);
         }
-        mark_visited(vis, next, lane);
-        if (lane == 0) t16[s] = (uint16_t)next;
+        if (VREG_VIS) {
+            vreg |= (lane == (next >> 5)) ? (1u << (next & 31)) : 0u;
+        } else {
+            mark_visited(vis_lds, next, lane);
+        }
+        tbuf = (lane == (s & 63)) ? next : tbuf;
+        if ((s & 63) == 63 || s == a.n - 1) {
+            if (lane <= (s & 63)) t16[(s & ~63) + lane] = (uint16_t)tbuf;
+        }
         c = next;
     }
     finish_counters(a, fb, scanned, lane);
@@ -474,61 +626,135 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
     if (lane == 0) a.lengths[ant] = len;
 }
 
-__global__ __launch_bounds__(256) void k_construct_staged(const ConstructArgs a) {
-    extern __shared__ uint32_t lds[];
+// Dispatch on the register-bitmap and 32-lane cases (wave-uniform).
+__device__ __forceinline__ void construct_staged_dispatch(const ConstructArgs& a, int32_t ant, bool young,
+                                                          float ratio, const float* wl, const uint16_t* nl,
+                                                          uint32_t* vis, uint16_t* t16, uint32_t iter, int lane) {
+    if (a.n <= 2048) {
+        if (a.k <= 32)
+            construct_staged_ant<true, true>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+        else
+            construct_staged_ant<true, false>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+    } else {
+        if (a.k <= 32)
+            construct_staged_ant<false, true>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+        else
+            construct_staged_ant<false, false>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+    }
+}
+
+// Block-wide copies global -> LDS, 16 bytes per thread and load where the
+// addresses allow (hipMalloc buffers are 256-byte aligned).
+// 16-byte vectors, four loads in flight per thread before the LDS stores.
+template <typename V>
+__device__ __forceinline__ void stage_vec(V* d4, const V* __restrict__ s4, int32_t v) {
+    const int32_t stride = blockDim.x;
+    int32_t x = threadIdx.x;
+    for (; x + 3 * stride < v; x += 4 * stride) {
+        const V a0 = s4[x], a1 = s4[x + stride], a2 = s4[x + 2 * stride], a3 = s4[x + 3 * stride];
+        d4[x] = a0;
+        d4[x + stride] = a1;
+        d4[x + 2 * stride] = a2;
+        d4[x + 3 * stride] = a3;
+    }
+    for (; x < v; x += stride) d4[x] = s4[x];
+}
+
+__device__ __forceinline__ void stage_f32(float* dst, const float* __restrict__ src, int32_t count) {
+    const int32_t v = count / 4;
+    stage_vec(reinterpret_cast<float4*>(dst), reinterpret_cast<const float4*>(src), v);
+    for (int32_t x = 4 * v + threadIdx.x; x < count; x += blockDim.x) dst[x] = src[x];
+}
+
+__device__ __forceinline__ void stage_u16(uint16_t* dst, const uint16_t* __restrict__ src, int32_t count) {
+    const int32_t v = count / 8;
+    stage_vec(reinterpret_cast<uint4*>(dst), reinterpret_cast<const uint4*>(src), v);
+    for (int32_t x = 8 * v + threadIdx.x; x < count; x += blockDim.x) dst[x] = src[x];
+}
+
+// Young-ant weights for the whole instance, four candidates per thread per
+// pass (16-byte tau and eta loads).
+__device__ __forceinline__ void stage_young(float* wl, const ConstructArgs& a, float ratio, int32_t nk) {
+    const int32_t v = nk / 4;
+    const float4* t4 = reinterpret_cast<const float4*>(a.tau_nn);
+    const float4* e4 = reinterpret_cast<const float4*>(a.eta_nn);
+    float4* w4 = reinterpret_cast<float4*>(wl);
+    for (int32_t x = threadIdx.x; x < v; x += blockDim.x) {
+        const float4 t = t4[x];
+        const float4 e = e4[x];
+        const int32_t r0 = (4 * x) / a.k;
+        const int32_t r1 = (4 * x + 1) / a.k;
+        const int32_t r2 = (4 * x + 2) / a.k;
+        const int32_t r3 = (4 * x + 3) / a.k;
+        float4 w;
+        w.x = young_formula(t.x, a.mean[r0], ratio, e.x, a.alpha);
+        w.y = young_formula(t.y, a.mean[r1], ratio, e.y, a.alpha);
+        w.z = young_formula(t.z, a.mean[r2], ratio, e.z, a.alpha);
+        w.w = young_formula(t.w, a.mean[r3], ratio, e.w, a.alpha);
+        w4[x] = w;
+    }
+    for (int32_t x = 4 * v + threadIdx.x; x < nk; x += blockDim.x)
+        wl[x] = young_formula(a.tau_nn[x], a.mean[x / a.k], ratio, a.eta_nn[x], a.alpha);
+}
+
+__global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a) {
+    extern __shared__ __align__(16) uint32_t lds[];
     __shared__ int32_t s_ant;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
     const int32_t nk = a.n * a.k;
+    // LDS layout: weights (nk fp32), candidate lists (nk uint16), each region
+    // padded to 16 bytes, then per wave the visited bitmap and the tour.
     float* wl = reinterpret_cast<float*>(lds);
-    uint16_t* nl = reinterpret_cast<uint16_t*>(wl + nk);
-    uint32_t* per = reinterpret_cast<uint32_t*>(nl + ((nk + 1) & ~1));
+    uint16_t* nl = reinterpret_cast<uint16_t*>(wl + ((nk + 3) & ~3));
+    uint32_t* per = reinterpret_cast<uint32_t*>(nl + ((nk + 7) & ~7));
     const int32_t per_words = a.words + (a.n + 1) / 2;
     uint32_t* vis = per + (size_t)wave * per_words;
     uint16_t* t16 = reinterpret_cast<uint16_t*>(vis + a.words);
     const uint32_t iter = a.state->iter;
     const int32_t converged = a.state->converged;
     const int32_t normal_blocks = gridDim.x - a.young_blocks;
+    const bool young_block = (int32_t)blockIdx.x >= normal_blocks;
+    if (young_block && (!a.aging || !converged)) return;
 
-    for (int32_t x = threadIdx.x; x < nk; x += blockDim.x) nl[x] = a.nn16[x];
+    stage_u16(nl, a.nn16, nk);
 
-    if ((int32_t)blockIdx.x < normal_blocks) {
-        const int32_t ant = blockIdx.x * nwaves + wave;
-        for (int32_t x = threadIdx.x; x < nk; x += blockDim.x) wl[x] = a.choice_nn[x];
-        __syncthreads();
-        if (ant >= a.m) return;
-        if (a.aging && (!a.valid[ant] || ant_is_young(a, ant, converged))) return;
-        construct_staged_ant(a, ant, false, 0.0f, wl, nl, vis, t16, iter, lane);
-        return;
-    }
-    if (!a.aging || !converged) return;
-    // Young blocks: block y takes the young ants y, y + young_blocks, ...
+    // Normal blocks run one round (W ants); young block y runs the young
+    // ants y, y + young_blocks, ... one per round on wave 0.
     for (int32_t y = blockIdx.x - normal_blocks;; y += a.young_blocks) {
-        if (wave == 0) {
-            int32_t found = -1, seen = 0;
-            for (int32_t a0 = 0; a0 < a.m && found < 0; a0 += 64) {
-                const int32_t k = a0 + lane;
-                unsigned long long mk = __ballot(k < a.m && ant_is_young(a, k, converged));
-                const int cnt = __popcll(mk);
-                if (seen + cnt > y) {
-                    for (int t = 0; t < y - seen; ++t) mk &= mk - 1;
-                    found = a0 + __ffsll((long long)mk) - 1;
+        int32_t ant;
+        float ratio = 0.0f;
+        bool active;
+        if (!young_block) {
+            ant = blockIdx.x * nwaves + wave;
+            stage_f32(wl, a.choice_nn, nk);
+            active = ant < a.m && !(a.aging && (!a.valid[ant] || ant_is_young(a, ant, converged)));
+        } else {
+            if (wave == 0) {
+                int32_t found = -1, seen = 0;
+                for (int32_t a0 = 0; a0 < a.m && found < 0; a0 += 64) {
+                    const int32_t k = a0 + lane;
+                    unsigned long long mk = __ballot(k < a.m && ant_is_young(a, k, converged));
+                    const int cnt = __popcll(mk);
+                    if (seen + cnt > y) {
+                        for (int t = 0; t < y - seen; ++t) mk &= mk - 1;
+                        found = a0 + __ffsll((long long)mk) - 1;
+                    }
+                    seen += cnt;
                 }
-                seen += cnt;
+                if (lane == 0) s_ant = found;
             }
-            if (lane == 0) s_ant = found;
+            __syncthreads();
+            ant = s_ant;
+            if (ant < 0) return;
+            ratio = (float)a.age[ant] / (float)a.min_age;
+            stage_young(wl, a, ratio, nk);
+            active = wave == 0;
         }
         __syncthreads();
-        const int32_t ant = s_ant;
-        if (ant < 0) return;
-        const float ratio = (float)a.age[ant] / (float)a.min_age;
-        for (int32_t x = threadIdx.x; x < nk; x += blockDim.x) {
-            const int32_t row = x / a.k;
-            wl[x] = young_formula(a.tau_nn[x], a.mean[row], ratio, a.eta_nn[x], a.alpha);
-        }
-        __syncthreads();
-        if (wave == 0) construct_staged_ant(a, ant, true, ratio, wl, nl, vis, t16, iter, lane);
+        if (active) construct_staged_dispatch(a, ant, young_block, ratio, wl, nl, vis, t16, iter, lane);
+        if (!young_block) return;
         __syncthreads();
     }
 }
@@ -554,8 +780,8 @@ __global__ __launch_bounds__(256) void k_nn_aux(const int32_t* __restrict__ nn, 
 // running chunk total T (sequential over chunks).  Fallback: the farthest
 // unvisited city (select_local_best inc/Colony.cuh:33-43).  LDS per ant:
 // bitmap, weights and prefixes of the whole row, chunk offsets.
-__global__ __launch_bounds__(256) void k_construct_full(const ConstructArgs a) {
-    extern __shared__ uint32_t lds[];
+__global__ __launch_bounds__(512) void k_construct_full(const ConstructArgs a) {
+    extern __shared__ __align__(16) uint32_t lds[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int32_t ant = blockIdx.x * (blockDim.x >> 6) + wave;
@@ -584,7 +810,7 @@ __global__ __launch_bounds__(256) void k_construct_full(const ConstructArgs a) {
     int32_t c = start;
     unsigned long long fb = 0;
     for (int32_t s = 1; s < a.n; ++s) {
-        if ((s & 255) == 0) rng_refill(rng, s, (uint32_t)ant, iter, a, lane);
+        rng_step(rng, s, (uint32_t)ant, iter, a, lane);
         const float u = word_to_uniform(rng_word(rng, s));
         const int32_t* nrow = a.nn + (size_t)c * a.n;
         const float* crow = a.choice + (size_t)c * a.P;
@@ -863,7 +1089,8 @@ hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
 
 size_t staged_lds_bytes(int32_t n, int32_t k, int32_t words, int32_t ants_per_block) {
     const size_t nk = (size_t)n * k;
-    return nk * 4 + ((nk + 1) & ~(size_t)1) * 2 + (size_t)ants_per_block * (words + (n + 1) / 2) * 4;
+    return ((nk + 3) & ~(size_t)3) * 4 + ((nk + 7) & ~(size_t)7) * 2 +
+           (size_t)ants_per_block * (words + (n + 1) / 2) * 4;
 }
 
 hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_per_block, hipStream_t s) {

@@ -28,6 +28,7 @@ import numpy as np  # noqa: E402
 from acogpu_amd import capi, instances  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, vendor figure
+KERNEL_NAMES = {0: "k_construct_cand", 1: "k_construct_full", 2: "k_construct_staged"}
 
 
 def log(msg):
@@ -76,6 +77,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ants-per-block", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -84,6 +87,13 @@ def main():
 
     n, m, k = 1000, 1000, 20
     xy = instances.clustered(n, clusters=10, side=10000, sigma=400.0, seed=1)
+    if args.calibrate_copy:
+        col = capi.Colony(c2_params(64, 8), xy=xy[:64])
+        ms = col.calibrate_copy(args.calibrate_copy, reps=5)
+        log(f"bench: copy of {args.calibrate_copy} bytes: {ms:.4f} ms, "
+            f"{2 * args.calibrate_copy / (ms * 1e-3) / 1e9:.1f} GB/s (read + write)")
+        col.close()
+        return 0
     p = c2_params(n, m)
     p.ants_per_block = args.ants_per_block
     log(f"bench: creating colony n={n} m={m} k={k} RBAS w=6 aging")
@@ -107,11 +117,13 @@ def main():
     kms = [float(x) for x in col.profile(args.profile_iters)]
     fb1 = col.counters()
     scanned_per_launch = float(fb1[1] - fb0[1]) / args.profile_iters
+    fbsteps_per_launch = float(fb1[0] - fb0[0]) / args.profile_iters
     nvalid = col.stats()[2]
     bytes_launch = construct_bytes_per_launch(n, k, nvalid, scanned_per_launch)
     achieved = float(bytes_launch / (kms[1] * 1e-3) / 1e9)
     log(f"bench: kernel ms sweep {kms[0]:.4f} construct {kms[1]:.4f} stats {kms[2]:.4f} "
-        f"deposit {kms[3]:.4f}; fallback cities scanned/launch {scanned_per_launch:.0f}")
+        f"deposit {kms[3]:.4f}; fallback steps/launch {fbsteps_per_launch:.0f}, "
+        f"cities scanned/launch {scanned_per_launch:.0f}")
 
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "construct_c2_traffic.json")
@@ -147,7 +159,7 @@ def main():
                    "construct_path": cfg["path"], "ants_per_block": cfg["ants_per_block"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": "k_construct_cand", "kernel_ms": float(kms[1]),
+                     "kernel": KERNEL_NAMES[cfg["path"]], "kernel_ms": float(kms[1]),
                      "bytes_per_launch": bytes_launch},
         "kernel_ms": {"sweep": float(kms[0]), "construct": float(kms[1]), "stats": float(kms[2]),
                       "deposit": float(kms[3])},

@@ -77,7 +77,7 @@ typedef struct aco_params {
     int32_t min_pop;        /* -x minPopulationLimit                          */
     int32_t max_pop;        /* -y maxPopulationLimit                          */
     int32_t broker;         /* "broker" build variant: accepted, no behaviour */
-    int32_t ants_per_block; /* construction launch shape, 0 = autotuned       */
+    int32_t ants_per_block; /* construction launch shape 1..8, 0 = automatic  */
     uint64_t seed;          /* Philox key (reference: RANDOM_SEED, inc/Colony.h:15) */
     uint32_t stream_id;     /* Philox counter word 3 (instance id)            */
     int32_t construct_path; /* candidate mode: 0 auto, 1 global memory, 2 LDS-staged */

@@ -49,16 +49,19 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[3] = c3;
 }
 
-/* One 32-bit word per (seed, iteration, ant, step, stream): counter
- * {step/4, ant, iteration, stream}, key {seed lo, seed hi}, word step%4.
- * Step 0 is the start city (src/Colony.cu:513-524), steps 1..n-1 the
- * roulette draws (src/Colony.cu:826). */
+/* One 32-bit word per (seed, iteration, ant, step, stream): key {seed lo,
+ * seed hi}, counter {block, ant, iteration, stream} with
+ * block = 64 * (step / 256) + step % 64 and output word (step / 64) % 4.
+ * (A wave computes 64 blocks at once, lane l holding block 64*(s/256) + l,
+ * and reads one lane per step.)  Step 0 is the start city
+ * (src/Colony.cu:513-524), steps 1..n-1 the roulette draws
+ * (src/Colony.cu:826). */
 uint32_t orc_rng_word(uint64_t seed, uint32_t iter, uint32_t ant, uint32_t step, uint32_t stream) {
-    uint32_t ctr[4] = {step >> 2, ant, iter, stream};
+    uint32_t ctr[4] = {((step >> 8) << 6) | (step & 63u), ant, iter, stream};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t out[4];
     orc_philox4x32_10(ctr, key, out);
-    return out[step & 3u];
+    return out[(step >> 6) & 3u];
 }
 
 /* u in [0,1) with 24 random bits: exact in fp32. */

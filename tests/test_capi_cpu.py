@@ -52,6 +52,7 @@ def test_reference_defaults():
     (dict(mode=5), "mode"),
     (dict(mode=1, n=9000), "full-row"),
     (dict(ants_per_block=9), "ants_per_block"),
+    (dict(ants_per_block=-1), "ants_per_block"),
     (dict(construct_path=3), "construct_path"),
 ])
 def test_invalid_parameters_are_rejected(kw, frag):

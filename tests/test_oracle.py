@@ -30,9 +30,12 @@ def test_philox_known_answers():
 
 def test_rng_word_counter_layout():
     seed = (0x12345678 << 32) | 0x9ABCDEF0
-    for it, ant, step, stream in [(0, 0, 0, 0), (5, 17, 6, 3), (1999, 999, 998, 0)]:
-        blk = orc.philox([step >> 2, ant, it, stream], [0x9ABCDEF0, 0x12345678])
-        assert orc.rng_word(seed, it, ant, step, stream) == int(blk[step & 3])
+    for it, ant, step, stream in [(0, 0, 0, 0), (5, 17, 6, 3), (1999, 999, 998, 0), (7, 3, 300, 1)]:
+        blk = orc.philox([64 * (step // 256) + step % 64, ant, it, stream], [0x9ABCDEF0, 0x12345678])
+        assert orc.rng_word(seed, it, ant, step, stream) == int(blk[(step // 64) % 4])
+    # every (block, word) pair is used by exactly one step
+    pairs = {(64 * (s // 256) + s % 64, (s // 64) % 4) for s in range(5000)}
+    assert len(pairs) == 5000
 
 
 def test_uniform_and_start_city_distribution():
