@@ -70,6 +70,18 @@ SIGNATURES = {
     "aco_profile": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_calibrate_copy": (ctypes.c_int, [_vp, _i64, _i32, _vp]),
     "aco_destroy": (ctypes.c_int, [_vp]),
+    "aco_batch_create": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp]),
+    "aco_batch_initialize": (ctypes.c_int, [_vp]),
+    "aco_batch_run": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_batch_get_best": (ctypes.c_int, [_vp, _vp, _vp]),
+    "aco_batch_get_tours": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_batch_get_lengths": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_batch_get_pheromone": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_batch_set_pheromone": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_batch_get_config": (ctypes.c_int, [_vp, _vp]),
+    "aco_batch_autotune": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_batch_destroy": (ctypes.c_int, [_vp]),
+    "aco_batch_last_error": (ctypes.c_char_p, [_vp]),
     "aco_last_error": (ctypes.c_char_p, [_vp]),
     "aco_version": (ctypes.c_char_p, []),
 }
@@ -261,3 +273,86 @@ class Colony:
         t = np.zeros(2, dtype=np.uint64)
         self._chk(self._L.aco_get_counters(self._ctx, _p(t)))
         return int(t[0]), int(t[1])
+
+
+class Batch:
+    """One aco_batch: many independent instances solved together on the GPU."""
+
+    def __init__(self, params: Params, n_list, xy_list, m_list=None):
+        self._L = lib()
+        self.params = params
+        self.n = np.ascontiguousarray(n_list, dtype=np.int32)
+        self.count = len(self.n)
+        self.m = self.n.copy() if m_list is None else np.ascontiguousarray(m_list, dtype=np.int32)
+        xy = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.int32).reshape(-1) for c in xy_list]))
+        self._ctx = ctypes.c_void_p()
+        mp = None if m_list is None else self.m
+        rc = self._L.aco_batch_create(ctypes.byref(params), self.count, _p(self.n), _p(mp), _p(xy),
+                                      ctypes.byref(self._ctx))
+        if rc != 0:
+            raise AcoError(f"aco_batch_create failed ({rc}): {self._L.aco_batch_last_error(None).decode()}")
+        self.city_off = np.concatenate([[0], np.cumsum(self.n)[:-1]]).astype(np.int64)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise AcoError(f"({rc}) {self._L.aco_batch_last_error(self._ctx).decode()}")
+
+    def close(self):
+        if self._ctx:
+            self._L.aco_batch_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def initialize(self):
+        self._chk(self._L.aco_batch_initialize(self._ctx))
+
+    def run(self, iterations):
+        ms = ctypes.c_float()
+        self._chk(self._L.aco_batch_run(self._ctx, int(iterations), ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, reps=3):
+        w = ctypes.c_int32()
+        self._chk(self._L.aco_batch_autotune(self._ctx, int(reps), ctypes.byref(w)))
+        return w.value
+
+    def best(self):
+        gb = np.zeros(self.count, dtype=np.int64)
+        tours = np.zeros(int(self.n.sum()), dtype=np.int32)
+        self._chk(self._L.aco_batch_get_best(self._ctx, _p(gb), _p(tours)))
+        return gb, [tours[o:o + n] for o, n in zip(self.city_off, self.n)]
+
+    def tours(self, i):
+        t = np.zeros((int(self.m[i]), int(self.n[i])), dtype=np.int32)
+        self._chk(self._L.aco_batch_get_tours(self._ctx, int(i), _p(t)))
+        return t
+
+    def lengths(self, i):
+        t = np.zeros(int(self.m[i]), dtype=np.int64)
+        self._chk(self._L.aco_batch_get_lengths(self._ctx, int(i), _p(t)))
+        return t
+
+    def pheromone(self, i):
+        t = np.zeros((int(self.n[i]), int(self.n[i])), dtype=np.float32)
+        self._chk(self._L.aco_batch_get_pheromone(self._ctx, int(i), _p(t)))
+        return t
+
+    def set_pheromone(self, i, tau):
+        t = np.ascontiguousarray(tau, dtype=np.float32)
+        self._chk(self._L.aco_batch_set_pheromone(self._ctx, int(i), _p(t)))
+
+    def config(self):
+        t = np.zeros(4, dtype=np.int32)
+        self._chk(self._L.aco_batch_get_config(self._ctx, _p(t)))
+        return {"ants_per_block": int(t[0]), "blocks": int(t[1]), "items": int(t[2]), "lds_bytes": int(t[3])}

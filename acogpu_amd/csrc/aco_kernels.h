@@ -92,6 +92,59 @@ struct DepositArgs {
     int32_t n, P, dense;
 };
 
+// ---------------------------------------------------------------------------
+// Batched instances: all matrices of all instances live in shared buffers;
+// InstMeta gives each instance's sizes and offsets.
+struct InstMeta {
+    int32_t n, m, P, words;
+    int64_t mat_off;   // element offset of row 0 in the pitched n x P matrices
+    int64_t nk_off;    // offset in the n x k arrays
+    int64_t tour_off;  // offset in tours (m x n)
+    int32_t ant_off;   // offset in per-ant arrays
+    int32_t city_off;  // offset in per-city arrays (global row of city 0)
+    int32_t dep_off;   // offset in the deposit lists (m + w + 2 per instance)
+    int32_t pad;
+};
+
+struct BatchBuffers {
+    const InstMeta* meta;
+    int32_t count, k, total_rows, total_ants;
+    const int32_t* row_inst;  // global row -> instance
+    const int32_t* ant_inst;  // global ant -> instance
+    const int32_t* xy;        // concatenated coordinates, 2 per city
+    int32_t* dist;
+    float* tau_cur;
+    float* tau_next;
+    float* choice;
+    int32_t* nn;
+    uint16_t* nn16;
+    float* choice_nn;
+    float* eta_nn;
+    const float* eta;
+    int32_t eta_len;
+    int32_t* tours;
+    int64_t* lengths;
+    int32_t* best_tours;     // per-city layout (city_off)
+    int32_t* dep_ant;
+    float* dep_val;
+    DevState* state;         // one per instance
+    const int2* items;       // construction work items (instance, first ant), longest first
+    int32_t nitems;
+    uint32_t* queue;         // work-queue counter, reset by the statistics kernel
+    int64_t* cnn;            // per-instance nearest-neighbour tour length
+    float* tau0;             // per-instance initial pheromone
+    float alpha, one_minus_rho;
+    int32_t variant, w, ants_per_block, tau0_policy;
+    float tau0_const, rho;
+    uint32_t seed_lo, seed_hi;
+};
+
+hipError_t launch_batch_init(const BatchBuffers& b, int32_t pmax, int32_t nmax, hipStream_t s);
+hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, int32_t nmax, int32_t grid,
+                                         hipStream_t s);
+size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block);
+int32_t batch_construct_blocks_per_cu(int32_t ants_per_block, size_t lds_bytes);
+
 hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s);
 hipError_t launch_fix_diagonal(int32_t* dist, int32_t n, int32_t P, hipStream_t s);
 hipError_t launch_dist_max(const int32_t* dist, int32_t n, int32_t P, int32_t* out, hipStream_t s);

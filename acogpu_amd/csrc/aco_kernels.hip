@@ -68,12 +68,8 @@ __global__ __launch_bounds__(256) void k_dist_max(const int32_t* __restrict__ di
 // over the (d, j) keys strictly above the previous one -> the stable (d, j)
 // order of dev_compute_nnlist (inc/Colony.cuh:101-123).  k < n excludes the
 // city itself; k == n is the full row (self last, d = MAX_DISTANCE).
-__global__ __launch_bounds__(256) void k_nn_lists(const int32_t* __restrict__ dist, int32_t n, int32_t P,
-                                                  int32_t k, int32_t* __restrict__ nn) {
-    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (i >= n) return;
-    const int32_t* row = dist + (size_t)i * P;
+__device__ __forceinline__ void nn_row(const int32_t* __restrict__ row, int32_t n, int32_t k, int32_t i,
+                                       int32_t* __restrict__ out, int lane) {
     const bool include_self = (k == n);
     uint64_t last = 0;
     for (int32_t r = 0; r < k; ++r) {
@@ -90,8 +86,15 @@ __global__ __launch_bounds__(256) void k_nn_lists(const int32_t* __restrict__ di
             best = o < best ? o : best;
         }
         last = best;
-        if (lane == 0) nn[(size_t)i * k + r] = (int32_t)(uint32_t)best;
+        if (lane == 0) out[r] = (int32_t)(uint32_t)best;
     }
+}
+
+__global__ __launch_bounds__(256) void k_nn_lists(const int32_t* __restrict__ dist, int32_t n, int32_t P,
+                                                  int32_t k, int32_t* __restrict__ nn) {
+    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    nn_row(dist + (size_t)i * P, n, k, i, nn + (size_t)i * k, threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(256) void k_fill_rows(float* __restrict__ a, int32_t n, int32_t P, float v) {
@@ -103,13 +106,9 @@ __global__ __launch_bounds__(256) void k_fill_rows(float* __restrict__ a, int32_
 // Nearest-neighbour tour from `start` (ties to the lowest index): C_nn of the
 // initial-pheromone formulas (Colony::greedyDistance, a stub in the
 // reference, src/Colony.cu:1053-1055).  One block; the visited bitmap in LDS.
-__global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dist, int32_t n, int32_t P,
-                                                 int32_t start, int64_t* __restrict__ out) {
-    extern __shared__ __align__(16) uint32_t sh[];
-    uint32_t* vis = sh;
+__device__ void greedy_block(const int32_t* __restrict__ dist, int32_t n, int32_t P, int32_t start,
+                             int64_t* __restrict__ out, uint32_t* vis, uint64_t* red, int32_t& s_next) {
     const int32_t words = (n + 31) / 32;
-    uint64_t* red = reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1));
-    __shared__ int32_t s_next;
     for (int32_t w = threadIdx.x; w < words; w += blockDim.x) vis[w] = 0;
     __syncthreads();
     if (threadIdx.x == 0) vis[start >> 5] |= 1u << (start & 31);
@@ -144,6 +143,14 @@ __global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dis
     if (threadIdx.x == 0) *out = len + dist[(size_t)c * P + start];
 }
 
+__global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dist, int32_t n, int32_t P,
+                                                 int32_t start, int64_t* __restrict__ out) {
+    extern __shared__ __align__(16) uint32_t sh[];
+    __shared__ int32_t s_next;
+    const int32_t words = (n + 31) / 32;
+    greedy_block(dist, n, P, start, out, sh, reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1)), s_next);
+}
+
 // ---------------------------------------------------------------------------
 // Fused dense sweep, one wave per row (4 rows per 256-thread block), 16-byte
 // coalesced loads and stores.  Per element:
@@ -152,14 +159,7 @@ __global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dis
 // and per row the mean of tau (dev_compute_pheromone_mean, inc/Colony.cuh:164-190),
 // lane l summing columns 256s + 4l + q in order, lanes combined by the DPP scan.
 // Algorithmic traffic: 16 bytes per element (read tau, dist; write choice, tau_next).
-__global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
-    const int lane = threadIdx.x & 63;
-    // updateAntAges (src/Colony.cu:353-364): every ant ages by one at the
-    // start of an iteration, before construction reads the ages.
-    if (a.age != nullptr)
-        for (int32_t k = blockIdx.x * 256 + threadIdx.x; k < a.m; k += gridDim.x * 256) a.age[k] += 1;
-    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= a.n) return;
+__device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lane) {
     const size_t row = (size_t)i * a.P;
     const float* __restrict__ trow = a.tau + row;
     const int32_t* __restrict__ drow = a.dist + row;
@@ -193,6 +193,17 @@ __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
             if (a.tau_nn != nullptr) a.tau_nn[(size_t)i * a.k + r] = t;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
+    const int lane = threadIdx.x & 63;
+    // updateAntAges (src/Colony.cu:353-364): every ant ages by one at the
+    // start of an iteration, before construction reads the ages.
+    if (a.age != nullptr)
+        for (int32_t k = blockIdx.x * 256 + threadIdx.x; k < a.m; k += gridDim.x * 256) a.age[k] += 1;
+    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.n) return;
+    sweep_row(a, i, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -911,11 +922,9 @@ __device__ int32_t block_min_i32(int32_t v, int32_t* red) {
     return b;
 }
 
-__global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
-    __shared__ LenKey red[16];
-    __shared__ int32_t redi[16];
-    __shared__ int32_t s_improved;
-    extern __shared__ uint32_t taken[];  // RBAS: ants already ranked
+// `taken` holds (m+31)/32 words (RBAS ranking); red/redi one slot per wave.
+__device__ void stats_block(const StatsArgs& a, LenKey* red, int32_t* redi, uint32_t* taken, int32_t& s_improved,
+                            int32_t& s_recruit) {
     DevState* st = a.state;
     const LenKey none{(int64_t)ACO_MAX_DISTANCE - 1, 0x7fffffff};
 
@@ -1003,7 +1012,6 @@ __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
             st->converged = 1;
         }
         // recruiteAnts: first invalid ant, when the colony stagnates or is small.
-        __shared__ int32_t s_recruit;
         if (threadIdx.x == 0) s_recruit = (st->gperf >= a.global_perf || st->num_valid < a.min_pop);
         __syncthreads();
         if (s_recruit) {
@@ -1022,12 +1030,19 @@ __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
     if (threadIdx.x == 0) st->iter += 1;
 }
 
+__global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
+    __shared__ LenKey red[16];
+    __shared__ int32_t redi[16];
+    __shared__ int32_t s_improved, s_recruit;
+    extern __shared__ uint32_t taken[];  // RBAS: ants already ranked
+    stats_block(a, red, redi, taken, s_improved, s_recruit);
+}
+
 // ---------------------------------------------------------------------------
 // Deposit: one block per list entry, fp32 atomics on both (i,j) and (j,i) of
 // every tour edge (dev_update_pheromones src/AntSystem.cu:105-128, which only
 // updated one direction; the CPU branch src/AntSystem.cu:296-316 is symmetric).
-__global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
-    const int32_t e = blockIdx.x;
+__device__ __forceinline__ void deposit_entry(const DepositArgs& a, int32_t e) {
     if (e >= a.state->ndep) return;
     const int32_t ant = a.dense ? e : a.dep_ant[e];
     const float d = a.dep_val[e];
@@ -1041,9 +1056,234 @@ __global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) { deposit_entry(a, blockIdx.x); }
+
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256)
         dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Batched instances.  Every kernel maps its global index (row, ant, instance
+// or work item) to an instance and rebuilds that instance's argument struct,
+// then runs the same device code as the single-instance kernels; instance i
+// uses Philox stream i, so its tours equal a single colony's with stream_id i.
+__device__ __forceinline__ SweepArgs batch_sweep_args(const BatchBuffers& b, const InstMeta& im) {
+    SweepArgs a{};
+    a.tau = b.tau_cur + im.mat_off;
+    a.dist = b.dist + im.mat_off;
+    a.eta = b.eta;
+    a.eta_len = b.eta_len;
+    a.nn = b.nn + im.nk_off;
+    a.k = b.k;
+    a.choice = b.choice + im.mat_off;
+    a.choice_nn = b.choice_nn + im.nk_off;
+    a.tau_nn = nullptr;
+    a.tau_next = b.tau_next + im.mat_off;
+    a.mean = nullptr;
+    a.age = nullptr;
+    a.m = im.m;
+    a.n = im.n;
+    a.P = im.P;
+    a.alpha = b.alpha;
+    a.one_minus_rho = b.one_minus_rho;
+    a.want_mean = 0;
+    return a;
+}
+
+__device__ __forceinline__ ConstructArgs batch_construct_args(const BatchBuffers& b, int32_t inst,
+                                                              const InstMeta& im) {
+    ConstructArgs a{};
+    a.nn = b.nn + im.nk_off;
+    a.nn16 = b.nn16 + im.nk_off;
+    a.choice_nn = b.choice_nn + im.nk_off;
+    a.tau_nn = nullptr;
+    a.eta_nn = b.eta_nn + im.nk_off;
+    a.choice = b.choice + im.mat_off;
+    a.dist = b.dist + im.mat_off;
+    a.tau = b.tau_cur + im.mat_off;
+    a.mean = nullptr;
+    a.eta = b.eta;
+    a.eta_len = b.eta_len;
+    a.tours = b.tours + im.tour_off;
+    a.lengths = b.lengths + im.ant_off;
+    a.age = nullptr;
+    a.valid = nullptr;
+    a.state = b.state + inst;
+    a.n = im.n;
+    a.P = im.P;
+    a.m = im.m;
+    a.k = b.k;
+    a.aging = 0;
+    a.min_age = 1;
+    a.alpha = b.alpha;
+    a.seed_lo = b.seed_lo;
+    a.seed_hi = b.seed_hi;
+    a.stream = (uint32_t)inst;
+    a.words = im.words;
+    a.young_blocks = 0;
+    return a;
+}
+
+__device__ __forceinline__ DepositArgs batch_deposit_args(const BatchBuffers& b, int32_t inst, const InstMeta& im) {
+    DepositArgs a{};
+    a.tours = b.tours + im.tour_off;
+    a.best_tour = b.best_tours + im.city_off;
+    a.dep_ant = b.dep_ant + im.dep_off;
+    a.dep_val = b.dep_val + im.dep_off;
+    a.state = b.state + inst;
+    a.tau_next = b.tau_next + im.mat_off;
+    a.n = im.n;
+    a.P = im.P;
+    a.dense = b.variant == 0;
+    return a;
+}
+
+__global__ __launch_bounds__(256) void kb_dist(const BatchBuffers b) {
+    const int32_t r = blockIdx.y;
+    const InstMeta im = b.meta[b.row_inst[r]];
+    const int32_t i = r - im.city_off;
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= im.P) return;
+    const int32_t* xy = b.xy + 2 * (size_t)im.city_off;
+    int32_t d = ACO_MAX_DISTANCE;
+    if (j < im.n && j != i)
+        d = euc2d_exact((int64_t)xy[2 * i] - xy[2 * j], (int64_t)xy[2 * i + 1] - xy[2 * j + 1]);
+    b.dist[im.mat_off + (size_t)i * im.P + j] = d;
+}
+
+__global__ __launch_bounds__(256) void kb_nn(const BatchBuffers b) {
+    const int32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= b.total_rows) return;
+    const InstMeta im = b.meta[b.row_inst[r]];
+    const int32_t i = r - im.city_off;
+    nn_row(b.dist + im.mat_off + (size_t)i * im.P, im.n, b.k, i, b.nn + im.nk_off + (size_t)i * b.k,
+           threadIdx.x & 63);
+}
+
+__global__ __launch_bounds__(256) void kb_nn_aux(const BatchBuffers b) {
+    const int32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= b.total_rows) return;
+    const InstMeta im = b.meta[b.row_inst[r]];
+    const int32_t i = r - im.city_off;
+    for (int32_t q = lane; q < b.k; q += 64) {
+        const size_t x = im.nk_off + (size_t)i * b.k + q;
+        const int32_t j = b.nn[x];
+        b.nn16[x] = (uint16_t)j;
+        b.eta_nn[x] = eta_of(b.eta, b.eta_len, b.dist[im.mat_off + (size_t)i * im.P + j]);
+    }
+}
+
+// One block per instance: C_nn, then tau0 for the instance's policy.
+__global__ __launch_bounds__(1024) void kb_greedy(const BatchBuffers b) {
+    extern __shared__ __align__(16) uint32_t sh[];
+    __shared__ int32_t s_next;
+    const int32_t inst = blockIdx.x;
+    const InstMeta im = b.meta[inst];
+    const int32_t words = (im.n + 31) / 32;
+    greedy_block(b.dist + im.mat_off, im.n, im.P, 0, b.cnn + inst, sh,
+                 reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1)), s_next);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double cnn = (double)b.cnn[inst];
+        float t0 = b.tau0_const;
+        if (b.tau0_policy == 1) t0 = (float)((double)im.m / cnn);
+        if (b.tau0_policy == 2) t0 = (float)(0.5 * b.w * (b.w - 1) / ((double)b.rho * cnn));
+        b.tau0[inst] = t0;
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_fill_tau(const BatchBuffers b) {
+    const int32_t r = blockIdx.y;
+    const int32_t inst = b.row_inst[r];
+    const InstMeta im = b.meta[inst];
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= im.P) return;
+    const size_t x = im.mat_off + (size_t)(r - im.city_off) * im.P + j;
+    b.tau_cur[x] = j < im.n ? b.tau0[inst] : 0.0f;
+    b.tau_next[x] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void kb_sweep(const BatchBuffers b) {
+    const int32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= b.total_rows) return;
+    const InstMeta im = b.meta[b.row_inst[r]];
+    sweep_row(batch_sweep_args(b, im), r - im.city_off, threadIdx.x & 63);
+}
+
+// One persistent launch: blocks take work items (instance, W ants) from an
+// atomic queue, longest instances first, stage the instance's candidate lists
+// and weights in LDS and construct W tours (one per wave).
+__global__ __launch_bounds__(512) void kb_construct(const BatchBuffers b) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    __shared__ int32_t s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int32_t)atomicAdd(b.queue, 1u);
+        __syncthreads();
+        const int32_t item = s_item;
+        __syncthreads();
+        if (item >= b.nitems) return;
+        const int2 it = b.items[item];
+        const InstMeta im = b.meta[it.x];
+        const ConstructArgs a = batch_construct_args(b, it.x, im);
+        const int32_t nk = im.n * b.k;
+        float* wl = reinterpret_cast<float*>(lds);
+        uint16_t* nl = reinterpret_cast<uint16_t*>(wl + ((nk + 3) & ~3));
+        uint32_t* per = reinterpret_cast<uint32_t*>(nl + ((nk + 7) & ~7));
+        uint32_t* vis = per + (size_t)wave * (im.words + (im.n + 1) / 2);
+        uint16_t* t16 = reinterpret_cast<uint16_t*>(vis + im.words);
+        stage_u16(nl, a.nn16, nk);
+        stage_f32(wl, a.choice_nn, nk);
+        __syncthreads();
+        const int32_t ant = it.y + wave;
+        if (ant < im.m) construct_staged_dispatch(a, ant, false, 0.0f, wl, nl, vis, t16, a.state->iter, lane);
+        __syncthreads();
+    }
+}
+
+// One block per instance; block 0 also re-arms the construction queue.
+__global__ __launch_bounds__(256) void kb_stats(const BatchBuffers b) {
+    __shared__ LenKey red[16];
+    __shared__ int32_t redi[16];
+    __shared__ int32_t s_improved, s_recruit;
+    extern __shared__ uint32_t taken[];
+    const int32_t inst = blockIdx.x;
+    const InstMeta im = b.meta[inst];
+    StatsArgs a{};
+    a.tours = b.tours + im.tour_off;
+    a.lengths = b.lengths + im.ant_off;
+    a.valid = nullptr;
+    a.age = nullptr;
+    a.perf = nullptr;
+    a.prev_len = nullptr;
+    a.best_tour = b.best_tours + im.city_off;
+    a.dep_ant = b.dep_ant + im.dep_off;
+    a.dep_val = b.dep_val + im.dep_off;
+    a.state = b.state + inst;
+    a.n = im.n;
+    a.m = im.m;
+    a.variant = b.variant;
+    a.w = b.w < im.n ? b.w : im.n;
+    a.aging = 0;
+    stats_block(a, red, redi, taken, s_improved, s_recruit);
+    if (inst == 0 && threadIdx.x == 0) *b.queue = 0u;
+}
+
+// AS: one block per ant of every instance; RBAS: w + 1 blocks per instance.
+__global__ __launch_bounds__(256) void kb_deposit(const BatchBuffers b) {
+    int32_t inst, e;
+    if (b.variant == 0) {
+        inst = b.ant_inst[blockIdx.x];
+        e = blockIdx.x - b.meta[inst].ant_off;
+    } else {
+        inst = blockIdx.x / (b.w + 1);
+        e = blockIdx.x % (b.w + 1);
+    }
+    const InstMeta im = b.meta[inst];
+    deposit_entry(batch_deposit_args(b, inst, im), e);
 }
 
 // ---------------------------------------------------------------------------
@@ -1124,6 +1364,51 @@ hipError_t launch_stats(const StatsArgs& a, hipStream_t s) {
 
 hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s) {
     hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block) {
+    return staged_lds_bytes(nmax, k, (nmax + 31) / 32, ants_per_block);
+}
+
+int32_t batch_construct_blocks_per_cu(int32_t ants_per_block, size_t lds_bytes) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kb_construct, 64 * ants_per_block, lds_bytes) != hipSuccess)
+        nb = 1;
+    return nb > 0 ? nb : 1;
+}
+
+// Initialisation: distances, candidate lists, heuristic on candidate edges,
+// C_nn and tau0, pheromone fill.
+hipError_t launch_batch_init(const BatchBuffers& b, int32_t pmax, int32_t nmax, hipStream_t s) {
+    hipLaunchKernelGGL(kb_dist, dim3(cdiv(pmax, 256), b.total_rows), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(kb_nn, dim3(cdiv(b.total_rows, 4)), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(kb_nn_aux, dim3(cdiv(b.total_rows, 4)), dim3(256), 0, s, b);
+    const int32_t words = (nmax + 31) / 32;
+    const size_t glds = (size_t)((words + 1) & ~1) * 4 + 16 * sizeof(uint64_t);
+    hipLaunchKernelGGL(kb_greedy, dim3(b.count), dim3(1024), glds, s, b);
+    hipLaunchKernelGGL(kb_fill_tau, dim3(cdiv(pmax, 256), b.total_rows), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+// which: 0 sweep, 1 construct (grid = persistent blocks), 2 stats, 3 deposit.
+hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, int32_t nmax, int32_t grid,
+                                         hipStream_t s) {
+    switch (which) {
+        case 0:
+            hipLaunchKernelGGL(kb_sweep, dim3(cdiv(b.total_rows, 4)), dim3(256), 0, s, b);
+            break;
+        case 1:
+            hipLaunchKernelGGL(kb_construct, dim3(grid), dim3(64 * b.ants_per_block),
+                               batch_construct_lds_bytes(nmax, b.k, b.ants_per_block), s, b);
+            break;
+        case 2:  // grid carries the RBAS ranking bitmap words, (max m + 31) / 32
+            hipLaunchKernelGGL(kb_stats, dim3(b.count), dim3(256), (size_t)grid * sizeof(uint32_t), s, b);
+            break;
+        default:
+            hipLaunchKernelGGL(kb_deposit, dim3(b.variant == 0 ? b.total_ants : b.count * (b.w + 1)), dim3(256), 0,
+                               s, b);
+    }
     return hipGetLastError();
 }
 

@@ -164,6 +164,35 @@ int aco_profile(aco_ctx *c, int32_t iterations, float *kernel_ms);
  * device with a plain streaming kernel `reps` times; returns ms per copy. */
 int aco_calibrate_copy(aco_ctx *c, int64_t bytes, int32_t reps, float *ms);
 
+/* ------------------------------------------------------------------------
+ * Batched solver: `count` independent instances solved together, one
+ * persistent construction launch per iteration.  No counterpart in the
+ * reference (one instance per process, src/Setup.cpp:73-129); every instance
+ * runs the same iteration as an aco_ctx.  Shared parameters come from p (its n
+ * and m are ignored; modes: candidate lists staged in LDS, no aging); n[i]
+ * cities and m[i] ants (m == NULL: m[i] = n[i]) per instance, xy the
+ * concatenated integer coordinates (2 n[i] per instance).  Instance i draws
+ * from Philox stream i, so its tours equal an aco_ctx's with stream_id = i. */
+typedef struct aco_batch aco_batch;
+int aco_batch_create(const aco_params *p, int32_t count, const int32_t *n, const int32_t *m, const int32_t *xy,
+                     aco_batch **out);
+int aco_batch_initialize(aco_batch *b);
+int aco_batch_run(aco_batch *b, int32_t iterations, float *elapsed_ms);
+/* glob_best[count]; tours (may be NULL): sum of n[i], instance after instance */
+int aco_batch_get_best(aco_batch *b, int64_t *glob_best, int32_t *tours);
+int aco_batch_get_tours(aco_batch *b, int32_t instance, int32_t *tours);     /* m[i] x n[i] */
+int aco_batch_get_lengths(aco_batch *b, int32_t instance, int64_t *lengths); /* m[i]       */
+int aco_batch_get_pheromone(aco_batch *b, int32_t instance, float *tau);      /* n[i] x n[i] */
+int aco_batch_set_pheromone(aco_batch *b, int32_t instance, const float *tau);
+/* config = {ants per block, persistent blocks, work items, LDS bytes per block} */
+int aco_batch_get_config(aco_batch *b, int32_t *config);
+/* Times construction launches for each ants-per-block choice (a launch
+ * parameter only: tours do not change) and keeps the fastest; reps launches
+ * per choice; chosen (may be NULL) gets the choice. */
+int aco_batch_autotune(aco_batch *b, int32_t reps, int32_t *chosen);
+int aco_batch_destroy(aco_batch *b);
+const char *aco_batch_last_error(const aco_batch *b);
+
 /* Colony::~Colony */
 int aco_destroy(aco_ctx *c);
 

@@ -629,6 +629,7 @@ double orc_run(int32_t n, int32_t m, const int32_t *dist, const int32_t *params,
     int32_t variant = params[0], mode = params[1], k = params[2], w = params[3];
     int32_t aging = params[4], min_age = params[5], max_age = params[6];
     int32_t global_perf = params[7], min_pop = params[8], iters = params[9];
+    uint32_t stream = (uint32_t)params[10];
     float alpha = fparams[0], beta = fparams[1], rho = fparams[2], tau0 = fparams[3];
     int64_t nn2 = (int64_t)n * n;
 
@@ -672,7 +673,7 @@ double orc_run(int32_t n, int32_t m, const int32_t *dist, const int32_t *params,
                 orc_sweep(n, 0, tau, dist, eta, eta_len, NULL, alpha, rho, 0, choice, NULL, NULL, NULL);
             orc_construct(mode, n, m, k, dist, NULL, nn, choice, NULL, tau, mean, eta, eta_len,
                           alpha, aging ? age : NULL, aging ? valid : NULL, converged, min_age,
-                          seed, (uint32_t)it, 0, tours, lengths, NULL);
+                          seed, (uint32_t)it, stream, tours, lengths, NULL);
         } else {
             orc_sweep(n, k, tau, dist, eta, eta_len, mode == ORC_MODE_CANDIDATE ? nn : NULL,
                       alpha, rho, 0, choice, mode == ORC_MODE_CANDIDATE ? choice_nn : NULL, NULL,
@@ -680,7 +681,7 @@ double orc_run(int32_t n, int32_t m, const int32_t *dist, const int32_t *params,
             orc_construct(mode, n, m, k, dist, mode == ORC_MODE_CANDIDATE ? nn : NULL,
                           mode == ORC_MODE_FULLROW ? nn : NULL, choice, choice_nn, tau, mean, eta,
                           eta_len, alpha, aging ? age : NULL, aging ? valid : NULL, converged,
-                          min_age, seed, (uint32_t)it, 0, tours, lengths, NULL);
+                          min_age, seed, (uint32_t)it, stream, tours, lengths, NULL);
         }
         int32_t nd = orc_stats(variant, n, m, w, tours, lengths, aging ? valid : NULL,
                                aging ? perf : NULL, aging ? prev_len : NULL, st, best_tour,

@@ -105,7 +105,7 @@ int32_t orc_greedy_length(const int32_t *dist, int32_t n, int32_t start);
 
 /* Whole-run driver with the reference CPU semantics (timed CPU baseline).
  * params: [0]=variant [1]=mode [2]=k [3]=w [4]=aging [5]=min_age [6]=max_age
- *         [7]=global_perf [8]=min_pop [9]=iterations
+ *         [7]=global_perf [8]=min_pop [9]=iterations [10]=Philox stream
  * fparams: [0]=alpha [1]=beta [2]=rho [3]=tau0
  * out: [0]=glob_best [1]=last iter_best ; returns elapsed seconds of the
  * iteration loop (steady clock), or -1 on allocation failure. */

@@ -241,13 +241,13 @@ def greedy_length(dist, start=0):
 
 def run(dist, m, *, variant=AS, mode=MODE_CANDIDATE, k=20, w=6, aging=False, min_age=30,
         max_age=50, global_perf=10, min_pop=40, iterations=10, alpha=1.0, beta=2.0, rho=0.5,
-        tau0=1e-6, seed=1):
+        tau0=1e-6, seed=1, stream=0):
     """Whole run with the reference CPU operation order; returns
     (seconds, glob_best, last_iter_best, best_tour)."""
     dist = np.ascontiguousarray(dist, dtype=np.int32)
     n = dist.shape[0]
     params = np.array([variant, mode, k, w, int(aging), min_age, max_age, global_perf, min_pop,
-                       iterations], dtype=np.int32)
+                       iterations, stream], dtype=np.int32)
     fparams = np.array([alpha, beta, rho, tau0], dtype=np.float32)
     out = np.zeros(2, dtype=np.int64)
     best = np.zeros(n, dtype=np.int32)

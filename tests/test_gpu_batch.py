@@ -1,0 +1,132 @@
+"""GPU parity of the batched solver: every instance of a batch must behave as
+a single colony with stream_id = instance index (itself pinned against the
+oracle), through the persistent construction launch and the graph replay."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from acogpu_amd import capi, instances
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def make_batch_inputs(count, lo, hi, seed, side=1000):
+    sizes = instances.batched_sizes(count, lo, hi, seed=seed)
+    xys = [instances.uniform(int(n), side, seed=seed * 100000 + i) for i, n in enumerate(sizes)]
+    return sizes, xys
+
+
+def batch_params(**kw):
+    base = dict(n=0, m=0, k=16, beta=2.0, rho=0.5, variant=capi.VARIANT_AS, tau0_policy=capi.TAU0_AS, seed=77)
+    base.update(kw)
+    return capi.default_params(**base)
+
+
+@pytest.mark.parametrize("apb", [1, 4, 8])
+def test_batch_first_iterations_equal_single_colonies(apb):
+    say(f"case: batch of 10 vs single colonies, ants_per_block={apb}")
+    sizes, xys = make_batch_inputs(10, 20, 300, seed=1)
+    p = batch_params(ants_per_block=apb)
+    with capi.Batch(p, sizes, xys) as b:
+        b.initialize()
+        b.run(1)
+        gb, best = b.best()
+        for i in (0, 3, 7, 9):
+            n = int(sizes[i])
+            q = batch_params(n=n, m=n, stream_id=i)
+            with capi.Colony(q, xy=xys[i]) as c:
+                c.initialize()
+                c.iterate()
+                np.testing.assert_array_equal(b.tours(i), c.tours())
+                np.testing.assert_array_equal(b.lengths(i), c.lengths())
+                assert gb[i] == c.stats()[1]
+                np.testing.assert_array_equal(best[i], c.best_tour())
+                np.testing.assert_allclose(b.pheromone(i), c.pheromone(), rtol=1e-6)
+
+
+def test_batch_iteration_replay_through_oracle():
+    say("case: batch of 16, iteration 4 replayed through the oracle")
+    sizes, xys = make_batch_inputs(16, 30, 250, seed=2)
+    p = batch_params()
+    with capi.Batch(p, sizes, xys) as b:
+        b.initialize()
+        b.run(3)
+        gb0, best0 = b.best()
+        taus = {i: b.pheromone(i) for i in (1, 5, 11, 15)}
+        b.run(1)
+        gb1, best1 = b.best()
+        for i, tau in taus.items():
+            n = int(sizes[i])
+            dist = orc.dist_from_int_coords(xys[i])
+            eta = orc.eta_table(2.0, orc.max_offdiag(dist))
+            nn = orc.nn_lists(dist, 16)
+            choice, cnn, _, _ = orc.sweep(tau, dist, eta, nn, 1.0, 0.5)
+            tours, lengths, _ = orc.construct(orc.MODE_CANDIDATE, dist, nn, choice, cnn, tau, None, eta, 1.0, n,
+                                              77, 3, stream=i)
+            np.testing.assert_array_equal(b.tours(i), tours)
+            np.testing.assert_array_equal(b.lengths(i), lengths)
+            st = np.array([orc.MAX_DISTANCE - 1, gb0[i], -1, 0], dtype=np.int64)
+            bt = best0[i].copy()
+            dep_ant, dep_w = orc.stats(orc.AS, tours, lengths, 1, st, bt)
+            assert gb1[i] == st[1]
+            np.testing.assert_array_equal(best1[i], bt)
+            tau64 = tau.astype(np.float64)
+            orc.evaporate_f64(tau64, 0.5)
+            orc.deposit_f64(tau64, tours, lengths, bt, int(st[1]), dep_ant, dep_w)
+            got = b.pheromone(i).astype(np.float64)
+            assert (np.abs(got - tau64) / tau64).max() <= 1e-5
+
+
+def test_batch_deterministic_rank_based_run_matches_oracle():
+    """Rank-based with w = 1 deposits one tour per instance: the 12-iteration
+    run is deterministic and equals the oracle's fp32 run per instance."""
+    say("case: batch of 8, RBAS w=1, 12 iterations vs oracle runs")
+    sizes, xys = make_batch_inputs(8, 40, 200, seed=3)
+    p = batch_params(variant=capi.VARIANT_RBAS, w=1, tau0_policy=capi.TAU0_CONST, tau0=0.01, rho=0.2)
+    with capi.Batch(p, sizes, xys) as b:
+        b.initialize()
+        b.run(12)
+        gb, best = b.best()
+        for i in range(8):
+            dist = orc.dist_from_int_coords(xys[i])
+            n = int(sizes[i])
+            _, ogb, _, obest = orc.run(dist, n, variant=orc.RBAS, mode=orc.MODE_CANDIDATE, k=16, w=1,
+                                       iterations=12, alpha=1.0, beta=2.0, rho=0.2, tau0=0.01, seed=77, stream=i)
+            assert gb[i] == ogb
+            np.testing.assert_array_equal(best[i], obest)
+
+
+def test_batch_autotune_changes_only_the_launch():
+    say("case: autotune on a batch of 24")
+    sizes, xys = make_batch_inputs(24, 50, 500, seed=4)
+    p = batch_params(ants_per_block=1)
+    with capi.Batch(p, sizes, xys) as a, capi.Batch(p, sizes, xys) as b:
+        a.initialize()
+        b.initialize()
+        w = b.autotune(reps=2)
+        assert w in (1, 2, 4, 8) and b.config()["ants_per_block"] == w
+        a.run(1)
+        b.run(1)
+        for i in range(24):
+            np.testing.assert_array_equal(a.tours(i), b.tours(i))
+
+
+def test_batch_4096_uneven_instances_properties():
+    say("case: 4096 instances n log-uniform [50, 500], m = n, k = 16, 4 iterations")
+    sizes, xys = make_batch_inputs(4096, 50, 500, seed=5)
+    p = batch_params()
+    with capi.Batch(p, sizes, xys) as b:
+        b.initialize()
+        say(f"config {b.config()}")
+        b.run(4)
+        gb, best = b.best()
+        rng = np.random.default_rng(0)
+        for i in rng.choice(4096, size=24, replace=False):
+            n = int(sizes[i])
+            assert sorted(best[i].tolist()) == list(range(n))
+            dist = orc.dist_from_int_coords(xys[i])
+            assert instances.tour_length(dist, best[i]) == gb[i]
