@@ -68,6 +68,92 @@ def cpu_baseline(xy, iterations):
     return 1000 * iterations / secs, secs
 
 
+def bench_batched(args):
+    """4096 independent instances (n log-uniform in [50, 500], integer
+    coordinates uniform in [0,1000]^2, m = n, k = 16, Ant System, alpha 1,
+    beta 2, rho 0.5, tau0 = m / C_nn), `steps` iterations each: one batch with
+    the persistent construction launch vs. the same instances solved one after
+    another with the single-instance solver (a sample of them, same seeds)."""
+    count, iters = args.batch_count, args.steps
+    sizes = instances.batched_sizes(count, 50, 500, seed=5)
+    xys = [instances.uniform(int(n), 1000, seed=500000 + i) for i, n in enumerate(sizes)]
+    p = capi.default_params(n=0, m=0, k=16, beta=2.0, rho=0.5, variant=capi.VARIANT_AS,
+                            tau0_policy=capi.TAU0_AS, seed=99)
+    log(f"bench: batch of {count} instances, {int(sizes.sum())} cities, {iters} iterations")
+    b = capi.Batch(p, sizes, xys)
+    b.initialize()
+    w = b.autotune(reps=2)
+    b.initialize()
+    log(f"bench: autotuned ants per block {w}, config {b.config()}")
+    b.run(args.warmup)
+    b.initialize()
+    t0 = time.perf_counter()
+    b.run(iters)
+    wall = time.perf_counter() - t0
+    gb, _ = b.best()
+    b.close()
+    batch_rate = count / wall
+    log(f"bench: batched {wall:.3f} s -> {batch_rate:.1f} instances/s")
+    rng = np.random.default_rng(1)
+    sample = rng.choice(count, size=min(args.seq_sample, count), replace=False)
+    seq_time = 0.0
+    for i in sample:
+        n = int(sizes[i])
+        q = capi.default_params(n=n, m=n, k=16, beta=2.0, rho=0.5, variant=capi.VARIANT_AS,
+                                tau0_policy=capi.TAU0_AS, seed=99, stream_id=int(i))
+        c = capi.Colony(q, xy=xys[i])
+        c.initialize()
+        c.run(2)
+        c.initialize()
+        t0 = time.perf_counter()
+        c.run(iters)
+        seq_time += time.perf_counter() - t0
+        c.close()
+    seq_rate = len(sample) / seq_time
+    log(f"bench: sequential {len(sample)} instances {seq_time:.3f} s -> {seq_rate:.1f} instances/s")
+    return {
+        "metric": "instances/sec, batched uneven instances", "value": batch_rate, "unit": "instances/s",
+        "n_gpus": 1, "steps": iters, "warmup": args.warmup, "ms_per_step": wall * 1e3 / iters,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{count} instances, n log-uniform [50,500], m=n, k=16, AS, beta 2, rho 0.5",
+                   "iterations": iters, "ants_per_block": w},
+        "sequential": {"value": seq_rate, "unit": "instances/s", "sample": int(len(sample)),
+                       "kind": "single-instance solver, one instance after another"},
+        "speedup_vs_sequential": batch_rate / seq_rate,
+        "mean_best_length": float(np.mean(gb)),
+    }
+
+
+def bench_large(args):
+    """n = 20000 cities uniform in [0,100000]^2, m = 1024, k = 32, Ant System
+    with fp32 atomic deposits, alpha 1, beta 2, rho 0.2, tau0 = m / C_nn."""
+    n, m = 20000, 1024
+    xy = instances.uniform(n, 100000, seed=3)
+    p = capi.default_params(n=n, m=m, k=32, beta=2.0, rho=0.2, variant=capi.VARIANT_AS,
+                            tau0_policy=capi.TAU0_AS, seed=5)
+    col = capi.Colony(p, xy=xy)
+    col.initialize()
+    col.run(args.warmup)
+    t0 = time.perf_counter()
+    col.run(args.steps)
+    wall = time.perf_counter() - t0
+    fb0 = col.counters()
+    kms = [float(x) for x in col.profile(2)]
+    fb1 = col.counters()
+    ib, gb, _ = col.stats()
+    col.close()
+    log(f"bench: n=20000 {wall * 1e3 / args.steps:.2f} ms/iteration, kernels {kms}, "
+        f"fallback steps/launch {(fb1[0] - fb0[0]) / 2:.0f}")
+    return {
+        "metric": "tours/sec at n=20000", "value": m * args.steps / wall, "unit": "tours/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "n=20000 uniform [0,100000]^2, m=1024, k=32, AS, beta 2, rho 0.2"},
+        "kernel_ms": {"sweep": kms[0], "construct": kms[1], "stats": kms[2], "deposit": kms[3]},
+        "best_length": gb,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +163,11 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ants-per-block", type=int, default=0)
+    ap.add_argument("--workload", choices=["c2", "batched", "large"], default="c2",
+                    help="c2: the headline n=1000 tours/s line (default); batched: 4096 uneven instances; "
+                         "large: n=20000")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -84,6 +175,11 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(instances and ants shard without exchange; see the sharding notes)", file=sys.stderr)
         return 2
+
+    if args.workload != "c2":
+        line = bench_batched(args) if args.workload == "batched" else bench_large(args)
+        print(json.dumps(line), flush=True)
+        return 0
 
     n, m, k = 1000, 1000, 20
     xy = instances.clustered(n, clusters=10, side=10000, sigma=400.0, seed=1)
