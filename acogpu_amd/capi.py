@@ -66,6 +66,9 @@ SIGNATURES = {
     "aco_set_aging": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "aco_set_best": (ctypes.c_int, [_vp, _i64, _vp]),
     "aco_get_counters": (ctypes.c_int, [_vp, _vp]),
+    "aco_set_heuristic_nn": (ctypes.c_int, [_vp, _vp]),
+    "aco_get_heuristic_nn": (ctypes.c_int, [_vp, _vp]),
+    "aco_batch_set_heuristic_nn": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_get_config": (ctypes.c_int, [_vp, _vp]),
     "aco_profile": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_calibrate_copy": (ctypes.c_int, [_vp, _i64, _i32, _vp]),
@@ -263,6 +266,15 @@ class Colony:
                                         _p(conv(perf, np.int32)), _p(conv(prev_len, np.int64)),
                                         _p(conv(scalars, np.int32))))
 
+    def set_heuristic_nn(self, h=None):
+        t = None if h is None else np.ascontiguousarray(h, dtype=np.float32)
+        self._chk(self._L.aco_set_heuristic_nn(self._ctx, _p(t)))
+
+    def heuristic_nn(self):
+        t = np.zeros((self.n, self.params.k), dtype=np.float32)
+        self._chk(self._L.aco_get_heuristic_nn(self._ctx, _p(t)))
+        return t
+
     def config(self):
         t = np.zeros(4, dtype=np.int32)
         self._chk(self._L.aco_get_config(self._ctx, _p(t)))
@@ -351,6 +363,10 @@ class Batch:
     def set_pheromone(self, i, tau):
         t = np.ascontiguousarray(tau, dtype=np.float32)
         self._chk(self._L.aco_batch_set_pheromone(self._ctx, int(i), _p(t)))
+
+    def set_heuristic_nn(self, i, h):
+        t = np.ascontiguousarray(h, dtype=np.float32)
+        self._chk(self._L.aco_batch_set_heuristic_nn(self._ctx, int(i), _p(t)))
 
     def config(self):
         t = np.zeros(4, dtype=np.int32)

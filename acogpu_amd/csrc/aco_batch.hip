@@ -470,3 +470,11 @@ extern "C" int aco_batch_get_config(aco_batch* b, int32_t* config) {
     config[3] = (int32_t)aco::batch_construct_lds_bytes(b->nmax, b->k, b->apb);
     return ACO_OK;
 }
+
+extern "C" int aco_batch_set_heuristic_nn(aco_batch* b, int32_t i, const float* h) {
+    int rc = check_inst(b, i);
+    if (rc || !h) return rc ? rc : ACO_E_ARG;
+    const InstMeta& im = b->meta[i];
+    BHIP(b, hipMemcpy(b->buf.eta_nn + im.nk_off, h, sizeof(float) * im.n * b->k, hipMemcpyHostToDevice));
+    return ACO_OK;
+}

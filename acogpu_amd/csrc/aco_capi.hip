@@ -403,6 +403,7 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur) {
     sa.k = c->k;
     sa.choice = c->d_choice;
     sa.choice_nn = c->d_choice_nn;
+    sa.eta_nn = c->d_eta_nn;
     sa.tau_nn = (cand && c->p.aging) ? c->d_tau_nn : nullptr;
     sa.tau_next = tau_next;
     sa.mean = c->d_mean;
@@ -832,4 +833,25 @@ extern "C" int aco_phase(aco_ctx* c, int32_t phase, float* ms) {
     c->next_phase = next_of[phase];
     if (phase == ACO_PHASE_UPDATE_PHEROMONE) c->cur ^= 1;
     return ACO_OK;
+}
+
+// Heuristic on candidate edges (n x k): replaces eta^beta on the NN-list
+// edges only (the fallback keeps plain eta^beta); NULL restores eta^beta.
+extern "C" int aco_set_heuristic_nn(aco_ctx* c, const float* h) {
+    if (!c) return ACO_E_ARG;
+    if (c->k == 0) return fail(c, ACO_E_STATE, "no candidate lists in full-row mode");
+    if (!c->initialized) return fail(c, ACO_E_STATE, "aco_set_heuristic_nn before aco_initialize");
+    if (h == nullptr) {
+        HIP_TRY(c, aco::launch_nn_aux(c->d_nn, c->d_dist, c->d_eta, c->eta_len, c->n, c->P, c->k, c->d_nn16,
+                                      c->d_eta_nn, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return ACO_OK;
+    }
+    return copy_in(c, c->d_eta_nn, h, sizeof(float) * (size_t)c->n * c->k);
+}
+
+extern "C" int aco_get_heuristic_nn(aco_ctx* c, float* h) {
+    if (!c || !h) return ACO_E_ARG;
+    if (c->k == 0) return fail(c, ACO_E_STATE, "no candidate lists in full-row mode");
+    return copy_out(c, h, c->d_eta_nn, sizeof(float) * (size_t)c->n * c->k);
 }

@@ -32,6 +32,7 @@ struct SweepArgs {
     int32_t k;
     float* choice;           // pitched n x P
     float* choice_nn;        // n x k
+    const float* eta_nn;     // n x k heuristic on candidate edges (eta^beta or a learned prior)
     float* tau_nn;           // n x k pheromone on candidate edges (aging)
     float* tau_next;         // evaporated pheromone (pitched)
     float* mean;             // n row means (aging)

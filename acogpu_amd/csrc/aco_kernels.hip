@@ -189,7 +189,7 @@ __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lan
         for (int32_t r = lane; r < a.k; r += 64) {
             const int32_t j = a.nn[(size_t)i * a.k + r];
             const float t = trow[j];
-            a.choice_nn[(size_t)i * a.k + r] = apply_alpha(t, a.alpha) * eta_of(a.eta, a.eta_len, drow[j]);
+            a.choice_nn[(size_t)i * a.k + r] = apply_alpha(t, a.alpha) * a.eta_nn[(size_t)i * a.k + r];
             if (a.tau_nn != nullptr) a.tau_nn[(size_t)i * a.k + r] = t;
         }
     }
@@ -1078,6 +1078,7 @@ __device__ __forceinline__ SweepArgs batch_sweep_args(const BatchBuffers& b, con
     a.k = b.k;
     a.choice = b.choice + im.mat_off;
     a.choice_nn = b.choice_nn + im.nk_off;
+    a.eta_nn = b.eta_nn + im.nk_off;
     a.tau_nn = nullptr;
     a.tau_next = b.tau_next + im.mat_off;
     a.mean = nullptr;

@@ -145,6 +145,13 @@ int aco_get_aging(aco_ctx *c, int32_t *age, uint8_t *valid, int32_t *perf, int64
                   int32_t *scalars);
 int aco_set_aging(aco_ctx *c, const int32_t *age, const uint8_t *valid, const int32_t *perf,
                   const int64_t *prev_len, const int32_t *scalars);
+/* Heuristic on the candidate edges, n*k values in candidate-list order:
+ * replaces eta^beta on NN-list edges only (fallback steps keep plain
+ * eta^beta); NULL restores eta^beta.  No counterpart in the reference: the
+ * entry point for a learned edge prior (aco_prior_apply writes it on the
+ * device). */
+int aco_set_heuristic_nn(aco_ctx *c, const float *h);
+int aco_get_heuristic_nn(aco_ctx *c, float *h);
 /* best-so-far length/tour overwrite (parity replays) */
 int aco_set_best(aco_ctx *c, int64_t glob_best, const int32_t *tour);
 /* counters = {fallback steps, unvisited cities scanned by fallback steps} since create */
@@ -186,6 +193,8 @@ int aco_batch_get_pheromone(aco_batch *b, int32_t instance, float *tau);      /*
 int aco_batch_set_pheromone(aco_batch *b, int32_t instance, const float *tau);
 /* config = {ants per block, persistent blocks, work items, LDS bytes per block} */
 int aco_batch_get_config(aco_batch *b, int32_t *config);
+/* candidate-edge heuristic of one instance (n[i] * k values), as aco_set_heuristic_nn */
+int aco_batch_set_heuristic_nn(aco_batch *b, int32_t instance, const float *h);
 /* Times construction launches for each ants-per-block choice (a launch
  * parameter only: tours do not change) and keeps the fastest; reps launches
  * per choice; chosen (may be NULL) gets the choice. */

@@ -223,7 +223,7 @@ int32_t orc_select_candidate(const float w[64], float u, float *total_out) {
  * columns 256s+4l+q (q = 0..3, s ascending) sequentially, then the 64 lane
  * partials are combined by orc_wave_scan. */
 void orc_sweep(int32_t n, int32_t k, const float *tau, const int32_t *dist,
-               const float *eta, int32_t eta_len, const int32_t *nn,
+               const float *eta, int32_t eta_len, const int32_t *nn, const float *heur_nn,
                float alpha, float rho, int32_t want_mean,
                float *choice, float *choice_nn, float *tau_evap, float *mean) {
     const float one_minus_rho = 1.0f - rho;
@@ -254,8 +254,11 @@ void orc_sweep(int32_t n, int32_t k, const float *tau, const int32_t *dist,
             mean[i] = acc[63] / (float)n;
         }
         if (choice_nn && choice && nn)
-            for (int32_t r = 0; r < k; r++)
-                choice_nn[(int64_t)i * k + r] = choice[(int64_t)i * n + nn[(int64_t)i * k + r]];
+            for (int32_t r = 0; r < k; r++) {
+                int64_t e = (int64_t)i * k + r;
+                choice_nn[e] = heur_nn ? apply_alpha(trow[nn[e]], alpha) * heur_nn[e]
+                                       : choice[(int64_t)i * n + nn[e]];
+            }
     }
 }
 
@@ -292,7 +295,7 @@ static int32_t farthest_unvisited(const int32_t *nnrow, const uint8_t *vis, int3
 
 void orc_construct(int32_t mode, int32_t n, int32_t m, int32_t k,
                    const int32_t *dist, const int32_t *nn, const int32_t *nnfull,
-                   const float *choice, const float *choice_nn,
+                   const float *choice, const float *choice_nn, const float *heur_nn,
                    const float *tau, const float *mean,
                    const float *eta, int32_t eta_len, float alpha,
                    const int32_t *age, const uint8_t *valid, int32_t converged,
@@ -325,7 +328,18 @@ void orc_construct(int32_t mode, int32_t n, int32_t m, int32_t k,
                     w[r] = 0.0f;
                     if (r < k) {
                         int32_t j = nn[(int64_t)c * k + r];
-                        if (!vis[j]) w[r] = wc.young ? edge_weight(&wc, c, j) : choice_nn[(int64_t)c * k + r];
+                        if (!vis[j]) {
+                            if (!wc.young) {
+                                w[r] = choice_nn[(int64_t)c * k + r];
+                            } else if (heur_nn) {  /* young weight with the candidate-edge heuristic */
+                                int64_t xy = (int64_t)c * n + j;
+                                float mc = mean[c];
+                                float p = ((tau[xy] - mc) * wc.ratio) + mc;
+                                w[r] = apply_alpha(p, alpha) * heur_nn[(int64_t)c * k + r];
+                            } else {
+                                w[r] = edge_weight(&wc, c, j);
+                            }
+                        }
                     }
                 }
                 int32_t sel = orc_select_candidate(w, u, NULL);
@@ -665,21 +679,21 @@ double orc_run(int32_t n, int32_t m, const int32_t *dist, const int32_t *params,
     double t0 = now_seconds();
     for (int32_t it = 0; it < iters; it++) {
         if (aging) {
-            orc_sweep(n, 0, tau, dist, eta, eta_len, NULL, alpha, rho, 1, NULL, NULL, NULL, mean);
+            orc_sweep(n, 0, tau, dist, eta, eta_len, NULL, NULL, alpha, rho, 1, NULL, NULL, NULL, mean);
             orc_aging_start(m, age);
         }
         if (mode == ORC_MODE_REFSEQ) {
             if (!aging)
-                orc_sweep(n, 0, tau, dist, eta, eta_len, NULL, alpha, rho, 0, choice, NULL, NULL, NULL);
-            orc_construct(mode, n, m, k, dist, NULL, nn, choice, NULL, tau, mean, eta, eta_len,
+                orc_sweep(n, 0, tau, dist, eta, eta_len, NULL, NULL, alpha, rho, 0, choice, NULL, NULL, NULL);
+            orc_construct(mode, n, m, k, dist, NULL, nn, choice, NULL, NULL, tau, mean, eta, eta_len,
                           alpha, aging ? age : NULL, aging ? valid : NULL, converged, min_age,
                           seed, (uint32_t)it, stream, tours, lengths, NULL);
         } else {
-            orc_sweep(n, k, tau, dist, eta, eta_len, mode == ORC_MODE_CANDIDATE ? nn : NULL,
+            orc_sweep(n, k, tau, dist, eta, eta_len, mode == ORC_MODE_CANDIDATE ? nn : NULL, NULL,
                       alpha, rho, 0, choice, mode == ORC_MODE_CANDIDATE ? choice_nn : NULL, NULL,
                       NULL);
             orc_construct(mode, n, m, k, dist, mode == ORC_MODE_CANDIDATE ? nn : NULL,
-                          mode == ORC_MODE_FULLROW ? nn : NULL, choice, choice_nn, tau, mean, eta,
+                          mode == ORC_MODE_FULLROW ? nn : NULL, choice, choice_nn, NULL, tau, mean, eta,
                           eta_len, alpha, aging ? age : NULL, aging ? valid : NULL, converged,
                           min_age, seed, (uint32_t)it, stream, tours, lengths, NULL);
         }

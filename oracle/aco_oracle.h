@@ -63,8 +63,10 @@ float orc_eta(const float *table, int32_t table_len, int32_t d);
 void orc_wave_scan(float v[64]);
 int32_t orc_select_candidate(const float w[64], float u, float *total_out);
 
+/* heur_nn (n*k, may be NULL): heuristic on candidate edges replacing eta
+ * for choice_nn (a learned prior); the full choice matrix keeps eta. */
 void orc_sweep(int32_t n, int32_t k, const float *tau, const int32_t *dist,
-               const float *eta, int32_t eta_len, const int32_t *nn,
+               const float *eta, int32_t eta_len, const int32_t *nn, const float *heur_nn,
                float alpha, float rho, int32_t want_mean,
                float *choice, float *choice_nn, float *tau_evap, float *mean);
 
@@ -74,7 +76,7 @@ void orc_sweep(int32_t n, int32_t k, const float *tau, const int32_t *dist,
  * unvisited cities scanned by fallback steps]. */
 void orc_construct(int32_t mode, int32_t n, int32_t m, int32_t k,
                    const int32_t *dist, const int32_t *nn, const int32_t *nnfull,
-                   const float *choice, const float *choice_nn,
+                   const float *choice, const float *choice_nn, const float *heur_nn,
                    const float *tau, const float *mean,
                    const float *eta, int32_t eta_len, float alpha,
                    const int32_t *age, const uint8_t *valid, int32_t converged,

@@ -64,8 +64,8 @@ def _declare(L):
     L.orc_wave_scan.argtypes = [vp]
     L.orc_select_candidate.argtypes = [vp, f32, vp]
     L.orc_select_candidate.restype = i32
-    L.orc_sweep.argtypes = [i32, i32, vp, vp, vp, i32, vp, f32, f32, i32, vp, vp, vp, vp]
-    L.orc_construct.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32,
+    L.orc_sweep.argtypes = [i32, i32, vp, vp, vp, i32, vp, vp, f32, f32, i32, vp, vp, vp, vp]
+    L.orc_construct.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32,
                                 vp, vp, i32, i32, u64, u32, u32, vp, vp, vp]
     L.orc_stats.argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.orc_stats.restype = i32
@@ -159,7 +159,7 @@ def select_candidate(w, u):
     return int(r), float(tot[0])
 
 
-def sweep(tau, dist, eta, nn, alpha, rho, want_mean=False):
+def sweep(tau, dist, eta, nn, alpha, rho, want_mean=False, heur_nn=None):
     tau = np.ascontiguousarray(tau, dtype=np.float32)
     dist = np.ascontiguousarray(dist, dtype=np.int32)
     n = tau.shape[0]
@@ -169,13 +169,14 @@ def sweep(tau, dist, eta, nn, alpha, rho, want_mean=False):
     tau_evap = np.empty((n, n), dtype=np.float32)
     mean = np.zeros(n, dtype=np.float32)
     nn_c = None if nn is None else np.ascontiguousarray(nn, dtype=np.int32)
-    lib().orc_sweep(n, k, _p(tau), _p(dist), _p(eta), len(eta), _p(nn_c), float(alpha), float(rho),
+    h = None if heur_nn is None else np.ascontiguousarray(heur_nn, dtype=np.float32)
+    lib().orc_sweep(n, k, _p(tau), _p(dist), _p(eta), len(eta), _p(nn_c), _p(h), float(alpha), float(rho),
                     int(want_mean), _p(choice), _p(choice_nn) if k else None, _p(tau_evap), _p(mean))
     return choice, (choice_nn[:, :k] if k else None), tau_evap, mean
 
 
 def construct(mode, dist, nn, choice, choice_nn, tau, mean, eta, alpha, m, seed, it,
-              stream=0, age=None, valid=None, converged=0, min_age=1, tours=None, lengths=None):
+              stream=0, age=None, valid=None, converged=0, min_age=1, tours=None, lengths=None, heur_nn=None):
     dist = np.ascontiguousarray(dist, dtype=np.int32)
     n = dist.shape[0]
     k = nn.shape[1] if nn is not None else 0
@@ -193,7 +194,8 @@ def construct(mode, dist, nn, choice, choice_nn, tau, mean, eta, alpha, m, seed,
     tau_c = np.ascontiguousarray(tau, dtype=np.float32) if tau is not None else None
     mean_c = np.ascontiguousarray(mean, dtype=np.float32) if mean is not None else None
     ch = np.ascontiguousarray(choice, dtype=np.float32) if choice is not None else None
-    lib().orc_construct(mode, n, m, k, _p(dist), _p(cand), _p(full), _p(ch), _p(cnn), _p(tau_c),
+    h = None if heur_nn is None else np.ascontiguousarray(heur_nn, dtype=np.float32)
+    lib().orc_construct(mode, n, m, k, _p(dist), _p(cand), _p(full), _p(ch), _p(cnn), _p(h), _p(tau_c),
                         _p(mean_c), _p(eta), len(eta), float(alpha), _p(age_c), _p(val_c),
                         int(converged), int(min_age), int(seed), int(it), int(stream), _p(tours),
                         _p(lengths), _p(fb))

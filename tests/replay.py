@@ -38,7 +38,9 @@ def snapshot(col: capi.Colony) -> dict:
     }
 
 
-def replay(p: capi.Params, dist: np.ndarray, snap: dict) -> dict:
+def replay(p: capi.Params, dist: np.ndarray, snap: dict, heur_nn: np.ndarray | None = None) -> dict:
+    """heur_nn: the candidate-edge heuristic installed with set_heuristic_nn
+    (None = the default eta^beta)."""
     n, m = p.n, p.m
     aging = bool(p.aging)
     cand = p.mode == capi.MODE_CANDIDATE
@@ -49,14 +51,14 @@ def replay(p: capi.Params, dist: np.ndarray, snap: dict) -> dict:
     if aging:
         orc.aging_start(age)
     choice, choice_nn, tau_evap, mean = orc.sweep(snap["tau"], dist, eta, nn if cand else None,
-                                                   p.alpha, p.rho, want_mean=aging)
+                                                   p.alpha, p.rho, want_mean=aging, heur_nn=heur_nn)
     tours = snap["tours"].copy()
     lengths = snap["lengths"].copy()
     tours, lengths, fb = orc.construct(
         orc.MODE_CANDIDATE if cand else orc.MODE_FULLROW, dist, nn, choice, choice_nn, snap["tau"], mean,
         eta, p.alpha, m, p.seed, snap["iter"], p.stream_id, age=age if aging else None,
         valid=valid if aging else None, converged=snap["converged"], min_age=p.min_age,
-        tours=tours, lengths=lengths)
+        tours=tours, lengths=lengths, heur_nn=heur_nn)
     perf = snap["perf"].copy()
     prev_len = snap["prev_len"].copy()
     st = np.array([MAXD - 1, snap["glob_best"], -1, snap["gperf"]], dtype=np.int64)

@@ -139,6 +139,41 @@ def test_rbas_replay_forced_young_state(path):
         check_iteration(col, p, dist, snap, replay(p, dist, snap))
 
 
+@pytest.mark.parametrize("path", PATHS)
+def test_heuristic_override_replay(path):
+    """A candidate-edge heuristic replaces eta^beta on NN edges (old and young
+    ants); the fallback keeps eta^beta; None restores the default."""
+    say(f"case: heuristic override RBAS aging forced state n=300 m=64 k=16 path={path}")
+    rng = np.random.default_rng(31)
+    xy = instances.uniform(300, 3000, seed=31)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=300, m=64, k=16, w=4, beta=2.0, rho=0.25, variant=capi.VARIANT_RBAS,
+                            aging=1, min_age=10, max_age=12, global_perf=5, min_pop=60, seed=123,
+                            construct_path=path)
+    with make(p, xy=xy) as col:
+        default = col.heuristic_nn()  # eta^beta; pinned by the replay after the restore below
+        h =np.exp(rng.uniform(np.log(1e-6), np.log(1.0), size=(300, 16))).astype(np.float32)
+        h[:, 0] = 0.0  # zero weights: never chosen by roulette
+        col.set_heuristic_nn(h)
+        np.testing.assert_array_equal(col.heuristic_nn(), h)
+        col.set_pheromone(random_tau(rng, 300))
+        age = rng.integers(0, 14, size=64).astype(np.int32)
+        valid = (rng.random(64) < 0.8).astype(np.uint8)
+        col.set_aging(age=age, valid=valid, perf=np.zeros(64, np.int32), prev_len=np.zeros(64, np.int64),
+                      scalars=np.array([int(valid.sum()), 1, 4, 17], np.int32))
+        col.set_best(10 ** 9, np.arange(300, dtype=np.int32))
+        for _ in range(2):
+            snap = snapshot(col)
+            col.iterate()
+            ref = replay(p, dist, snap, heur_nn=h)
+            check_iteration(col, p, dist, snap, ref)
+        col.set_heuristic_nn(None)
+        np.testing.assert_array_equal(col.heuristic_nn(), default)
+        snap = snapshot(col)
+        col.iterate()
+        check_iteration(col, p, dist, snap, replay(p, dist, snap))
+
+
 @pytest.mark.parametrize("n,m,seed", [(70, 20, 1), (300, 24, 2)])
 def test_fullrow_iteration_replay(n, m, seed):
     say(f"case: full-row AS n={n} m={m} seed={seed}")
