@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result", "-Wno-unused-value", "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["aco_kernels.hip", "aco_capi.hip", "aco_batch.hip"]
+LIB_SOURCES = ["aco_kernels.hip", "aco_capi.hip", "aco_batch.hip", "aco_prior.hip"]
 
 
 def _run(cmd):

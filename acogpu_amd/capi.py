@@ -87,7 +87,43 @@ SIGNATURES = {
     "aco_batch_last_error": (ctypes.c_char_p, [_vp]),
     "aco_last_error": (ctypes.c_char_p, [_vp]),
     "aco_version": (ctypes.c_char_p, []),
+    "aco_prior_config_init": (None, [_vp]),
+    "aco_prior_create": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_get_params": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_set_params": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_collect": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_collect_batch": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_clear": (ctypes.c_int, [_vp]),
+    "aco_prior_size": (ctypes.c_int64, [_vp]),
+    "aco_prior_train": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_prior_apply": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_apply_batch": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_features": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "aco_prior_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
+    "aco_prior_gradient": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "aco_prior_adam_step": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_destroy": (ctypes.c_int, [_vp]),
+    "aco_prior_last_error": (ctypes.c_char_p, [_vp]),
 }
+
+PRIOR_PARAMS = 4801
+
+
+class PriorConfig(ctypes.Structure):
+    """aco_prior_config (include/acogpu.h)."""
+    _fields_ = [("seed", ctypes.c_uint64), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("pos_weight", ctypes.c_float),
+                ("gamma", ctypes.c_float), ("minibatch", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8)]
+
+
+def default_prior_config(**kw) -> PriorConfig:
+    c = PriorConfig()
+    lib().aco_prior_config_init(ctypes.byref(c))
+    for key, val in kw.items():
+        if not hasattr(c, key):
+            raise AttributeError(key)
+        setattr(c, key, val)
+    return c
 
 
 def lib():
@@ -372,3 +408,94 @@ class Batch:
         t = np.zeros(4, dtype=np.int32)
         self._chk(self._L.aco_batch_get_config(self._ctx, _p(t)))
         return {"ants_per_block": int(t[0]), "blocks": int(t[1]), "items": int(t[2]), "lds_bytes": int(t[3])}
+
+
+class Prior:
+    """One aco_prior: the learned edge prior (MLP on candidate-edge features)."""
+
+    def __init__(self, config: PriorConfig | None = None):
+        self._L = lib()
+        self.config = config if config is not None else default_prior_config()
+        self._ctx = ctypes.c_void_p()
+        rc = self._L.aco_prior_create(ctypes.byref(self.config), ctypes.byref(self._ctx))
+        if rc != 0:
+            raise AcoError(f"aco_prior_create failed ({rc}): {self._L.aco_prior_last_error(None).decode()}")
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise AcoError(f"({rc}) {self._L.aco_prior_last_error(self._ctx).decode()}")
+
+    def close(self):
+        if self._ctx:
+            self._L.aco_prior_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def params(self):
+        t = np.zeros(PRIOR_PARAMS, dtype=np.float32)
+        self._chk(self._L.aco_prior_get_params(self._ctx, _p(t)))
+        return t
+
+    def set_params(self, params):
+        t = np.ascontiguousarray(params, dtype=np.float32)
+        assert t.shape == (PRIOR_PARAMS,)
+        self._chk(self._L.aco_prior_set_params(self._ctx, _p(t)))
+
+    def collect(self, colony: "Colony"):
+        self._chk(self._L.aco_prior_collect(self._ctx, colony._ctx))
+
+    def collect_batch(self, batch: "Batch"):
+        self._chk(self._L.aco_prior_collect_batch(self._ctx, batch._ctx))
+
+    def clear(self):
+        self._chk(self._L.aco_prior_clear(self._ctx))
+
+    def size(self) -> int:
+        return int(self._L.aco_prior_size(self._ctx))
+
+    def train(self, epochs: int):
+        loss = np.zeros(max(epochs, 1), dtype=np.float32)
+        self._chk(self._L.aco_prior_train(self._ctx, int(epochs), _p(loss)))
+        return loss[:epochs]
+
+    def apply(self, colony: "Colony"):
+        self._chk(self._L.aco_prior_apply(self._ctx, colony._ctx))
+
+    def apply_batch(self, batch: "Batch"):
+        self._chk(self._L.aco_prior_apply_batch(self._ctx, batch._ctx))
+
+    def features(self, colony: "Colony"):
+        n, k = colony.n, colony.params.k
+        f = np.zeros((n, k, 8), dtype=np.float32)
+        y = np.zeros((n, k), dtype=np.float32)
+        self._chk(self._L.aco_prior_features(self._ctx, colony._ctx, _p(f), _p(y)))
+        return f, y
+
+    def forward(self, x):
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros(len(xx), dtype=np.float32)
+        self._chk(self._L.aco_prior_forward(self._ctx, _p(xx), len(xx), _p(out)))
+        return out
+
+    def gradient(self, x, y):
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 8)
+        yy = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        g = np.zeros(PRIOR_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._chk(self._L.aco_prior_gradient(self._ctx, _p(xx), _p(yy), len(xx), _p(g), ctypes.byref(loss)))
+        return float(loss.value), g
+
+    def adam_step(self, grad):
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        self._chk(self._L.aco_prior_adam_step(self._ctx, _p(g)))

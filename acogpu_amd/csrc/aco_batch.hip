@@ -14,6 +14,7 @@
 
 #include "../../include/acogpu.h"
 #include "aco_kernels.h"
+#include "aco_prior.h"
 
 using aco::BatchBuffers;
 using aco::DevState;
@@ -476,5 +477,29 @@ extern "C" int aco_batch_set_heuristic_nn(aco_batch* b, int32_t i, const float* 
     if (rc || !h) return rc ? rc : ACO_E_ARG;
     const InstMeta& im = b->meta[i];
     BHIP(b, hipMemcpy(b->buf.eta_nn + im.nk_off, h, sizeof(float) * im.n * b->k, hipMemcpyHostToDevice));
+    return ACO_OK;
+}
+
+int aco::batch_count(aco_batch* b) { return b ? b->count : 0; }
+
+// Device arrays of one batch instance for the learned edge prior.
+int aco::batch_view(aco_batch* b, int32_t i, aco::InstanceView* v, char* err, size_t errlen) {
+    if (!b || i < 0 || i >= b->count) {
+        snprintf(err, errlen, "batch instance %d out of range", i);
+        return ACO_E_ARG;
+    }
+    if (!b->initialized) {
+        snprintf(err, errlen, "batch not initialized");
+        return ACO_E_STATE;
+    }
+    const InstMeta& im = b->meta[i];
+    v->in.dist = b->buf.dist + im.mat_off;
+    v->in.nn = b->buf.nn + im.nk_off;
+    v->in.n = im.n;
+    v->in.P = im.P;
+    v->in.k = b->k;
+    v->best_tour = b->buf.best_tours + im.city_off;
+    v->eta_nn = b->buf.eta_nn + im.nk_off;
+    v->stream = b->stream;
     return ACO_OK;
 }

@@ -14,6 +14,7 @@
 
 #include "../../include/acogpu.h"
 #include "aco_kernels.h"
+#include "aco_prior.h"
 
 using aco::DevState;
 
@@ -854,4 +855,26 @@ extern "C" int aco_get_heuristic_nn(aco_ctx* c, float* h) {
     if (!c || !h) return ACO_E_ARG;
     if (c->k == 0) return fail(c, ACO_E_STATE, "no candidate lists in full-row mode");
     return copy_out(c, h, c->d_eta_nn, sizeof(float) * (size_t)c->n * c->k);
+}
+
+// Device arrays of a colony for the learned edge prior (aco_prior.hip).
+int aco::colony_view(aco_ctx* c, aco::InstanceView* v, char* err, size_t errlen) {
+    if (!c) return ACO_E_ARG;
+    if (c->k == 0) {
+        snprintf(err, errlen, "the prior needs candidate lists (full-row mode has none)");
+        return ACO_E_STATE;
+    }
+    if (!c->initialized) {
+        snprintf(err, errlen, "colony not initialized");
+        return ACO_E_STATE;
+    }
+    v->in.dist = c->d_dist;
+    v->in.nn = c->d_nn;
+    v->in.n = c->n;
+    v->in.P = c->P;
+    v->in.k = c->k;
+    v->best_tour = c->d_best_tour;
+    v->eta_nn = c->d_eta_nn;
+    v->stream = c->stream;
+    return ACO_OK;
 }

@@ -202,6 +202,63 @@ int aco_batch_autotune(aco_batch *b, int32_t reps, int32_t *chosen);
 int aco_batch_destroy(aco_batch *b);
 const char *aco_batch_last_error(const aco_batch *b);
 
+/* ------------------------------------------------------------------------
+ * Learned edge prior.  No counterpart in the reference (its heuristic is
+ * eta^beta only, src/AntSystem.cu:56-77).  An MLP 8 -> 64 -> 64 -> 1 with
+ * GELU, bf16 weights/activations and fp32 accumulation scores every
+ * candidate edge (i, nn[i][r]) from 8 features of the distance matrix; the
+ * heuristic sigmoid(score)^gamma then replaces eta^beta on the candidate
+ * edges (aco_set_heuristic_nn), fallback steps keep eta^beta.  Training data
+ * are candidate edges labelled 1 when they lie on a solver's best-so-far
+ * tour; training runs on the device (weighted binary cross-entropy, Adam).
+ *
+ * Edge features, d = dist(i,j), D_i^q = distance to the q-th neighbour of i:
+ *   f0 d/(D_i^{k-1}+1)  f1 d/(D_j^{k-1}+1)  f2 r/k  f3 (rank of i in NN(j),
+ *   k if absent)/k  f4 d n/(sum_i D_i^0 + n)  f5 D_i^0/(D_i^{k-1}+1)
+ *   f6 D_j^0/(D_j^{k-1}+1)  f7 (d-D_i^0)/(D_i^{k-1}-D_i^0+1).
+ * Parameters: ACO_PRIOR_PARAMS floats, W1 (64x8, row = output) at 0, b1 at
+ * 512, W2 (64x64) at 576, b2 at 4672, w3 (64) at 4736, b3 at 4800. */
+#define ACO_PRIOR_PARAMS 4801
+#define ACO_PRIOR_FEATURES 8
+
+typedef struct aco_prior aco_prior;
+typedef struct aco_prior_config {
+    uint64_t seed;      /* weight initialisation (uniform Glorot, zero biases) */
+    float lr;           /* Adam step size */
+    float beta1, beta2, eps;
+    float pos_weight;   /* loss weight of positive (on-tour) edges */
+    float gamma;        /* heuristic = sigmoid(score)^gamma */
+    int32_t minibatch;  /* edges per Adam step */
+    int32_t reserved[8];
+} aco_prior_config;
+
+void aco_prior_config_init(aco_prior_config *cfg);
+int aco_prior_create(const aco_prior_config *cfg, aco_prior **out);
+int aco_prior_get_params(aco_prior *pr, float *params);
+int aco_prior_set_params(aco_prior *pr, const float *params); /* resets Adam moments */
+/* Appends the candidate edges of a colony / every batch instance, labelled
+ * by its best-so-far tour, to the training set (device resident). */
+int aco_prior_collect(aco_prior *pr, aco_ctx *c);
+int aco_prior_collect_batch(aco_prior *pr, aco_batch *b);
+int aco_prior_clear(aco_prior *pr);
+int64_t aco_prior_size(const aco_prior *pr);
+/* `epochs` passes of minibatch Adam over the training set in collection
+ * order; loss (may be NULL) gets the mean loss of each epoch. */
+int aco_prior_train(aco_prior *pr, int32_t epochs, float *loss);
+/* Writes the prior heuristic into the candidate edges of an initialized
+ * colony / every batch instance (after aco_initialize, which resets them). */
+int aco_prior_apply(aco_prior *pr, aco_ctx *c);
+int aco_prior_apply_batch(aco_prior *pr, aco_batch *b);
+/* Host-array taps: features (n*k*8) and labels (n*k) of a colony's edges;
+ * scores of `count` feature rows; loss gradient (no update); one Adam step
+ * from a given gradient. */
+int aco_prior_features(aco_prior *pr, aco_ctx *c, float *features, float *labels);
+int aco_prior_forward(aco_prior *pr, const float *x, int64_t count, float *scores);
+int aco_prior_gradient(aco_prior *pr, const float *x, const float *y, int64_t count, float *grad, float *loss);
+int aco_prior_adam_step(aco_prior *pr, const float *grad);
+int aco_prior_destroy(aco_prior *pr);
+const char *aco_prior_last_error(const aco_prior *pr);
+
 /* Colony::~Colony */
 int aco_destroy(aco_ctx *c);
 

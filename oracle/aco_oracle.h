@@ -114,6 +114,18 @@ int32_t orc_greedy_length(const int32_t *dist, int32_t n, int32_t start);
 double orc_run(int32_t n, int32_t m, const int32_t *dist, const int32_t *params,
                const float *fparams, uint64_t seed, int64_t *out, int32_t *best_tour);
 
+/* Learned edge prior (prior_oracle.c). */
+void orc_prior_features(int32_t n, int32_t k, const int32_t *dist, const int32_t *nn, float *feat);
+void orc_prior_labels(int32_t n, int32_t k, const int32_t *nn, const int32_t *tour, float *y);
+double orc_bf16(double x);
+double orc_gelu(double x);
+double orc_gelu_grad(double x);
+void orc_mlp_forward(const float *params, const float *x, int64_t count, int32_t bf16, double *out);
+double orc_mlp_grad(const float *params, const float *x, const float *y, int64_t count, float pos_weight,
+                    int32_t bf16, double *grad);
+void orc_adam_step(int32_t nparams, float *params, const float *grad, float *m, float *v, int32_t step, float lr,
+                   float beta1, float beta2, float eps);
+
 #ifdef __cplusplus
 }
 #endif

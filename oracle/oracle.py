@@ -76,6 +76,15 @@ def _declare(L):
     L.orc_greedy_length.argtypes = [vp, i32, i32]
     L.orc_greedy_length.restype = i32
     L.orc_run.argtypes = [i32, i32, vp, vp, vp, u64, vp, vp]
+    L.orc_prior_features.argtypes = [i32, i32, vp, vp, vp]
+    L.orc_prior_labels.argtypes = [i32, i32, vp, vp, vp]
+    for name in ("orc_bf16", "orc_gelu", "orc_gelu_grad"):
+        getattr(L, name).argtypes = [f64]
+        getattr(L, name).restype = f64
+    L.orc_mlp_forward.argtypes = [vp, vp, ctypes.c_int64, i32, vp]
+    L.orc_mlp_grad.argtypes = [vp, vp, vp, ctypes.c_int64, f32, i32, vp]
+    L.orc_mlp_grad.restype = f64
+    L.orc_adam_step.argtypes = [i32, vp, vp, vp, vp, i32, f32, f32, f32, f32]
     L.orc_run.restype = f64
 
 
@@ -257,3 +266,64 @@ def run(dist, m, *, variant=AS, mode=MODE_CANDIDATE, k=20, w=6, aging=False, min
     if secs < 0:
         raise MemoryError("oracle run: allocation failed")
     return secs, int(out[0]), int(out[1]), best
+
+
+# ---------------------------------------------------------------------------
+# Learned edge prior (prior_oracle.c)
+
+PRIOR_PARAMS = 4801
+
+
+def prior_features(dist, nn):
+    n, k = nn.shape
+    d = np.ascontiguousarray(dist, dtype=np.int32)
+    q = np.ascontiguousarray(nn, dtype=np.int32)
+    out = np.zeros((n, k, 8), dtype=np.float32)
+    lib().orc_prior_features(n, k, _p(d), _p(q), _p(out))
+    return out
+
+
+def prior_labels(nn, tour):
+    n, k = nn.shape
+    q = np.ascontiguousarray(nn, dtype=np.int32)
+    t = np.ascontiguousarray(tour, dtype=np.int32)
+    out = np.zeros((n, k), dtype=np.float32)
+    lib().orc_prior_labels(n, k, _p(q), _p(t), _p(out))
+    return out
+
+
+def bf16(x):
+    return lib().orc_bf16(float(x))
+
+
+def gelu(x):
+    return lib().orc_gelu(float(x))
+
+
+def gelu_grad(x):
+    return lib().orc_gelu_grad(float(x))
+
+
+def mlp_forward(params, x, bf16_emulation=True):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros(len(xx), dtype=np.float64)
+    lib().orc_mlp_forward(_p(p), _p(xx), len(xx), int(bf16_emulation), _p(out))
+    return out
+
+
+def mlp_grad(params, x, y, pos_weight=1.0, bf16_emulation=True):
+    """Returns (loss, gradient) of the mean weighted BCE with logits."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 8)
+    yy = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+    g = np.zeros(PRIOR_PARAMS, dtype=np.float64)
+    loss = lib().orc_mlp_grad(_p(p), _p(xx), _p(yy), len(xx), float(pos_weight), int(bf16_emulation), _p(g))
+    return loss, g
+
+
+def adam_step(params, grad, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """In place on float32 params, m, v."""
+    g = np.ascontiguousarray(grad, dtype=np.float32)
+    lib().orc_adam_step(len(params), _p(params), _p(g), _p(m), _p(v), int(step), float(lr), float(beta1),
+                        float(beta2), float(eps))

@@ -71,3 +71,16 @@ def test_exactly_one_input_required():
     assert b"exactly one" in L.aco_last_error(None)
     assert L.aco_destroy(None) == 0
     assert L.aco_version().startswith(b"acogpu_amd")
+
+
+def test_prior_config_layout_and_validation():
+    # uint64 seed, 6 floats, int32 minibatch, int32 reserved[8] -> 68 bytes, 8-aligned
+    assert ctypes.sizeof(capi.PriorConfig) == 72
+    c = capi.default_prior_config()
+    assert c.minibatch == 65536 and c.gamma == 1.0 and c.pos_weight == 1.0
+    L = capi.lib()
+    out = ctypes.c_void_p()
+    bad = capi.default_prior_config(lr=0.0)
+    assert L.aco_prior_create(ctypes.byref(bad), ctypes.byref(out)) == -1  # ACO_E_ARG
+    assert b"invalid prior configuration" in L.aco_prior_last_error(None)
+    assert not out.value

@@ -1,0 +1,174 @@
+"""GPU parity of the learned edge prior against the CPU oracle.
+
+Features, labels and Adam are bit-exact.  The MLP runs bf16 MFMA with fp32
+accumulation; the oracle evaluates the same bf16 roundings in fp64, so the
+remaining differences are fp32 accumulation order and the rare bf16 rounding
+flip it causes.  Tolerances (stated at each assertion):
+  forward   |gpu - oracle| <= 1e-4 (1 + |oracle|) for >= 99.5% of edges,
+            <= 2e-2 (1 + |oracle|) for all
+  gradient  relative L2 error <= 5e-3 per parameter block, loss rel 1e-5
+"""
+import numpy as np
+import pytest
+from scipy.special import ndtr
+
+import oracle as orc
+from acogpu_amd import capi, instances
+from replay import check_iteration, replay, snapshot
+
+pytestmark = pytest.mark.gpu
+
+W1, B1, W2, B2, W3, B3 = 0, 512, 576, 4672, 4736, 4800
+BLOCKS = {"W1": (W1, B1), "b1": (B1, W2), "W2": (W2, B2), "b2": (B2, W3), "w3": (W3, B3), "b3": (B3, B3 + 1)}
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def random_params(rng, scale=0.3):
+    return (rng.standard_normal(orc.PRIOR_PARAMS) * scale).astype(np.float32)
+
+
+def random_features(rng, count):
+    return (rng.random((count, 8)) * 2).astype(np.float32)
+
+
+def test_features_and_labels_bit_exact():
+    say("case: prior features/labels n=300 k=16 after 5 AS iterations")
+    xy = instances.uniform(300, 2000, seed=17)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=300, m=50, k=16, beta=2.0, rho=0.3, seed=9)
+    with capi.Colony(p, xy=xy) as col, capi.Prior() as pr:
+        col.initialize()
+        col.run(5)
+        f, y = pr.features(col)
+        nn = orc.nn_lists(dist, 16)
+        np.testing.assert_array_equal(f, orc.prior_features(dist, nn))
+        np.testing.assert_array_equal(y, orc.prior_labels(nn, col.best_tour()))
+        assert 0 < y.sum() <= 2 * 300
+
+
+def test_forward_one_path_closed_form_on_mfma():
+    """Single nonzero path through W1[3][5] and W2[7][3]: pins the MFMA operand
+    layout (a transposed operand reads another input or unit)."""
+    say("case: prior forward one-path network, 37 edges")
+    a, b, c, d0, w, e = 1.25, -0.5, 0.75, 0.125, -1.5, 0.25
+    params = np.zeros(orc.PRIOR_PARAMS, np.float32)
+    params[W1 + 3 * 8 + 5], params[B1 + 3], params[W2 + 7 * 64 + 3] = a, b, c
+    params[B2 + 7], params[W3 + 7], params[B3] = d0, w, e
+    x = random_features(np.random.default_rng(3), 37)
+    with capi.Prior() as pr:
+        pr.set_params(params)
+        got = pr.forward(x)
+    want = orc.mlp_forward(params, x, True)
+    g = lambda v: v * ndtr(v)
+    exact = w * g(c * g(a * x[:, 5].astype(np.float64) + b) + d0) + e
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(got, exact, rtol=2e-2, atol=2e-2)  # bf16 activations
+
+
+@pytest.mark.parametrize("count", [1, 16, 1007, 40000])
+def test_forward_matches_oracle(count):
+    say(f"case: prior forward random network count={count}")
+    rng = np.random.default_rng(count)
+    params = random_params(rng)
+    x = random_features(rng, count)
+    with capi.Prior() as pr:
+        pr.set_params(params)
+        got = pr.forward(x).astype(np.float64)
+    want = orc.mlp_forward(params, x, True)
+    err = np.abs(got - want) / (1 + np.abs(want))
+    assert np.mean(err <= 1e-4) >= 0.995, f"only {np.mean(err <= 1e-4):.4f} within 1e-4"
+    assert err.max() <= 2e-2, f"max err {err.max():.3e}"
+
+
+@pytest.mark.parametrize("count,pos_weight", [(5, 1.0), (4096 + 5, 2.5), (30000, 1.0)])
+def test_gradient_matches_oracle(count, pos_weight):
+    say(f"case: prior gradient count={count} pos_weight={pos_weight}")
+    rng = np.random.default_rng(100 + count)
+    params = random_params(rng)
+    x = random_features(rng, count)
+    y = (rng.random(count) < 0.25).astype(np.float32)
+    cfg = capi.default_prior_config(pos_weight=pos_weight)
+    with capi.Prior(cfg) as pr:
+        pr.set_params(params)
+        loss, g = pr.gradient(x, y)
+    oloss, og = orc.mlp_grad(params, x, y, pos_weight, bf16_emulation=True)
+    assert loss == pytest.approx(oloss, rel=1e-5)
+    for name, (lo, hi) in BLOCKS.items():
+        ref = og[lo:hi]
+        rel = np.linalg.norm(g[lo:hi] - ref) / max(np.linalg.norm(ref), 1e-30)
+        assert rel <= 5e-3, f"{name}: relative L2 error {rel:.3e}"
+
+
+def test_adam_steps_bit_exact():
+    say("case: prior Adam, 3 steps")
+    rng = np.random.default_rng(5)
+    params = random_params(rng)
+    cfg = capi.default_prior_config(lr=3e-3)
+    p = params.copy()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    with capi.Prior(cfg) as pr:
+        pr.set_params(params)
+        for step in range(1, 4):
+            g = (rng.standard_normal(orc.PRIOR_PARAMS) * 0.05).astype(np.float32)
+            pr.adam_step(g)
+            orc.adam_step(p, g, m, v, step, 3e-3)
+            np.testing.assert_array_equal(pr.params(), p)
+
+
+def test_apply_writes_heuristic_and_iteration_replays():
+    say("case: prior apply to colony n=400 k=20 gamma=2, then one replayed iteration")
+    xy = instances.uniform(400, 4000, seed=23)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=400, m=64, k=20, beta=2.0, rho=0.2, seed=5)
+    rng = np.random.default_rng(23)
+    params = random_params(rng, 0.2)
+    with capi.Colony(p, xy=xy) as col, capi.Prior(capi.default_prior_config(gamma=2.0)) as pr:
+        col.initialize()
+        pr.set_params(params)
+        pr.apply(col)
+        h = col.heuristic_nn()
+        nn = orc.nn_lists(dist, 20)
+        logits = orc.mlp_forward(params, orc.prior_features(dist, nn).reshape(-1, 8), True)
+        want = np.maximum((1 / (1 + np.exp(-logits))) ** 2.0, 1e-12).reshape(400, 20)
+        np.testing.assert_allclose(h, want, rtol=2e-3)
+        snap = snapshot(col)
+        col.iterate()
+        check_iteration(col, p, dist, snap, replay(p, dist, snap, heur_nn=h))
+
+
+def test_training_on_batch_reduces_loss_and_ranks_tour_edges():
+    say("case: prior training on a batch of 48 instances n=120, held-out AUC")
+    sizes = np.full(48, 120, np.int32)
+    xys = [instances.uniform(120, 1000, seed=500 + i) for i in range(48)]
+    bp = capi.default_params(n=0, m=0, k=12, beta=3.0, rho=0.1, variant=capi.VARIANT_RBAS,
+                             tau0_policy=capi.TAU0_RBAS, w=6, seed=1)
+    cfg = capi.default_prior_config(minibatch=2048, lr=3e-3)
+    with capi.Batch(bp, sizes, xys) as b, capi.Prior(cfg) as pr:
+        b.initialize()
+        b.run(40)
+        pr.collect_batch(b)
+        assert pr.size() == 48 * 120 * 12
+        loss = pr.train(8)
+        say(f"loss per epoch {np.round(loss, 4).tolist()}")
+        assert loss[-1] < loss[0]
+        # held-out instance: scores rank on-tour candidate edges high
+        xy = instances.uniform(120, 1000, seed=999)
+        p = capi.default_params(n=120, m=120, k=12, beta=3.0, rho=0.1, variant=capi.VARIANT_RBAS,
+                                tau0_policy=capi.TAU0_RBAS, w=6, seed=2)
+        with capi.Colony(p, xy=xy) as col:
+            col.initialize()
+            col.run(40)
+            f, y = pr.features(col)
+            s = pr.forward(f.reshape(-1, 8))
+        y = y.reshape(-1)
+        order = np.argsort(s, kind="stable")
+        ranks = np.empty(len(s))
+        ranks[order] = np.arange(1, len(s) + 1)
+        pos = y > 0
+        auc = (ranks[pos].sum() - pos.sum() * (pos.sum() + 1) / 2) / (pos.sum() * (~pos).sum())
+        say(f"held-out AUC {auc:.4f}")
+        assert auc > 0.85
