@@ -102,6 +102,7 @@ SIGNATURES = {
     "aco_prior_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
     "aco_prior_gradient": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
     "aco_prior_adam_step": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_profile": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_prior_destroy": (ctypes.c_int, [_vp]),
     "aco_prior_last_error": (ctypes.c_char_p, [_vp]),
 }
@@ -113,7 +114,8 @@ class PriorConfig(ctypes.Structure):
     """aco_prior_config (include/acogpu.h)."""
     _fields_ = [("seed", ctypes.c_uint64), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
                 ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("pos_weight", ctypes.c_float),
-                ("gamma", ctypes.c_float), ("minibatch", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8)]
+                ("gamma", ctypes.c_float), ("minibatch", ctypes.c_int32), ("transform", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 7)]
 
 
 def default_prior_config(**kw) -> PriorConfig:
@@ -375,6 +377,11 @@ class Batch:
         self._chk(self._L.aco_batch_autotune(self._ctx, int(reps), ctypes.byref(w)))
         return w.value
 
+    def best_lengths(self):
+        gb = np.zeros(self.count, dtype=np.int64)
+        self._chk(self._L.aco_batch_get_best(self._ctx, _p(gb), None))
+        return gb
+
     def best(self):
         gb = np.zeros(self.count, dtype=np.int64)
         tours = np.zeros(int(self.n.sum()), dtype=np.int32)
@@ -495,6 +502,12 @@ class Prior:
         loss = ctypes.c_float()
         self._chk(self._L.aco_prior_gradient(self._ctx, _p(xx), _p(yy), len(xx), _p(g), ctypes.byref(loss)))
         return float(loss.value), g
+
+    def profile(self, reps=20):
+        """(forward ms, backward ms, edges) per launch over one minibatch."""
+        t = np.zeros(3, dtype=np.float32)
+        self._chk(self._L.aco_prior_profile(self._ctx, int(reps), _p(t)))
+        return float(t[0]), float(t[1]), int(t[2])
 
     def adam_step(self, grad):
         g = np.ascontiguousarray(grad, dtype=np.float32)

@@ -25,6 +25,8 @@ enum : int32_t {
     MLP_W3 = 4736,   // 64
     MLP_B3 = 4800,   // 1
     MLP_PARAMS = 4801,
+    MLP_STRIDE = 4802,      // partial-gradient row: parameters + loss
+    MLP_MAX_WAVES = 1024,   // backward waves (partial rows) per launch
 };
 static_assert(MLP_PARAMS == ACO_PRIOR_PARAMS, "parameter count");
 
@@ -49,11 +51,14 @@ int batch_view(aco_batch* b, int32_t i, InstanceView* v, char* err, size_t errle
 hipError_t launch_features(const PriorInstance& in, unsigned long long* sum, float* feat, hipStream_t s);
 hipError_t launch_labels(const PriorInstance& in, const int32_t* tour, int32_t* pos, float* y, hipStream_t s);
 hipError_t launch_mlp_forward(const MlpParams& p, const float* x, int64_t count, float* out, hipStream_t s);
+// grad (MLP_PARAMS) is overwritten, *loss accumulates; partial holds
+// (MLP_MAX_WAVES + MLP_MAX_WAVES / 16) x MLP_STRIDE floats of scratch.
 hipError_t launch_mlp_backward(const MlpParams& p, const float* x, const float* y, int64_t count, float pos_weight,
-                               float* grad, float* loss, hipStream_t s);
+                               float* partial, float* grad, float* loss, hipStream_t s);
 hipError_t launch_adam(const MlpParams& p, float* grad, float* m, float* v, float lr, float beta1, float beta2,
                        float eps, float bc1, float bc2, hipStream_t s);
 hipError_t launch_pack_bf16(const MlpParams& p, hipStream_t s);
-hipError_t launch_logit_to_heuristic(const float* logit, int64_t count, float gamma, float* h, hipStream_t s);
+hipError_t launch_logit_to_heuristic(const float* logit, int64_t count, float gamma, int32_t transform, float* h,
+                                     hipStream_t s);
 
 }  // namespace aco

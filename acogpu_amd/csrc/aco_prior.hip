@@ -128,13 +128,33 @@ __global__ __launch_bounds__(256) void k_labels(const PriorInstance in, const in
 // MLP tiles.  One wave handles 16 edges per tile.  Weights (bf16): W1 is
 // 64 x 16 (inputs 8..15 zero), W2 64 x 64, row = output feature.
 struct MlpRegs {
-    bf16x4 w1a[4];      // layer 1 A operands, output block ob
-    bf16x4 w2a[4][4];   // layer 2 A operands [ob][ib]
-    float b1[4][4];     // bias of output feature 16 ob + 4 (l/16) + q
-    float b2[4][4];
-    float w3[4][4];
-    float b3;
+    bf16x4 w1a[4];     // layer 1 A operands, output block ob
+    bf16x4 w2a[4][4];  // layer 2 A operands [ob][ib]
 };
+
+// b1, b2, w3 (64 each) and b3, staged in LDS by every MLP kernel.
+enum : int { VEC_B1 = 0, VEC_B2 = 64, VEC_W3 = 128, VEC_B3 = 192, VEC_LEN = 196 };
+
+__device__ __forceinline__ void stage_vectors(const MlpParams& p, float* vec) {
+    for (int i = threadIdx.x; i < VEC_LEN; i += blockDim.x) {
+        float v = 0.0f;
+        if (i < 64)
+            v = p.master[MLP_B1 + i];
+        else if (i < 128)
+            v = p.master[MLP_B2 + i - 64];
+        else if (i < 192)
+            v = p.master[MLP_W3 + i - 128];
+        else if (i == 192)
+            v = p.master[MLP_B3];
+        vec[i] = v;
+    }
+    __syncthreads();
+}
+
+// Four consecutive vector entries of output features 16 ob + 4 (l/16) + q.
+__device__ __forceinline__ float4 vec4(const float* vec, int base, int ob, int grp) {
+    return *reinterpret_cast<const float4*>(vec + base + 16 * ob + 4 * grp);
+}
 
 __device__ __forceinline__ void load_mlp(const MlpParams& p, MlpRegs& R, int lane) {
     const int row = lane & 15, grp = lane >> 4;
@@ -145,14 +165,7 @@ __device__ __forceinline__ void load_mlp(const MlpParams& p, MlpRegs& R, int lan
             const uint16_t* w2 = p.w2_bf16 + (size_t)(16 * ob + row) * 64 + 16 * ib + 4 * grp;
             R.w2a[ob][ib] = bf16x4{(short)w2[0], (short)w2[1], (short)w2[2], (short)w2[3]};
         }
-        for (int q = 0; q < 4; ++q) {
-            const int o = 16 * ob + 4 * grp + q;
-            R.b1[ob][q] = p.master[MLP_B1 + o];
-            R.b2[ob][q] = p.master[MLP_B2 + o];
-            R.w3[ob][q] = p.master[MLP_W3 + o];
-        }
     }
-    R.b3 = p.master[MLP_B3];
 }
 
 // Input tile as the layer-1 B operand: features 4 (l/16) + q of edge l % 16.
@@ -171,11 +184,16 @@ struct Activations {
     float out;  // logit of edge l % 16 (valid in every lane group)
 };
 
-__device__ __forceinline__ void forward_tile(const MlpRegs& R, bf16x4 xb, Activations& A) {
+__device__ __forceinline__ void forward_tile(const MlpRegs& R, const float* vec, bf16x4 xb, Activations& A,
+                                             int grp) {
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
     for (int ob = 0; ob < 4; ++ob) {
         f32x4 z = mfma16(R.w1a[ob], xb, zero);
-        for (int q = 0; q < 4; ++q) z[q] = z[q] + R.b1[ob][q];
+        const float4 b = vec4(vec, VEC_B1, ob, grp);
+        z[0] = z[0] + b.x;
+        z[1] = z[1] + b.y;
+        z[2] = z[2] + b.z;
+        z[3] = z[3] + b.w;
         A.z1[ob] = z;
         A.h1[ob] = pack4(gelu(z[0]), gelu(z[1]), gelu(z[2]), gelu(z[3]));
     }
@@ -183,27 +201,37 @@ __device__ __forceinline__ void forward_tile(const MlpRegs& R, bf16x4 xb, Activa
     for (int ob = 0; ob < 4; ++ob) {
         f32x4 z = zero;
         for (int ib = 0; ib < 4; ++ib) z = mfma16(R.w2a[ob][ib], A.h1[ib], z);
-        for (int q = 0; q < 4; ++q) z[q] = z[q] + R.b2[ob][q];
+        const float4 b = vec4(vec, VEC_B2, ob, grp);
+        z[0] = z[0] + b.x;
+        z[1] = z[1] + b.y;
+        z[2] = z[2] + b.z;
+        z[3] = z[3] + b.w;
         A.z2[ob] = z;
         A.h2[ob] = pack4(gelu(z[0]), gelu(z[1]), gelu(z[2]), gelu(z[3]));
-        for (int q = 0; q < 4; ++q) partial = partial + R.w3[ob][q] * bf2f((uint16_t)A.h2[ob][q]);
+        const float4 w = vec4(vec, VEC_W3, ob, grp);
+        partial = partial + w.x * bf2f((uint16_t)A.h2[ob][0]);
+        partial = partial + w.y * bf2f((uint16_t)A.h2[ob][1]);
+        partial = partial + w.z * bf2f((uint16_t)A.h2[ob][2]);
+        partial = partial + w.w * bf2f((uint16_t)A.h2[ob][3]);
     }
     // the four lane groups hold disjoint features of the same edge
     partial = partial + __shfl_xor(partial, 16);
     partial = partial + __shfl_xor(partial, 32);
-    A.out = partial + R.b3;
+    A.out = partial + vec[VEC_B3];
 }
 
 __global__ __launch_bounds__(256) void k_mlp_forward(const MlpParams p, const float* __restrict__ x, int64_t count,
                                                      float* __restrict__ out) {
+    __shared__ __align__(16) float vec[VEC_LEN];
     const int lane = threadIdx.x & 63;
+    stage_vectors(p, vec);
     MlpRegs R;
     load_mlp(p, R, lane);
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t * 16 < count; t += waves) {
         const int64_t e0 = t * 16;
         Activations A;
-        forward_tile(R, load_x(x, e0, count, lane), A);
+        forward_tile(R, vec, load_x(x, e0, count, lane), A, lane >> 4);
         if (lane < 16 && e0 + lane < count) out[e0 + lane] = A.out;
     }
 }
@@ -222,12 +250,15 @@ __device__ __forceinline__ bf16x4 transpose16(bf16x4 v, uint16_t* scratch, int l
 }
 
 // Backward: binary cross-entropy with logits, positive weight `pos_weight`,
-// mean over `count` edges.  Gradients accumulate in fp32 (atomics per wave).
+// mean over `count` edges.  Every wave accumulates its tiles' gradients in
+// fp32 registers and stores one partial row.
 __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const float* __restrict__ x,
                                                       const float* __restrict__ y, int64_t count, float pos_weight,
-                                                      float* __restrict__ grad, float* __restrict__ loss) {
+                                                      float* __restrict__ partial) {
     __shared__ uint16_t tr[4][256];
+    __shared__ __align__(16) float vec[VEC_LEN];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane >> 4, col = lane & 15;
+    stage_vectors(p, vec);
     MlpRegs R;
     load_mlp(p, R, lane);
     // layer-2 transposed weights for dh1 = W2^T dz2: A[i][o] = W2[o][i]
@@ -254,7 +285,7 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
         const bool valid = e0 + col < count;
         const bf16x4 xb = load_x(x, e0, count, lane);
         Activations A;
-        forward_tile(R, xb, A);
+        forward_tile(R, vec, xb, A, grp);
         const float yy = valid ? y[e0 + col] : 0.0f;
         const float s = sigmoidf(A.out);
         // d loss / d logit for -w y log s - (1-y) log(1-s)
@@ -269,9 +300,11 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
         bf16x4 dz2b[4];
         for (int ob = 0; ob < 4; ++ob) {
             float dz[4];
+            const float4 w = vec4(vec, VEC_W3, ob, grp);
+            const float w3[4] = {w.x, w.y, w.z, w.w};
             for (int q = 0; q < 4; ++q) {
                 gw3[ob][q] += g * bf2f((uint16_t)A.h2[ob][q]);
-                dz[q] = g * R.w3[ob][q] * gelu_grad(A.z2[ob][q]);
+                dz[q] = g * w3[q] * gelu_grad(A.z2[ob][q]);
                 gb2[ob][q] += dz[q];
             }
             dz2b[ob] = pack4(dz[0], dz[1], dz[2], dz[3]);
@@ -304,12 +337,15 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
         const bf16x4 xt = transpose16(xb, sc, lane);
         for (int ob = 0; ob < 4; ++ob) gW1[ob] = mfma16(dz1t[ob], xt, gW1[ob]);
     }
+    // This wave's partial gradient row (params, then the loss); the rows are
+    // summed in a fixed order by k_reduce_grad1/2, so gradients are deterministic.
+    float* row = partial + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * MLP_STRIDE;
     // D layout of a gradient tile: row 4grp+q, column col
     for (int ob = 0; ob < 4; ++ob) {
         for (int q = 0; q < 4; ++q) {
             const int o = 16 * ob + 4 * grp + q;
-            if (col < 8) atomicAdd(grad + MLP_W1 + o * 8 + col, gW1[ob][q]);
-            for (int ib = 0; ib < 4; ++ib) atomicAdd(grad + MLP_W2 + o * 64 + 16 * ib + col, gW2[ob][ib][q]);
+            if (col < 8) row[MLP_W1 + o * 8 + col] = gW1[ob][q];
+            for (int ib = 0; ib < 4; ++ib) row[MLP_W2 + o * 64 + 16 * ib + col] = gW2[ob][ib][q];
         }
     }
     // bias / w3 gradients: reduce over the 16 edge lanes of each group
@@ -323,9 +359,9 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
             }
             if (col == 0) {
                 const int o = 16 * ob + 4 * grp + q;
-                atomicAdd(grad + MLP_B1 + o, v1);
-                atomicAdd(grad + MLP_B2 + o, v2);
-                atomicAdd(grad + MLP_W3 + o, v3);
+                row[MLP_B1 + o] = v1;
+                row[MLP_B2 + o] = v2;
+                row[MLP_W3 + o] = v3;
             }
         }
     for (int off = 32; off >= 1; off >>= 1) {
@@ -333,9 +369,37 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
         lsum += __shfl_xor(lsum, off);
     }
     if (lane == 0) {
-        atomicAdd(grad + MLP_B3, gb3);
-        atomicAdd(loss, lsum);
+        row[MLP_B3] = gb3;
+        row[MLP_PARAMS] = lsum;
     }
+}
+
+// Sum of the per-wave partial rows in a fixed order, in two stages.  Stage 1:
+// block (x, y) sums rows 64 y + 16 g .. +15 (g = thread / 64) of 64 columns
+// into row 4 y + g of `stage`.  Stage 2: grad = sum of the stage rows in
+// order, loss += its last column.
+__global__ __launch_bounds__(256) void k_reduce_grad1(const float* __restrict__ partial, int32_t rows,
+                                                      float* __restrict__ stage) {
+    const int32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int32_t g = threadIdx.x >> 6;
+    if (col >= MLP_STRIDE) return;
+    const int32_t r0 = blockIdx.y * 64 + g * 16;
+    float acc = 0.0f;
+    for (int32_t i = 0; i < 16; ++i)
+        if (r0 + i < rows) acc += partial[(size_t)(r0 + i) * MLP_STRIDE + col];
+    stage[(size_t)(blockIdx.y * 4 + g) * MLP_STRIDE + col] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_grad2(const float* __restrict__ stage, int32_t rows,
+                                                      float* __restrict__ grad, float* __restrict__ loss) {
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= MLP_STRIDE) return;
+    float acc = 0.0f;
+    for (int32_t r = 0; r < rows; ++r) acc += stage[(size_t)r * MLP_STRIDE + j];
+    if (j < MLP_PARAMS)
+        grad[j] = acc;
+    else
+        *loss += acc;
 }
 
 // Adam on the fp32 master parameters; bc1 = 1 - beta1^t and bc2 = 1 - beta2^t
@@ -366,12 +430,18 @@ __global__ __launch_bounds__(256) void k_pack_bf16(MlpParams p) {
     if (i < 64 * 64) p.w2_bf16[i] = f2bf(p.master[MLP_W2 + i]);
 }
 
-// Heuristic on candidate edges: sigmoid(logit)^gamma, floored at 1e-12 so
-// that no candidate weight is exactly zero.
+// Heuristic on candidate edges from the score l: transform 0 gives
+// sigmoid(l)^gamma (probability), transform 1 gives exp(gamma min(l, 80 /
+// gamma)) (odds^gamma, capped); both floored at 1e-12 so that no candidate
+// weight is exactly zero.
 __global__ __launch_bounds__(256) void k_logit_to_heuristic(const float* __restrict__ logit, int64_t count,
-                                                            float gamma, float* __restrict__ h) {
+                                                            float gamma, int32_t transform, float* __restrict__ h) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < count) h[i] = fmaxf(powf(sigmoidf(logit[i]), gamma), 1e-12f);
+    if (i < count) {
+        const float l = logit[i];
+        const float v = transform == 0 ? powf(sigmoidf(l), gamma) : expf(fminf(gamma * l, 80.0f));
+        h[i] = fmaxf(v, 1e-12f);
+    }
 }
 
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
@@ -401,10 +471,21 @@ hipError_t launch_mlp_forward(const MlpParams& p, const float* x, int64_t count,
     return hipGetLastError();
 }
 
+static unsigned backward_blocks(int64_t count) {
+    const int64_t tiles = (count + 15) / 16;
+    const int64_t waves = tiles < MLP_MAX_WAVES ? tiles : MLP_MAX_WAVES;
+    return (unsigned)((waves + 3) / 4 > 0 ? (waves + 3) / 4 : 1);
+}
+
 hipError_t launch_mlp_backward(const MlpParams& p, const float* x, const float* y, int64_t count, float pos_weight,
-                               float* grad, float* loss, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_backward, dim3(mlp_grid(count)), dim3(256), 0, s, p, x, y, count, pos_weight, grad,
-                       loss);
+                               float* partial, float* grad, float* loss, hipStream_t s) {
+    const unsigned blocks = backward_blocks(count);
+    hipLaunchKernelGGL(k_mlp_backward, dim3(blocks), dim3(256), 0, s, p, x, y, count, pos_weight, partial);
+    const int32_t rows = (int32_t)(blocks * 4);
+    const int32_t chunks = (rows + 63) / 64;
+    float* stage = partial + (size_t)MLP_MAX_WAVES * MLP_STRIDE;
+    hipLaunchKernelGGL(k_reduce_grad1, dim3(cdiv(MLP_STRIDE, 64), chunks), dim3(256), 0, s, partial, rows, stage);
+    hipLaunchKernelGGL(k_reduce_grad2, dim3(cdiv(MLP_STRIDE, 256)), dim3(256), 0, s, stage, chunks * 4, grad, loss);
     return hipGetLastError();
 }
 
@@ -420,8 +501,10 @@ hipError_t launch_pack_bf16(const MlpParams& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_logit_to_heuristic(const float* logit, int64_t count, float gamma, float* h, hipStream_t s) {
-    hipLaunchKernelGGL(k_logit_to_heuristic, dim3(cdiv(count, 256)), dim3(256), 0, s, logit, count, gamma, h);
+hipError_t launch_logit_to_heuristic(const float* logit, int64_t count, float gamma, int32_t transform, float* h,
+                                     hipStream_t s) {
+    hipLaunchKernelGGL(k_logit_to_heuristic, dim3(cdiv(count, 256)), dim3(256), 0, s, logit, count, gamma, transform,
+                       h);
     return hipGetLastError();
 }
 
@@ -443,6 +526,7 @@ struct aco_prior {
     float* d_v = nullptr;
     float* d_grad = nullptr;
     float* d_loss = nullptr;
+    float* d_partial = nullptr;  // backward partial-gradient rows
     unsigned long long* d_sum = nullptr;
     int32_t step = 0;
     // training set: features (count x 8) and labels (count)
@@ -503,7 +587,8 @@ extern "C" void aco_prior_config_init(aco_prior_config* cfg) {
     cfg->beta2 = 0.999f;
     cfg->eps = 1e-8f;
     cfg->pos_weight = 1.0f;
-    cfg->gamma = 1.0f;
+    cfg->gamma = 3.0f;
+    cfg->transform = 1;
     cfg->minibatch = 65536;
 }
 
@@ -548,6 +633,7 @@ extern "C" int aco_prior_destroy(aco_prior* pr) {
     pfree(pr->d_v);
     pfree(pr->d_grad);
     pfree(pr->d_loss);
+    pfree(pr->d_partial);
     pfree(pr->d_sum);
     pfree(pr->d_x);
     pfree(pr->d_y);
@@ -566,7 +652,8 @@ extern "C" int aco_prior_create(const aco_prior_config* cfg, aco_prior** out) {
     aco_prior_config_init(&c);
     if (cfg) c = *cfg;
     if (!(c.lr > 0.0f) || !(c.beta1 >= 0.0f && c.beta1 < 1.0f) || !(c.beta2 >= 0.0f && c.beta2 < 1.0f) ||
-        !(c.eps > 0.0f) || !(c.pos_weight > 0.0f) || !(c.gamma > 0.0f) || c.minibatch < 16)
+        !(c.eps > 0.0f) || !(c.pos_weight > 0.0f) || !(c.gamma > 0.0f) || c.minibatch < 16 ||
+        (c.transform != 0 && c.transform != 1))
         return pfail(nullptr, ACO_E_ARG, "invalid prior configuration");
     aco_prior* pr = new aco_prior();
     pr->cfg = c;
@@ -580,7 +667,8 @@ extern "C" int aco_prior_create(const aco_prior_config* cfg, aco_prior** out) {
     if ((rc = palloc(pr, &pr->d_master, ACO_PRIOR_PARAMS)) || (rc = palloc(pr, &pr->d_w1, 64 * 16)) ||
         (rc = palloc(pr, &pr->d_w2, 64 * 64)) || (rc = palloc(pr, &pr->d_m, ACO_PRIOR_PARAMS)) ||
         (rc = palloc(pr, &pr->d_v, ACO_PRIOR_PARAMS)) || (rc = palloc(pr, &pr->d_grad, ACO_PRIOR_PARAMS)) ||
-        (rc = palloc(pr, &pr->d_loss, 1)) || (rc = palloc(pr, &pr->d_sum, 1))) {
+        (rc = palloc(pr, &pr->d_loss, 1)) || (rc = palloc(pr, &pr->d_sum, 1)) ||
+        (rc = palloc(pr, &pr->d_partial, (size_t)(aco::MLP_MAX_WAVES + aco::MLP_MAX_WAVES / 16) * aco::MLP_STRIDE))) {
         snprintf(g_prior_err, sizeof(g_prior_err), "%s", pr->err);
         aco_prior_destroy(pr);
         return rc;
@@ -715,7 +803,8 @@ extern "C" int aco_prior_train(aco_prior* pr, int32_t epochs, float* loss) {
         for (int64_t off = 0; off < pr->count; off += mb, ++steps) {
             const int64_t cnt = pr->count - off < mb ? pr->count - off : mb;
             PHIP(pr, aco::launch_mlp_backward(pr->params(), pr->d_x + off * 8, pr->d_y + off, cnt,
-                                              pr->cfg.pos_weight, pr->d_grad, pr->d_loss, pr->stream));
+                                              pr->cfg.pos_weight, pr->d_partial, pr->d_grad, pr->d_loss,
+                                              pr->stream));
             int rc = adam_step(pr);
             if (rc) return rc;
         }
@@ -734,7 +823,8 @@ static int apply_view(aco_prior* pr, const InstanceView& v) {
     PHIP(pr, hipStreamSynchronize(v.stream));
     PHIP(pr, aco::launch_features(v.in, pr->d_sum, pr->d_feat, pr->stream));
     PHIP(pr, aco::launch_mlp_forward(pr->params(), pr->d_feat, edges, pr->d_logit, pr->stream));
-    PHIP(pr, aco::launch_logit_to_heuristic(pr->d_logit, edges, pr->cfg.gamma, v.eta_nn, pr->stream));
+    PHIP(pr, aco::launch_logit_to_heuristic(pr->d_logit, edges, pr->cfg.gamma, pr->cfg.transform, v.eta_nn,
+                                                pr->stream));
     return ACO_OK;
 }
 
@@ -809,8 +899,8 @@ extern "C" int aco_prior_gradient(aco_prior* pr, const float* x, const float* y,
     if ((rc = palloc(pr, &g, ACO_PRIOR_PARAMS + 1))) return rc;
     hipError_t e = hipMemsetAsync(g, 0, sizeof(float) * (ACO_PRIOR_PARAMS + 1), pr->stream);
     if (e == hipSuccess)
-        e = aco::launch_mlp_backward(pr->params(), pr->d_feat, pr->d_logit, count, pr->cfg.pos_weight, g,
-                                     g + ACO_PRIOR_PARAMS, pr->stream);
+        e = aco::launch_mlp_backward(pr->params(), pr->d_feat, pr->d_logit, count, pr->cfg.pos_weight,
+                                     pr->d_partial, g, g + ACO_PRIOR_PARAMS, pr->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(grad, g, sizeof(float) * ACO_PRIOR_PARAMS, hipMemcpyDeviceToHost, pr->stream);
     float l = 0.0f;
@@ -828,5 +918,38 @@ extern "C" int aco_prior_adam_step(aco_prior* pr, const float* grad) {
     int rc = adam_step(pr);
     if (rc) return rc;
     PHIP(pr, hipStreamSynchronize(pr->stream));
+    return ACO_OK;
+}
+
+extern "C" int aco_prior_profile(aco_prior* pr, int32_t reps, float* ms) {
+    if (!pr || !ms || reps <= 0) return ACO_E_ARG;
+    if (pr->count == 0) return pfail(pr, ACO_E_STATE, "empty training set");
+    const int64_t cnt = pr->count < pr->cfg.minibatch ? pr->count : pr->cfg.minibatch;
+    int rc = ensure_scratch(pr, cnt, 0);
+    if (rc) return rc;
+    float* g = nullptr;
+    if ((rc = palloc(pr, &g, ACO_PRIOR_PARAMS + 1))) return rc;
+    hipEvent_t ev[3] = {};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], pr->stream);
+    for (int32_t r = 0; r < reps && e == hipSuccess; ++r)
+        e = aco::launch_mlp_forward(pr->params(), pr->d_x, cnt, pr->d_logit, pr->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], pr->stream);
+    for (int32_t r = 0; r < reps && e == hipSuccess; ++r)
+        e = aco::launch_mlp_backward(pr->params(), pr->d_x, pr->d_y, cnt, pr->cfg.pos_weight, pr->d_partial, g,
+                                     g + ACO_PRIOR_PARAMS, pr->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], pr->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
+    float a = 0.0f, b = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+    for (int i = 0; i < 3; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    (void)hipFree(g);
+    if (e != hipSuccess) return pfail(pr, ACO_E_HIP, "profile failed: %s", hipGetErrorString(e));
+    ms[0] = a / (float)reps;
+    ms[1] = b / (float)reps;
+    ms[2] = (float)cnt;
     return ACO_OK;
 }

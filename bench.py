@@ -154,6 +154,125 @@ def bench_large(args):
     }
 
 
+def time_to_gap(curve, ref, gap=0.02):
+    """Per instance, the solver wall time at the first checkpoint whose
+    best-so-far length is within `gap` of ref (inf if never)."""
+    reach = np.full(len(ref), np.inf)
+    for _, secs, gb in curve:
+        hit = (gb <= (1 + gap) * ref) & np.isinf(reach)
+        reach[hit] = secs
+    return reach
+
+
+def quantile_or_none(x, q):
+    v = float(np.quantile(x, q)) if np.isfinite(x).mean() >= q else float("inf")
+    return v if np.isfinite(v) else None
+
+
+def bench_prior(args):
+    """Learned edge prior: wall time to a 2% gap, prior vs plain Ant System.
+
+    Training: 2000 instances n=200 (integer coordinates uniform in
+    [0,1000]^2), batched Ant System m=n, k=20, alpha 1, beta 5, rho 0.5,
+    tau0 = m/C_nn, 200 iterations; every candidate edge labelled by the
+    best-so-far tours; 12 epochs of Adam (minibatch 65536) on the device.
+    Evaluation: 256 instances n=1000 uniform in [0,1000]^2, m=1000, k=20,
+    alpha 1, rho 0.5, `steps` iterations of (a) plain AS beta 2, (b) plain AS
+    beta 5, (c) AS whose candidate-edge heuristic is the prior's exp(3 score)
+    (fallback steps eta^5).  Best-known per instance = shortest tour of the
+    three long runs.  Time to 2% = solver wall time (for (c) including the
+    prior's feature + forward pass) until best-so-far <= 1.02 best-known;
+    best-so-far lengths are read after every iteration up to 50, then every
+    10 iterations, outside the clock."""
+    nt, ne, iters = 2000, args.prior_instances, args.steps
+    t0 = time.perf_counter()
+    sizes = np.full(nt, 200, np.int32)
+    xys = [instances.uniform(200, 1000, seed=700000 + i) for i in range(nt)]
+    tp = capi.default_params(n=0, m=0, k=20, beta=5.0, rho=0.5, variant=capi.VARIANT_AS,
+                             tau0_policy=capi.TAU0_AS, seed=3)
+    tb = capi.Batch(tp, sizes, xys)
+    tb.initialize()
+    tb.run(200)
+    gen_s = time.perf_counter() - t0
+    pr = capi.Prior(capi.default_prior_config())
+    t0 = time.perf_counter()
+    pr.collect_batch(tb)
+    tb.close()
+    loss = pr.train(12)
+    train_s = time.perf_counter() - t0
+    fwd_ms, bwd_ms, edges = pr.profile(20)
+    log(f"bench: training data {gen_s:.2f} s, {pr.size()} edges, training {train_s:.2f} s, "
+        f"loss {loss[0]:.4f} -> {loss[-1]:.4f}, forward {fwd_ms:.3f} ms / backward {bwd_ms:.3f} ms per {edges} edges")
+    esz = np.full(ne, 1000, np.int32)
+    exy = [instances.uniform(1000, 1000, seed=900000 + i) for i in range(ne)]
+    solvers = [("plain AS beta 2", 2.0, False), ("plain AS beta 5", 5.0, False), ("AS + learned prior", 5.0, True)]
+    curves = {}
+    ms_it = {}
+    for label, beta, use_prior in solvers:
+        ep = capi.default_params(n=0, m=0, k=20, beta=beta, rho=0.5, variant=capi.VARIANT_AS,
+                                 tau0_policy=capi.TAU0_AS, seed=11)
+        e = capi.Batch(ep, esz, exy)
+        e.initialize()
+        el = 0.0
+        if use_prior:
+            t1 = time.perf_counter()
+            pr.apply_batch(e)
+            el += time.perf_counter() - t1
+        cur, it = [], 0
+        run_s = 0.0
+        while it < iters:
+            c = 1 if it < 50 else min(10, iters - it)
+            t1 = time.perf_counter()
+            e.run(c)
+            dt = time.perf_counter() - t1
+            el += dt
+            run_s += dt
+            it += c
+            cur.append((it, el, e.best_lengths()))
+            if it % 100 == 0:
+                log(f"bench: {label} iteration {it}, {el:.2f} s, mean best {cur[-1][2].mean():.1f}")
+        curves[label] = cur
+        ms_it[label] = run_s * 1e3 / iters
+        e.close()
+    pr.close()
+    ref = np.min(np.stack([c[-1][2] for c in curves.values()]), axis=0)
+    table = {}
+    for label, cur in curves.items():
+        reach = time_to_gap(cur, ref)
+        table[label] = {"t50_s": quantile_or_none(reach, 0.5), "t90_s": quantile_or_none(reach, 0.9),
+                        "t100_s": quantile_or_none(reach, 1.0), "frac_within_2pct": float(np.isfinite(reach).mean()),
+                        "ms_per_iteration": ms_it[label],
+                        "final_mean_gap_pct": float((cur[-1][2] / ref - 1).mean() * 100)}
+        log(f"bench: {label}: {table[label]}")
+    prior = table["AS + learned prior"]
+
+    def speedup(key):
+        plain = [table[k][key] for k in ("plain AS beta 2", "plain AS beta 5")]
+        best = min((t for t in plain if t is not None), default=None)
+        return best / prior[key] if (best and prior[key]) else None
+
+    flop_fwd, flop_bwd = 2 * (8 * 64 + 64 * 64 + 64), 2 * (64 * 64 * 2 + 64 * 8 + 64)
+    return {
+        "metric": "wall-seconds to 2% gap, learned prior vs plain AS", "value": prior["t50_s"], "unit": "s",
+        "n_gpus": 1, "steps": iters, "warmup": 0, "ms_per_step": prior["ms_per_iteration"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 MLP / f32 ACO",
+        "data": "synthetic",
+        "config": {"workload": f"{ne} instances n=1000 uniform [0,1000]^2, m=1000, k=20, AS alpha 1 rho 0.5, "
+                               f"{iters} iterations; prior trained on 2000 n=200 AS runs (200 iterations)",
+                   "statistic": "median over instances of the solver wall time until the instance is within 2% "
+                                "of its best-known (shortest tour of all three long runs)"},
+        "solvers": table,
+        "speedup_t50_vs_best_plain": speedup("t50_s"),
+        "speedup_t90_vs_best_plain": speedup("t90_s"),
+            "training": {"data_seconds": gen_s, "train_seconds": train_s, "edges": int(nt * 200 * 20),
+                     "epochs": 12, "loss_first": float(loss[0]), "loss_last": float(loss[-1])},
+        "mlp_kernels": {"edges_per_launch": edges, "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+                        "forward_tflops": flop_fwd * edges / (fwd_ms * 1e9),
+                        "backward_tflops": (flop_fwd + flop_bwd) * edges / (bwd_ms * 1e9),
+                        "peak_bf16_tflops": 2500.0},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,10 +282,11 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ants-per-block", type=int, default=0)
-    ap.add_argument("--workload", choices=["c2", "batched", "large"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "batched", "large", "prior"], default="c2",
                     help="c2: the headline n=1000 tours/s line (default); batched: 4096 uneven instances; "
-                         "large: n=20000")
+                         "large: n=20000; prior: learned edge prior time to 2% gap")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--prior-instances", type=int, default=256)
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
@@ -177,7 +297,7 @@ def main():
         return 2
 
     if args.workload != "c2":
-        line = bench_batched(args) if args.workload == "batched" else bench_large(args)
+        line = {"batched": bench_batched, "large": bench_large, "prior": bench_prior}[args.workload](args)
         print(json.dumps(line), flush=True)
         return 0
 

@@ -207,8 +207,9 @@ const char *aco_batch_last_error(const aco_batch *b);
  * eta^beta only, src/AntSystem.cu:56-77).  An MLP 8 -> 64 -> 64 -> 1 with
  * GELU, bf16 weights/activations and fp32 accumulation scores every
  * candidate edge (i, nn[i][r]) from 8 features of the distance matrix; the
- * heuristic sigmoid(score)^gamma then replaces eta^beta on the candidate
- * edges (aco_set_heuristic_nn), fallback steps keep eta^beta.  Training data
+ * heuristic sigmoid(score)^gamma (transform 0) or exp(gamma score)
+ * (transform 1) then replaces eta^beta on the candidate edges
+ * (aco_set_heuristic_nn); fallback steps keep eta^beta.  Training data
  * are candidate edges labelled 1 when they lie on a solver's best-so-far
  * tour; training runs on the device (weighted binary cross-entropy, Adam).
  *
@@ -227,9 +228,10 @@ typedef struct aco_prior_config {
     float lr;           /* Adam step size */
     float beta1, beta2, eps;
     float pos_weight;   /* loss weight of positive (on-tour) edges */
-    float gamma;        /* heuristic = sigmoid(score)^gamma */
+    float gamma;        /* heuristic exponent */
     int32_t minibatch;  /* edges per Adam step */
-    int32_t reserved[8];
+    int32_t transform;  /* 0: sigmoid(score)^gamma, 1: exp(gamma min(score, 80/gamma)) */
+    int32_t reserved[7];
 } aco_prior_config;
 
 void aco_prior_config_init(aco_prior_config *cfg);
@@ -256,6 +258,9 @@ int aco_prior_features(aco_prior *pr, aco_ctx *c, float *features, float *labels
 int aco_prior_forward(aco_prior *pr, const float *x, int64_t count, float *scores);
 int aco_prior_gradient(aco_prior *pr, const float *x, const float *y, int64_t count, float *grad, float *loss);
 int aco_prior_adam_step(aco_prior *pr, const float *grad);
+/* ms = {forward ms, backward ms, edges}: average of `reps` launches of each
+ * kernel over the first minibatch of the training set (HIP events). */
+int aco_prior_profile(aco_prior *pr, int32_t reps, float *ms);
 int aco_prior_destroy(aco_prior *pr);
 const char *aco_prior_last_error(const aco_prior *pr);
 

@@ -74,10 +74,10 @@ def test_exactly_one_input_required():
 
 
 def test_prior_config_layout_and_validation():
-    # uint64 seed, 6 floats, int32 minibatch, int32 reserved[8] -> 68 bytes, 8-aligned
+    # uint64 seed, 6 floats, int32 minibatch, int32 transform, int32 reserved[7] -> 68 bytes, 8-aligned
     assert ctypes.sizeof(capi.PriorConfig) == 72
     c = capi.default_prior_config()
-    assert c.minibatch == 65536 and c.gamma == 1.0 and c.pos_weight == 1.0
+    assert c.minibatch == 65536 and c.gamma == 3.0 and c.transform == 1 and c.pos_weight == 1.0
     L = capi.lib()
     out = ctypes.c_void_p()
     bad = capi.default_prior_config(lr=0.0)
