@@ -51,21 +51,47 @@ def construct_bytes_per_launch(n, k, tours, scanned):
     return tours * (n * (8 * k + 4) + 4 * n) + 4 * scanned
 
 
-def cpu_baseline(xy, iterations):
+def cpu_baseline(xy, iterations, threads=1):
     """The oracle's reference-semantics restatement (every city in NN order,
     sequential fp32 sums, farthest-unvisited fallback, aging, rank-based
-    deposit), one thread, on the same instance."""
+    deposit) on the same instance: one colony per thread (independent seeds),
+    `iterations` iterations each.  Returns (tours/s over all threads, wall
+    seconds)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
     dist = orc.dist_from_int_coords(xy)
-    n = dist.shape[0]
     cnn = orc.greedy_length(dist, 0)
     tau0 = float(np.float32(0.5 * 6 * 5 / (np.float64(np.float32(0.1)) * cnn)))
-    secs, gb, ib, _ = orc.run(dist, 1000, variant=orc.RBAS, mode=orc.MODE_REFSEQ, k=20, w=6, aging=True,
-                              min_age=30, max_age=50, global_perf=10, min_pop=40,
-                              iterations=iterations, alpha=1.0, beta=3.0, rho=0.1, tau0=tau0, seed=7)
-    return 1000 * iterations / secs, secs
+
+    def one(seed):
+        return orc.run(dist, 1000, variant=orc.RBAS, mode=orc.MODE_REFSEQ, k=20, w=6, aging=True,
+                       min_age=30, max_age=50, global_perf=10, min_pop=40,
+                       iterations=iterations, alpha=1.0, beta=3.0, rho=0.1, tau0=tau0, seed=seed)
+
+    t0 = time.perf_counter()
+    if threads == 1:
+        one(7)
+    else:  # the C oracle runs without the GIL held (ctypes)
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(one, range(7, 7 + threads)))
+    wall = time.perf_counter() - t0
+    return 1000 * iterations * threads / wall, wall
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
 
 
 def bench_batched(args):
@@ -280,6 +306,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--fullrow-iters", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ants-per-block", type=int, default=0)
     ap.add_argument("--workload", choices=["c2", "batched", "large", "prior"], default="c2",
@@ -310,6 +337,27 @@ def main():
             f"{2 * args.calibrate_copy / (ms * 1e-3) / 1e9:.1f} GB/s (read + write)")
         col.close()
         return 0
+    # CPU baselines first, before the GPU is touched
+    cpu = None
+    cpu_multi = None
+    model, ncpu = host_cpu()
+    if not args.no_cpu_baseline:
+        log(f"bench: host {model}, {ncpu} cores available")
+        log(f"bench: CPU baseline, {args.cpu_iters} iterations x {m} ants, reference semantics, 1 thread")
+        rate, secs = cpu_baseline(xy, args.cpu_iters)
+        cpu = {"value": rate, "unit": "tours/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iters} iterations x {m} ants of the same n=1000 instance, "
+                         f"reference full-row semantics (oracle refseq mode), {secs:.1f} s; host {model}, "
+                         f"{ncpu} cores available"}
+        log(f"bench: CPU {rate:.4g} tours/s ({secs:.1f} s)")
+        threads = min(16, ncpu)
+        if threads > 1:
+            rate_n, secs_n = cpu_baseline(xy, 1, threads)
+            cpu_multi = {"value": rate_n, "unit": "tours/s", "cores": threads,
+                         "sample": f"{threads} independent colonies (one per thread), 1 iteration x {m} ants each, "
+                                   f"{secs_n:.1f} s"}
+            log(f"bench: CPU {threads} threads {rate_n:.4g} tours/s ({secs_n:.1f} s)")
+
     p = c2_params(n, m)
     p.ants_per_block = args.ants_per_block
     log(f"bench: creating colony n={n} m={m} k={k} RBAS w=6 aging")
@@ -347,14 +395,19 @@ def main():
         with open(pmc) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        log(f"bench: CPU baseline, {args.cpu_iters} iterations x {m} ants, reference semantics, 1 thread")
-        rate, secs = cpu_baseline(xy, args.cpu_iters)
-        cpu = {"value": rate, "unit": "tours/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_iters} iterations x {m} ants of the same n=1000 instance, "
-                         f"reference full-row semantics (oracle refseq mode), {secs:.1f} s"}
-        log(f"bench: CPU {rate:.4g} tours/s ({secs:.1f} s)")
+    # The same colony with reference full-row semantics on the GPU (every
+    # city in NN order each step: apples-to-apples work per tour).
+    pf = c2_params(n, m)
+    pf.mode = capi.MODE_FULLROW
+    colf = capi.Colony(pf, xy=xy)
+    colf.initialize()
+    colf.run(2)
+    t0 = time.perf_counter()
+    colf.run(args.fullrow_iters)
+    wall_f = time.perf_counter() - t0
+    colf.close()
+    fullrow = {"value": m * args.fullrow_iters / wall_f, "unit": "tours/s", "iterations": args.fullrow_iters}
+    log(f"bench: full-row GPU {fullrow['value']:.4g} tours/s")
 
     line = {
         "metric": "tours/sec at n=1000",
@@ -380,6 +433,8 @@ def main():
         "kernel_ms": {"sweep": float(kms[0]), "construct": float(kms[1]), "stats": float(kms[2]),
                       "deposit": float(kms[3])},
         "cpu_baseline": cpu,
+        "cpu_multicore": cpu_multi,
+        "fullrow_gpu": fullrow,
         "best_length": gb,
     }
     print(json.dumps(line), flush=True)
