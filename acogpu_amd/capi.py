@@ -35,7 +35,7 @@ class Params(ctypes.Structure):
         ("global_perf", ctypes.c_int32), ("min_pop", ctypes.c_int32), ("max_pop", ctypes.c_int32),
         ("broker", ctypes.c_int32), ("ants_per_block", ctypes.c_int32), ("seed", ctypes.c_uint64),
         ("stream_id", ctypes.c_uint32), ("construct_path", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("deposit_path", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
     ]
 
 

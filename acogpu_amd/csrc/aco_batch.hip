@@ -117,6 +117,8 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
         return bfail(nullptr, ACO_E_ARG, "rho must be in [0,1], alpha and beta >= 0");
     if (p->ants_per_block < 0 || p->ants_per_block > 8)
         return bfail(nullptr, ACO_E_ARG, "ants_per_block must be in [0, 8]");
+    if (p->deposit_path < 0 || p->deposit_path > 2 || (p->deposit_path == 2 && p->variant != ACO_VARIANT_AS))
+        return bfail(nullptr, ACO_E_ARG, "deposit_path must be 0, 1 or 2 (2: Ant System only)");
 
     aco_batch* b = new aco_batch();
     b->p = *p;
@@ -158,7 +160,7 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     }
     const int32_t wdep = (p->variant == ACO_VARIANT_RBAS ? p->w : 1) + 2;
     b->meta.resize(count);
-    int64_t mat = 0, nk = 0, tours = 0, ants = 0, rows = 0, dep = 0;
+    int64_t mat = 0, nk = 0, tours = 0, ants = 0, rows = 0, dep = 0, adj = 0;
     for (int32_t i = 0; i < count; ++i) {
         InstMeta& im = b->meta[i];
         im.n = n[i];
@@ -172,6 +174,8 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
         im.city_off = (int32_t)rows;
         im.dep_off = (int32_t)dep;
         im.pad = 0;
+        im.adj_off = adj;
+        adj += (int64_t)im.n * 2 * im.m;
         mat += (int64_t)im.n * im.P;
         nk += (int64_t)im.n * p->k;
         tours += (int64_t)im.m * im.n;
@@ -229,6 +233,9 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     TRYB(balloc(b, &B.best_tours, (size_t)rows));
     TRYB(balloc(b, &B.dep_ant, (size_t)dep));
     TRYB(balloc(b, &B.dep_val, (size_t)dep));
+    B.adj = nullptr;
+    if (p->variant == ACO_VARIANT_AS && p->deposit_path != 1 && b->nmax <= aco::ADJ_MAX_N)
+        TRYB(balloc(b, &B.adj, (size_t)adj));
     TRYB(balloc(b, &B.state, (size_t)count));
     TRYB(balloc(b, &b->items, (size_t)ants));
     TRYB(balloc(b, &B.queue, 1));

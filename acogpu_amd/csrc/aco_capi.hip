@@ -45,6 +45,7 @@ struct aco_ctx {
     uint8_t* d_valid = nullptr;
     int32_t* d_best_tour = nullptr;
     int32_t* d_dep_ant = nullptr;
+    int32_t* d_adj = nullptr;  // Ant System adjacency slots, n x 2m (null: atomic deposit)
     float* d_dep_val = nullptr;
     DevState* d_state = nullptr;
     int64_t* d_scratch64 = nullptr;
@@ -107,7 +108,7 @@ static void free_all(aco_ctx* c) {
     void* ptrs[] = {c->d_dist, c->d_tau[0], c->d_tau[1], c->d_choice, c->d_choice_nn, c->d_nn, c->d_eta,
                     c->d_tau_nn, c->d_eta_nn, c->d_nn16,
                     c->d_mean, c->d_tours, c->d_len, c->d_prev_len, c->d_age, c->d_perf, c->d_valid,
-                    c->d_best_tour, c->d_dep_ant, c->d_dep_val, c->d_state, c->d_scratch64};
+                    c->d_best_tour, c->d_dep_ant, c->d_dep_val, c->d_state, c->d_scratch64, c->d_adj};
     for (void* q : ptrs)
         if (q) hipFree(q);
     for (auto& e : c->ev)
@@ -148,6 +149,10 @@ static int check_params(const aco_params* p) {
         return fail(nullptr, ACO_E_ARG, "ants_per_block must be in [0, 8]");
     if (p->construct_path < 0 || p->construct_path > 2)
         return fail(nullptr, ACO_E_ARG, "construct_path must be 0 (auto), 1 (global) or 2 (LDS-staged)");
+    if (p->deposit_path < 0 || p->deposit_path > 2)
+        return fail(nullptr, ACO_E_ARG, "deposit_path must be 0 (auto), 1 (atomics) or 2 (adjacency gather)");
+    if (p->deposit_path == 2 && (p->variant != ACO_VARIANT_AS || p->n > aco::ADJ_MAX_N))
+        return fail(nullptr, ACO_E_ARG, "the adjacency deposit needs the Ant System and n <= %d", aco::ADJ_MAX_N);
     return ACO_OK;
 }
 
@@ -264,6 +269,8 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     TRYB(dalloc(c, &c->d_best_tour, (size_t)c->n));
     TRYB(dalloc(c, &c->d_dep_ant, (size_t)c->m + c->p.w + 2));
     TRYB(dalloc(c, &c->d_dep_val, (size_t)c->m + c->p.w + 2));
+    if (c->p.variant == ACO_VARIANT_AS && c->n <= aco::ADJ_MAX_N && c->p.deposit_path != 1)
+        TRYB(dalloc(c, &c->d_adj, (size_t)c->n * 2 * c->m));
     TRYB(dalloc(c, &c->d_state, 1));
     TRYB(dalloc(c, &c->d_scratch64, 4));
     HIPB(hipMemsetAsync(c->d_dist, 0, nP * sizeof(int32_t), c->stream));
@@ -509,6 +516,9 @@ static hipError_t enqueue_deposit(aco_ctx* c, int cur) {
     da.n = c->n;
     da.P = c->P;
     da.dense = c->p.variant == ACO_VARIANT_AS;
+    da.adj = c->d_adj;
+    da.m = c->m;
+    if (da.dense && c->d_adj) return aco::launch_deposit_adj(da, c->stream);
     const int32_t grid = da.dense ? c->m : c->p.w + 1;
     if ((e = aco::launch_deposit(da, grid, c->stream)) != hipSuccess) return e;
     return hipSuccess;

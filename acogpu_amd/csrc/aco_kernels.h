@@ -91,7 +91,14 @@ struct DepositArgs {
     const DevState* state;
     float* tau_next;
     int32_t n, P, dense;
+    // Ant System adjacency deposit (dense list): row x of adj (2m slots)
+    // holds, for ant a, its successor (slot 2a) and predecessor (2a+1) of x.
+    int32_t* adj;
+    int32_t m;
 };
+
+// Largest n whose deposit row (n fp32) is accumulated in LDS.
+constexpr int32_t ADJ_MAX_N = 32768;
 
 // ---------------------------------------------------------------------------
 // Batched instances: all matrices of all instances live in shared buffers;
@@ -105,6 +112,7 @@ struct InstMeta {
     int32_t city_off;  // offset in per-city arrays (global row of city 0)
     int32_t dep_off;   // offset in the deposit lists (m + w + 2 per instance)
     int32_t pad;
+    int64_t adj_off;   // offset in the adjacency slots (n x 2m per instance)
 };
 
 struct BatchBuffers {
@@ -128,6 +136,7 @@ struct BatchBuffers {
     int32_t* best_tours;     // per-city layout (city_off)
     int32_t* dep_ant;
     float* dep_val;
+    int32_t* adj;            // Ant System adjacency slots (null: atomic deposit)
     DevState* state;         // one per instance
     const int2* items;       // construction work items (instance, first ant), longest first
     int32_t nitems;
@@ -161,6 +170,9 @@ hipError_t launch_nn_aux(const int32_t* nn, const int32_t* dist, const float* et
                          int32_t P, int32_t k, uint16_t* nn16, float* eta_nn, hipStream_t s);
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
 hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s);
+// Ant System deposit without global atomics: tours scattered into adj, then
+// every pheromone row gathers its deposits in LDS (needs a.adj, a.m, n <= ADJ_MAX_N).
+hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s);
 hipError_t launch_copy(const float4* src, float4* dst, int64_t count, hipStream_t s);
 
 }  // namespace aco

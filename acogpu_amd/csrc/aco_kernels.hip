@@ -1058,6 +1058,51 @@ __device__ __forceinline__ void deposit_entry(const DepositArgs& a, int32_t e) {
 
 __global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) { deposit_entry(a, blockIdx.x); }
 
+// Ant System deposit in two passes with no global atomics; the same sum as
+// deposit_entry (src/AntSystem.cu:295-317): tau_xy += dep_val[a] for every
+// ant a whose tour uses edge (x, y), in both directions.
+// Pass 1, one block per ant: ant a writes succ_a(x) to slot 2a and pred_a(y)
+// to slot 2a+1 of rows x and y of its tour edges (x, y).  Plain stores: every
+// slot has exactly one writer.
+__device__ __forceinline__ void adj_scatter_ant(const DepositArgs& a, int32_t ant) {
+    if (ant >= a.state->ndep || a.dep_val[ant] == 0.0f) return;
+    const int32_t* t = a.tours + (size_t)ant * a.n;
+    const size_t pitch = 2 * (size_t)a.m;
+    for (int32_t i = threadIdx.x; i < a.n; i += blockDim.x) {
+        const int32_t x = t[i];
+        const int32_t y = t[(i + 1 == a.n) ? 0 : i + 1];
+        a.adj[(size_t)x * pitch + 2 * ant] = y;
+        a.adj[(size_t)y * pitch + 2 * ant + 1] = x;
+    }
+}
+
+// Pass 2, one block per pheromone row x: the row's 2m slots are summed into
+// an LDS row by their ants' dep_val (slots of ants with dep_val 0 are stale
+// and skipped), then added once to tau_next where nonzero.
+__device__ __forceinline__ void adj_gather_row(const DepositArgs& a, int32_t x, float* delta) {
+    for (int32_t j = threadIdx.x; j < a.n; j += blockDim.x) delta[j] = 0.0f;
+    __syncthreads();
+    const int32_t* row = a.adj + (size_t)x * 2 * a.m;
+    const int32_t slots = 2 * min(a.m, a.state->ndep);
+    for (int32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const float d = a.dep_val[s >> 1];
+        if (d != 0.0f) atomicAdd(&delta[row[s]], d);
+    }
+    __syncthreads();
+    float* tn = a.tau_next + (size_t)x * a.P;
+    for (int32_t j = threadIdx.x; j < a.n; j += blockDim.x) {
+        const float v = delta[j];
+        if (v != 0.0f) tn[j] = tn[j] + v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adj_scatter(const DepositArgs a) { adj_scatter_ant(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_adj_gather(const DepositArgs a) {
+    extern __shared__ float delta_row[];
+    adj_gather_row(a, blockIdx.x, delta_row);
+}
+
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256)
         dst[i] = src[i];
@@ -1137,6 +1182,8 @@ __device__ __forceinline__ DepositArgs batch_deposit_args(const BatchBuffers& b,
     a.n = im.n;
     a.P = im.P;
     a.dense = b.variant == 0;
+    a.adj = b.adj ? b.adj + im.adj_off : nullptr;
+    a.m = im.m;
     return a;
 }
 
@@ -1274,6 +1321,19 @@ __global__ __launch_bounds__(256) void kb_stats(const BatchBuffers b) {
 }
 
 // AS: one block per ant of every instance; RBAS: w + 1 blocks per instance.
+__global__ __launch_bounds__(256) void kb_adj_scatter(const BatchBuffers b) {
+    const int32_t inst = b.ant_inst[blockIdx.x];
+    const InstMeta im = b.meta[inst];
+    adj_scatter_ant(batch_deposit_args(b, inst, im), blockIdx.x - im.ant_off);
+}
+
+__global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b) {
+    extern __shared__ float delta_row[];
+    const int32_t inst = b.row_inst[blockIdx.x];
+    const InstMeta im = b.meta[inst];
+    adj_gather_row(batch_deposit_args(b, inst, im), blockIdx.x - im.city_off, delta_row);
+}
+
 __global__ __launch_bounds__(256) void kb_deposit(const BatchBuffers b) {
     int32_t inst, e;
     if (b.variant == 0) {
@@ -1368,6 +1428,13 @@ hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s) {
+    if (!a.adj || !a.dense || a.n > ADJ_MAX_N) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_adj_scatter, dim3(a.m), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_adj_gather, dim3(a.n), dim3(256), (size_t)a.n * sizeof(float), s, a);
+    return hipGetLastError();
+}
+
 size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block) {
     return staged_lds_bytes(nmax, k, (nmax + 31) / 32, ants_per_block);
 }
@@ -1407,8 +1474,13 @@ hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, i
             hipLaunchKernelGGL(kb_stats, dim3(b.count), dim3(256), (size_t)grid * sizeof(uint32_t), s, b);
             break;
         default:
-            hipLaunchKernelGGL(kb_deposit, dim3(b.variant == 0 ? b.total_ants : b.count * (b.w + 1)), dim3(256), 0,
-                               s, b);
+            if (b.variant == 0 && b.adj) {
+                hipLaunchKernelGGL(kb_adj_scatter, dim3(b.total_ants), dim3(256), 0, s, b);
+                hipLaunchKernelGGL(kb_adj_gather, dim3(b.total_rows), dim3(256), (size_t)nmax * sizeof(float), s, b);
+            } else {
+                hipLaunchKernelGGL(kb_deposit, dim3(b.variant == 0 ? b.total_ants : b.count * (b.w + 1)), dim3(256),
+                                   0, s, b);
+            }
     }
     return hipGetLastError();
 }

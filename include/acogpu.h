@@ -81,7 +81,8 @@ typedef struct aco_params {
     uint64_t seed;          /* Philox key (reference: RANDOM_SEED, inc/Colony.h:15) */
     uint32_t stream_id;     /* Philox counter word 3 (instance id)            */
     int32_t construct_path; /* candidate mode: 0 auto, 1 global memory, 2 LDS-staged */
-    int32_t reserved[6];
+    int32_t deposit_path;   /* Ant System: 0 auto, 1 fp32 atomics, 2 adjacency gather (n <= 32768) */
+    int32_t reserved[5];
 } aco_params;
 
 /* Reference defaults (inc/default.h:8-39, src/Colony.cu:32-36). */

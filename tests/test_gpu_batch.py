@@ -48,10 +48,11 @@ def test_batch_first_iterations_equal_single_colonies(apb):
                 np.testing.assert_allclose(b.pheromone(i), c.pheromone(), rtol=1e-6)
 
 
-def test_batch_iteration_replay_through_oracle():
-    say("case: batch of 16, iteration 4 replayed through the oracle")
+@pytest.mark.parametrize("deposit_path", [1, 2])
+def test_batch_iteration_replay_through_oracle(deposit_path):
+    say(f"case: batch of 16, iteration 4 replayed through the oracle, deposit_path={deposit_path}")
     sizes, xys = make_batch_inputs(16, 30, 250, seed=2)
-    p = batch_params()
+    p = batch_params(deposit_path=deposit_path)
     with capi.Batch(p, sizes, xys) as b:
         b.initialize()
         b.run(3)

@@ -174,6 +174,40 @@ def test_heuristic_override_replay(path):
         check_iteration(col, p, dist, snap, replay(p, dist, snap))
 
 
+@pytest.mark.parametrize("aging", [0, 1])
+def test_as_deposit_paths_agree_and_replay(aging):
+    """Ant System deposit through fp32 atomics (path 1) and through the
+    adjacency scatter + per-row LDS gather (path 2): same tours, pheromone
+    within fp32 rounding of each other and within 1e-5 of the FP64 oracle;
+    with aging, invalid ants deposit nothing."""
+    say(f"case: AS deposit paths n=300 m=64 k=16 aging={aging}")
+    rng = np.random.default_rng(41)
+    xy = instances.uniform(300, 3000, seed=41)
+    dist = orc.dist_from_int_coords(xy)
+    tau = random_tau(rng, 300)
+    age = rng.integers(0, 14, size=64).astype(np.int32)
+    valid = (rng.random(64) < 0.7).astype(np.uint8)
+    cols = []
+    for path in (1, 2):
+        p = capi.default_params(n=300, m=64, k=16, beta=2.0, rho=0.25, variant=capi.VARIANT_AS, aging=aging,
+                                min_age=10, max_age=12, global_perf=5, min_pop=60, seed=8, deposit_path=path)
+        col = make(p, xy=xy)
+        col.set_pheromone(tau)
+        if aging:
+            col.set_aging(age=age, valid=valid, perf=np.zeros(64, np.int32), prev_len=np.zeros(64, np.int64),
+                          scalars=np.array([int(valid.sum()), 1, 4, 17], np.int32))
+        snap = snapshot(col)
+        col.iterate()
+        check_iteration(col, p, dist, snap, replay(p, dist, snap))
+        cols.append(col)
+    try:
+        np.testing.assert_array_equal(cols[0].tours(), cols[1].tours())
+        np.testing.assert_allclose(cols[0].pheromone(), cols[1].pheromone(), rtol=2e-6)
+    finally:
+        for c in cols:
+            c.close()
+
+
 @pytest.mark.parametrize("n,m,seed", [(70, 20, 1), (300, 24, 2)])
 def test_fullrow_iteration_replay(n, m, seed):
     say(f"case: full-row AS n={n} m={m} seed={seed}")
