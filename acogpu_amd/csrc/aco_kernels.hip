@@ -269,7 +269,7 @@ __device__ __forceinline__ void rng_refill(AntRng& r, int32_t s, uint32_t ant, u
 // Before using step s >= 1 (uniform branch, taken every 64 steps).
 __device__ __forceinline__ void rng_step(AntRng& r, int32_t s, uint32_t ant, uint32_t iter, const ConstructArgs& a,
                                          int lane) {
-    if ((s & 63) != 0) return;
+    if (__builtin_expect((s & 63) != 0, 1)) return;
     if ((s & 255) == 0) {
         rng_refill(r, s, ant, iter, a, lane);
     } else {  // words rotate: w1 serves steps 64..127 of the chunk, w2 the next 64, ...
@@ -574,15 +574,16 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
     } else {
         mark_visited(vis_lds, start, lane);
     }
-    const bool klane = lane < a.k;
-    const int32_t rl = klane ? lane : a.k - 1;
+    // lanes >= k read the zero weight staged after the n*k candidate entries
+    const int32_t lmul = lane < a.k ? 1 : 0;
+    const int32_t loff = lane < a.k ? lane : a.n * a.k;
     int32_t tbuf = start;
     int32_t c = start;
     unsigned long long fb = 0, scanned = 0;
     for (int32_t s = 1; s < a.n; ++s) {
         rng_step(rng, s, (uint32_t)ant, iter, a, lane);
         const float u = word_to_uniform(rng_word(rng, s));
-        const int32_t e = c * a.k + rl;
+        const int32_t e = c * a.k * lmul + loff;
         const int32_t j = nl[e];
         const float cw = wl[e];
         bool vj;
@@ -592,16 +593,18 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
         } else {
             vj = is_visited(vis_lds, j);
         }
-        const float w = (klane && !vj) ? cw : 0.0f;
+        const float w = vj ? 0.0f : cw;
         const float p = K32 ? wave_inclusive_scan32(w) : wave_inclusive_scan(w);
         const float total = readlane_f(p, K32 ? 31 : 63);
         const float rr = u * total;
-        const unsigned long long hit = __ballot(p > rr && w > 0.0f);
+        // the compare mask is the ballot (no round trip through a VGPR);
+        // the common case falls through, the rare ones are laid out apart
+        const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr && w > 0.0f);
         int32_t next;
-        if (hit) {
+        if (__builtin_expect(hit != 0ull, 1)) {
             next = readlane_i(j, __ffsll((long long)hit) - 1);
         } else if (total > 0.0f) {
-            const unsigned long long pos = __ballot(w > 0.0f);
+            const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
             next = readlane_i(j, 63 - __clzll((long long)pos));
         } else {
             ++fb;
@@ -614,7 +617,7 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             mark_visited(vis_lds, next, lane);
         }
         tbuf = (lane == (s & 63)) ? next : tbuf;
-        if ((s & 63) == 63 || s == a.n - 1) {
+        if (__builtin_expect((s & 63) == 63 || s == a.n - 1, 0)) {
             if (lane <= (s & 63)) t16[(s & ~63) + lane] = (uint16_t)tbuf;
         }
         c = next;
@@ -651,6 +654,32 @@ __device__ __forceinline__ void construct_staged_dispatch(const ConstructArgs& a
             construct_staged_ant<false, true>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
         else
             construct_staged_ant<false, false>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+    }
+}
+
+// LDS layout of the staged construction: weights (n*k fp32, then one zero
+// entry that lanes >= k read), candidate lists (n*k uint16 and one entry),
+// each region padded to 16 bytes, then per wave the visited bitmap and the
+// tour.  staged_lds_bytes() is the host-side size of the same layout.
+struct StagedLayout {
+    float* wl;
+    uint16_t* nl;
+    uint32_t* per;
+};
+
+__device__ __forceinline__ StagedLayout staged_layout(uint32_t* lds, int32_t nk) {
+    StagedLayout L;
+    L.wl = reinterpret_cast<float*>(lds);
+    L.nl = reinterpret_cast<uint16_t*>(L.wl + ((nk + 4) & ~3));
+    L.per = reinterpret_cast<uint32_t*>(L.nl + ((nk + 8) & ~7));
+    return L;
+}
+
+// The entries read by lanes >= k; staging never writes index nk.
+__device__ __forceinline__ void staged_zero_entry(const StagedLayout& L, int32_t nk) {
+    if (threadIdx.x == 0) {
+        L.wl[nk] = 0.0f;
+        L.nl[nk] = 0;
     }
 }
 
@@ -715,11 +744,10 @@ __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a)
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
     const int32_t nk = a.n * a.k;
-    // LDS layout: weights (nk fp32), candidate lists (nk uint16), each region
-    // padded to 16 bytes, then per wave the visited bitmap and the tour.
-    float* wl = reinterpret_cast<float*>(lds);
-    uint16_t* nl = reinterpret_cast<uint16_t*>(wl + ((nk + 3) & ~3));
-    uint32_t* per = reinterpret_cast<uint32_t*>(nl + ((nk + 7) & ~7));
+    const StagedLayout L = staged_layout(lds, nk);
+    float* wl = L.wl;
+    uint16_t* nl = L.nl;
+    uint32_t* per = L.per;
     const int32_t per_words = a.words + (a.n + 1) / 2;
     uint32_t* vis = per + (size_t)wave * per_words;
     uint16_t* t16 = reinterpret_cast<uint16_t*>(vis + a.words);
@@ -730,6 +758,7 @@ __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a)
     if (young_block && (!a.aging || !converged)) return;
 
     stage_u16(nl, a.nn16, nk);
+    staged_zero_entry(L, nk);
 
     // Normal blocks run one round (W ants); young block y runs the young
     // ants y, y + young_blocks, ... one per round on wave 0.
@@ -1278,13 +1307,14 @@ __global__ __launch_bounds__(512) void kb_construct(const BatchBuffers b) {
         const InstMeta im = b.meta[it.x];
         const ConstructArgs a = batch_construct_args(b, it.x, im);
         const int32_t nk = im.n * b.k;
-        float* wl = reinterpret_cast<float*>(lds);
-        uint16_t* nl = reinterpret_cast<uint16_t*>(wl + ((nk + 3) & ~3));
-        uint32_t* per = reinterpret_cast<uint32_t*>(nl + ((nk + 7) & ~7));
-        uint32_t* vis = per + (size_t)wave * (im.words + (im.n + 1) / 2);
+        const StagedLayout L = staged_layout(lds, nk);
+        float* wl = L.wl;
+        uint16_t* nl = L.nl;
+        uint32_t* vis = L.per + (size_t)wave * (im.words + (im.n + 1) / 2);
         uint16_t* t16 = reinterpret_cast<uint16_t*>(vis + im.words);
         stage_u16(nl, a.nn16, nk);
         stage_f32(wl, a.choice_nn, nk);
+        staged_zero_entry(L, nk);
         __syncthreads();
         const int32_t ant = it.y + wave;
         if (ant < im.m) construct_staged_dispatch(a, ant, false, 0.0f, wl, nl, vis, t16, a.state->iter, lane);
@@ -1390,7 +1420,7 @@ hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
 
 size_t staged_lds_bytes(int32_t n, int32_t k, int32_t words, int32_t ants_per_block) {
     const size_t nk = (size_t)n * k;
-    return ((nk + 3) & ~(size_t)3) * 4 + ((nk + 7) & ~(size_t)7) * 2 +
+    return ((nk + 4) & ~(size_t)3) * 4 + ((nk + 8) & ~(size_t)7) * 2 +
            (size_t)ants_per_block * (words + (n + 1) / 2) * 4;
 }
 
