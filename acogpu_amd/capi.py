@@ -83,6 +83,7 @@ SIGNATURES = {
     "aco_batch_set_pheromone": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_batch_get_config": (ctypes.c_int, [_vp, _vp]),
     "aco_batch_autotune": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_batch_tune_k": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _i32, _i32, ctypes.c_float, _vp, _vp]),
     "aco_batch_destroy": (ctypes.c_int, [_vp]),
     "aco_batch_last_error": (ctypes.c_char_p, [_vp]),
     "aco_last_error": (ctypes.c_char_p, [_vp]),
@@ -415,6 +416,22 @@ class Batch:
         t = np.zeros(4, dtype=np.int32)
         self._chk(self._L.aco_batch_get_config(self._ctx, _p(t)))
         return {"ants_per_block": int(t[0]), "blocks": int(t[1]), "items": int(t[2]), "lds_bytes": int(t[3])}
+
+
+def tune_k(params: Params, n_list, xy_list, ks, trial_iterations=20, tolerance=0.005, m_list=None):
+    """aco_batch_tune_k: (chosen k, report rows {ms/iteration, mean best, ants per block})."""
+    L = lib()
+    n = np.ascontiguousarray(n_list, dtype=np.int32)
+    m = None if m_list is None else np.ascontiguousarray(m_list, dtype=np.int32)
+    xy = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.int32).reshape(-1) for c in xy_list]))
+    k = np.ascontiguousarray(ks, dtype=np.int32)
+    rep = np.zeros((len(k), 3), dtype=np.float32)
+    chosen = ctypes.c_int32()
+    rc = L.aco_batch_tune_k(ctypes.byref(params), len(n), _p(n), _p(m), _p(xy), _p(k), len(k), int(trial_iterations),
+                            float(tolerance), ctypes.byref(chosen), _p(rep))
+    if rc != 0:
+        raise AcoError(f"aco_batch_tune_k failed ({rc}): {L.aco_batch_last_error(None).decode()}")
+    return int(chosen.value), rep
 
 
 class Prior:

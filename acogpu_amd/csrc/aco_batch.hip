@@ -510,3 +510,54 @@ int aco::batch_view(aco_batch* b, int32_t i, aco::InstanceView* v, char* err, si
     v->stream = b->stream;
     return ACO_OK;
 }
+
+// Candidate-list width: for each k in ks, a batch of the same instances runs
+// `trial_iterations` iterations (after the ants-per-block autotune); the
+// choice is the fastest k whose mean best tour is within `tolerance` of the
+// best mean over all k.  report (may be NULL) gets, per k, {ms per iteration,
+// mean best length, chosen ants per block}.  Unlike the ants-per-block tuner
+// this changes the algorithm (the candidate set), so it is a separate call.
+extern "C" int aco_batch_tune_k(const aco_params* p, int32_t count, const int32_t* n, const int32_t* m,
+                                const int32_t* xy, const int32_t* ks, int32_t nks, int32_t trial_iterations,
+                                float tolerance, int32_t* chosen_k, float* report) {
+    if (!p || !ks || nks < 1 || nks > 64 || trial_iterations < 1 || !(tolerance >= 0.0f) || !chosen_k)
+        return bfail(nullptr, ACO_E_ARG, "aco_batch_tune_k: bad arguments");
+    std::vector<double> ms((size_t)nks), mean((size_t)nks);
+    std::vector<int32_t> apb((size_t)nks);
+    for (int32_t t = 0; t < nks; ++t) {
+        aco_params q = *p;
+        q.k = ks[t];
+        aco_batch* b = nullptr;
+        int rc = aco_batch_create(&q, count, n, m, xy, &b);
+        if (rc) return rc;
+        int32_t w = 0;
+        float el = 0.0f;
+        if ((rc = aco_batch_initialize(b)) || (rc = aco_batch_autotune(b, 2, &w)) || (rc = aco_batch_initialize(b)) ||
+            (rc = aco_batch_run(b, trial_iterations, &el))) {
+            std::snprintf(g_batch_err, sizeof(g_batch_err), "k=%d: %s", ks[t], b->err);
+            aco_batch_destroy(b);
+            return rc;
+        }
+        std::vector<int64_t> gb((size_t)count);
+        rc = aco_batch_get_best(b, gb.data(), nullptr);
+        aco_batch_destroy(b);
+        if (rc) return rc;
+        double s = 0.0;
+        for (int64_t v : gb) s += (double)v;
+        ms[t] = el / trial_iterations;
+        mean[t] = s / count;
+        apb[t] = w;
+    }
+    const double best_mean = *std::min_element(mean.begin(), mean.end());
+    int32_t pick = -1;
+    for (int32_t t = 0; t < nks; ++t)
+        if (mean[t] <= best_mean * (1.0 + tolerance) && (pick < 0 || ms[t] < ms[pick])) pick = t;
+    *chosen_k = ks[pick];
+    if (report)
+        for (int32_t t = 0; t < nks; ++t) {
+            report[3 * t] = (float)ms[t];
+            report[3 * t + 1] = (float)mean[t];
+            report[3 * t + 2] = (float)apb[t];
+        }
+    return ACO_OK;
+}

@@ -137,6 +137,14 @@ def bench_batched(args):
         c.close()
     seq_rate = len(sample) / seq_time
     log(f"bench: sequential {len(sample)} instances {seq_time:.3f} s -> {seq_rate:.1f} instances/s")
+    tuned = None
+    if args.tune_k:
+        ks = [8, 12, 16, 20, 24]
+        k, rep = capi.tune_k(p, sizes, xys, ks, trial_iterations=20, tolerance=0.005)
+        tuned = {"chosen_k": k, "trial_iterations": 20, "tolerance": 0.005,
+                 "per_k": {str(kk): {"ms_per_iteration": float(r[0]), "mean_best": float(r[1]),
+                                     "ants_per_block": int(r[2])} for kk, r in zip(ks, rep)}}
+        log(f"bench: candidate-width tuner picks k={k}: {tuned['per_k']}")
     return {
         "metric": "instances/sec, batched uneven instances", "value": batch_rate, "unit": "instances/s",
         "n_gpus": 1, "steps": iters, "warmup": args.warmup, "ms_per_step": wall * 1e3 / iters,
@@ -147,6 +155,7 @@ def bench_batched(args):
                        "kind": "single-instance solver, one instance after another"},
         "speedup_vs_sequential": batch_rate / seq_rate,
         "mean_best_length": float(np.mean(gb)),
+        "candidate_width_tuner": tuned,
     }
 
 
@@ -314,6 +323,7 @@ def main():
                          "large: n=20000; prior: learned edge prior time to 2% gap")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--prior-instances", type=int, default=256)
+    ap.add_argument("--tune-k", type=int, default=1, help="batched: also run the candidate-width tuner (0 = off)")
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")

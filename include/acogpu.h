@@ -200,6 +200,15 @@ int aco_batch_set_heuristic_nn(aco_batch *b, int32_t instance, const float *h);
  * parameter only: tours do not change) and keeps the fastest; reps launches
  * per choice; chosen (may be NULL) gets the choice. */
 int aco_batch_autotune(aco_batch *b, int32_t reps, int32_t *chosen);
+/* Candidate-list width tuner: builds a batch of the same instances for each
+ * k in ks[nks], runs trial_iterations iterations (after aco_batch_autotune)
+ * and picks the fastest k whose mean best length is within `tolerance`
+ * (relative) of the best mean; report (may be NULL) gets nks x {ms per
+ * iteration, mean best length, ants per block}.  Changes the candidate set,
+ * hence the tours: the caller creates the production batch with chosen_k. */
+int aco_batch_tune_k(const aco_params *p, int32_t count, const int32_t *n, const int32_t *m, const int32_t *xy,
+                     const int32_t *ks, int32_t nks, int32_t trial_iterations, float tolerance, int32_t *chosen_k,
+                     float *report);
 int aco_batch_destroy(aco_batch *b);
 const char *aco_batch_last_error(const aco_batch *b);
 

@@ -131,3 +131,24 @@ def test_batch_4096_uneven_instances_properties():
             assert sorted(best[i].tolist()) == list(range(n))
             dist = orc.dist_from_int_coords(xys[i])
             assert instances.tour_length(dist, best[i]) == gb[i]
+
+
+def test_tune_k_picks_fastest_k_within_tolerance():
+    say("case: candidate-width tuner over k in {6, 10, 16, 24} on 48 instances, 8 trial iterations")
+    sizes, xys = make_batch_inputs(48, 40, 300, seed=6)
+    p = batch_params()
+    ks = [6, 10, 16, 24]
+    k, rep = capi.tune_k(p, sizes, xys, ks, trial_iterations=8, tolerance=0.01)
+    say(f"chosen k={k}; report {rep.tolist()}")
+    assert k in ks
+    assert (rep[:, 0] > 0).all() and (rep[:, 1] > 0).all() and set(rep[:, 2].tolist()) <= {1.0, 2.0, 4.0, 8.0}
+    ok = rep[:, 1] <= rep[:, 1].min() * 1.01 * (1 + 1e-6)
+    assert ok[ks.index(k)]
+    assert rep[ks.index(k), 0] <= rep[ok, 0].min() * (1 + 1e-6)
+    # the trial means are those of ordinary batches with that k
+    q = batch_params(k=k)
+    with capi.Batch(q, sizes, xys) as b:
+        b.initialize()
+        b.run(8)
+        gb, _ = b.best()
+    assert abs(gb.mean() - rep[ks.index(k), 1]) <= 0.02 * gb.mean()

@@ -368,3 +368,20 @@ def test_candidate_rbas_run_respects_tsplib_optimum(inst):
                                  iterations=300, beta=3.0, rho=0.1, tau0=tau0, seed=1)
     assert gb == instances.tour_length(d, best)
     assert opt <= gb <= 1.06 * opt
+
+
+def test_c1_refseq_fixture_reproduces():
+    """The committed 30-seed quality fixture is what the oracle computes
+    (first seed re-run)."""
+    import json
+    import os
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_c1_refseq as mk
+
+    ref = json.load(open(os.path.join(here, "golden", "c1_refseq.json")))
+    _, dist, tau0 = mk.c1_instance()
+    assert np.float32(tau0) == np.float32(ref["tau0"])
+    assert mk.refseq_best(dist, tau0, ref["seeds"][0]) == ref["best"][0]
