@@ -49,6 +49,7 @@ SIGNATURES = {
     "aco_initialize": (ctypes.c_int, [_vp]),
     "aco_iterate": (ctypes.c_int, [_vp, _vp]),
     "aco_run": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_run_until": (ctypes.c_int, [_vp, _i32, _i64, _i32, _vp, _vp]),
     "aco_phase": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_get_stats": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "aco_get_best_tour": (ctypes.c_int, [_vp, _vp]),
@@ -211,6 +212,13 @@ class Colony:
         ms = ctypes.c_float()
         self._chk(self._L.aco_run(self._ctx, int(iterations), ctypes.byref(ms)))
         return ms.value
+
+    def run_until(self, max_iterations, stop_at, chunk=32):
+        """(iterations run, halted) of aco_run_until."""
+        ran, halted = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self._L.aco_run_until(self._ctx, int(max_iterations), int(stop_at), int(chunk), ctypes.byref(ran),
+                                        ctypes.byref(halted)))
+        return ran.value, bool(halted.value)
 
     def profile(self, iterations):
         out = np.zeros(4, dtype=np.float32)

@@ -312,6 +312,7 @@ extern "C" int aco_batch_initialize(aco_batch* b) {
         s.iter_best = (int64_t)0x7fffffff - 1;
         s.glob_best = (int64_t)0x7fffffff;
         s.iter_best_ant = -1;
+        s.stop_at = -1;
     }
     BHIP(b, hipMemcpyAsync(b->buf.state, st.data(), sizeof(DevState) * b->count, hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t), b->stream));

@@ -345,6 +345,9 @@ static int reset_state(aco_ctx* c) {
     st.num_valid = c->m;  // intialpopulation = numAnts (src/Colony.cu:132)
     st.converged = 0;
     st.ndep = 0;
+    st.stop_at = -1;
+    st.halted = 0;
+    st.skip = 0;
     HIP_TRY(c, hipMemcpyAsync(c->d_state, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
     std::vector<uint8_t> ones((size_t)c->m, 1);
     HIP_TRY(c, hipMemcpyAsync(c->d_valid, ones.data(), (size_t)c->m, hipMemcpyHostToDevice, c->stream));
@@ -422,6 +425,7 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur) {
     sa.alpha = c->p.alpha;
     sa.one_minus_rho = 1.0f - c->p.rho;
     sa.want_mean = c->p.aging;
+    sa.state = c->d_state;
     if ((e = aco::launch_sweep(sa, c->stream)) != hipSuccess) return e;
     return hipSuccess;
 }
@@ -611,6 +615,51 @@ extern "C" int aco_run(aco_ctx* c, int32_t iterations, float* elapsed_ms) {
     HIP_TRY(c, hipEventRecord(c->ev[5], c->stream));
     HIP_TRY(c, hipEventSynchronize(c->ev[5]));
     if (elapsed_ms) HIP_TRY(c, hipEventElapsedTime(elapsed_ms, c->ev[0], c->ev[5]));
+    return ACO_OK;
+}
+
+static int copy_in(aco_ctx* c, void* dst, const void* src, size_t bytes);
+static int read_state(aco_ctx* c, DevState* st);
+
+// TSP::execute's loop (src/TSP.cpp:23-79, termination src/TSP.cpp:168-171)
+// without a host round trip per iteration: the statistics kernel halts the
+// run after the first iteration whose best <= stop_at and the remaining
+// graph-replayed iterations are skipped on the device; the host looks at the
+// state once per `chunk` iterations.
+extern "C" int aco_run_until(aco_ctx* c, int32_t max_iterations, int64_t stop_at, int32_t chunk,
+                             int32_t* iterations_run, int32_t* halted) {
+    if (!c) return ACO_E_ARG;
+    if (!c->initialized) return fail(c, ACO_E_STATE, "aco_run_until before aco_initialize");
+    if (max_iterations < 0 || chunk < 1) return fail(c, ACO_E_ARG, "negative iteration count or chunk < 1");
+    if (c->next_phase != ACO_PHASE_PROBABILITIES) return fail(c, ACO_E_STATE, "aco_run_until inside a phased iteration");
+    DevState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const uint32_t iter0 = st.iter;
+    const int cur0 = c->cur;
+    st.stop_at = stop_at < 0 ? -1 : stop_at;
+    st.halted = 0;
+    st.skip = 0;
+    if ((rc = copy_in(c, c->d_state, &st, sizeof(st)))) return rc;
+    int32_t left = max_iterations;
+    while (left > 0) {
+        const int32_t step = left < chunk ? left : chunk;
+        if ((rc = aco_run(c, step, nullptr))) return rc;
+        left -= step;
+        if ((rc = read_state(c, &st))) return rc;
+        if (st.halted) break;
+    }
+    const int32_t ran = (int32_t)(st.iter - iter0);
+    const int32_t was_halted = st.halted;
+    // skipped iterations wrote nothing: the current pheromone buffer follows
+    // the iterations that ran
+    c->cur = (cur0 + ran) & 1;
+    st.stop_at = -1;
+    st.halted = 0;
+    st.skip = 0;
+    if ((rc = copy_in(c, c->d_state, &st, sizeof(st)))) return rc;
+    if (iterations_run) *iterations_run = ran;
+    if (halted) *halted = was_halted;
     return ACO_OK;
 }
 

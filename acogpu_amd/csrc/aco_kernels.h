@@ -21,6 +21,9 @@ struct DevState {
     int32_t ndep;            // entries in the deposit list
     unsigned long long fb_steps;    // fallback steps taken
     unsigned long long fb_scanned;  // unvisited cities read by fallback steps
+    int64_t stop_at;         // halt after the first iteration whose best <= stop_at (-1: never)
+    int32_t halted;          // set by the statistics kernel; cleared by the host
+    int32_t skip;            // the current iteration is skipped (the sweep copies halted)
 };
 
 struct SweepArgs {
@@ -41,6 +44,7 @@ struct SweepArgs {
     int32_t n, P;
     float alpha, one_minus_rho;
     int32_t want_mean;
+    DevState* state;         // single colony: halting flags (null in a batch)
 };
 
 struct ConstructArgs {

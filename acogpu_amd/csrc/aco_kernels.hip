@@ -197,6 +197,13 @@ __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lan
 
 __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
     const int lane = threadIdx.x & 63;
+    // A halted run skips whole iterations: the sweep, first kernel of an
+    // iteration, publishes the decision for the other three.
+    if (a.state != nullptr) {
+        const int32_t h = a.state->halted;
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.state->skip = h;
+        if (h) return;
+    }
     // updateAntAges (src/Colony.cu:353-364): every ant ages by one at the
     // start of an iteration, before construction reads the ages.
     if (a.age != nullptr)
@@ -391,6 +398,7 @@ __device__ __forceinline__ bool ant_is_young(const ConstructArgs& a, int32_t ant
 // completes under the next step's candidate loads).
 __global__ __launch_bounds__(512) void k_construct_cand(const ConstructArgs a) {
     extern __shared__ __align__(16) uint32_t lds[];
+    if (a.state->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int32_t ant = blockIdx.x * (blockDim.x >> 6) + wave;
@@ -740,6 +748,7 @@ __device__ __forceinline__ void stage_young(float* wl, const ConstructArgs& a, f
 __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a) {
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ int32_t s_ant;
+    if (a.state->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
@@ -822,6 +831,7 @@ __global__ __launch_bounds__(256) void k_nn_aux(const int32_t* __restrict__ nn, 
 // bitmap, weights and prefixes of the whole row, chunk offsets.
 __global__ __launch_bounds__(512) void k_construct_full(const ConstructArgs a) {
     extern __shared__ __align__(16) uint32_t lds[];
+    if (a.state->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int32_t ant = blockIdx.x * (blockDim.x >> 6) + wave;
@@ -1056,7 +1066,12 @@ __device__ void stats_block(const StatsArgs& a, LenKey* red, int32_t* redi, uint
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) st->iter += 1;
+    if (threadIdx.x == 0) {
+        st->iter += 1;
+        // TSP::terminationCondition's optimum test (src/TSP.cpp:168-171) on the
+        // iteration best, evaluated on the device
+        if (st->stop_at >= 0 && st->iter_best <= st->stop_at) st->halted = 1;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
@@ -1064,6 +1079,7 @@ __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
     __shared__ int32_t redi[16];
     __shared__ int32_t s_improved, s_recruit;
     extern __shared__ uint32_t taken[];  // RBAS: ants already ranked
+    if (a.state->skip) return;
     stats_block(a, red, redi, taken, s_improved, s_recruit);
 }
 
@@ -1085,7 +1101,10 @@ __device__ __forceinline__ void deposit_entry(const DepositArgs& a, int32_t e) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) { deposit_entry(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
+    if (a.state->skip) return;
+    deposit_entry(a, blockIdx.x);
+}
 
 // Ant System deposit in two passes with no global atomics; the same sum as
 // deposit_entry (src/AntSystem.cu:295-317): tau_xy += dep_val[a] for every
@@ -1125,10 +1144,14 @@ __device__ __forceinline__ void adj_gather_row(const DepositArgs& a, int32_t x, 
     }
 }
 
-__global__ __launch_bounds__(256) void k_adj_scatter(const DepositArgs a) { adj_scatter_ant(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_adj_scatter(const DepositArgs a) {
+    if (a.state->skip) return;
+    adj_scatter_ant(a, blockIdx.x);
+}
 
 __global__ __launch_bounds__(256) void k_adj_gather(const DepositArgs a) {
     extern __shared__ float delta_row[];
+    if (a.state->skip) return;
     adj_gather_row(a, blockIdx.x, delta_row);
 }
 

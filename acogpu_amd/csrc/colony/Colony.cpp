@@ -92,6 +92,13 @@ int Colony::getNumAnts() { return numAnts; }
 int Colony::getNumCities() { return numCities; }
 int Colony::getReps() { return reps; }
 
+int Colony::run(int maxIterations) {
+    if (!ctx) initialize();
+    int32_t ran = 0, halted = 0;
+    check(aco_run_until(ctx, maxIterations, optimum > 0 ? (int64_t)optimum : -1, 64, &ran, &halted));
+    return ran;
+}
+
 long Colony::getIterBestDist() {
     if (!ctx) return kMaxDistance - 1;
     int64_t ib = 0, gb = 0;

@@ -57,6 +57,10 @@ class Colony {
     virtual const char* antSystemName() = 0;
     std::string getTour();  // 1-based, each city followed by ','
     void setOptimum(const int optimum) { this->optimum = optimum; }
+    // Extension: up to maxIterations iterations with no host round trip per
+    // iteration, stopping after the first iteration whose best reaches the
+    // optimum (setOptimum); returns the number of iterations run.
+    int run(int maxIterations);
 
     // aging (inc/Colony.h:128-139)
     int getMinRetirementAge();

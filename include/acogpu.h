@@ -122,6 +122,16 @@ int aco_phase(aco_ctx *c, int32_t phase, float *ms);
  * HIP-event time of the whole loop. */
 int aco_run(aco_ctx *c, int32_t iterations, float *elapsed_ms);
 
+/* TSP::execute's loop (src/TSP.cpp:23-79) with the optimum test of
+ * TSP::terminationCondition (src/TSP.cpp:168-171) evaluated on the device:
+ * up to max_iterations graph-replayed iterations, halting after the first
+ * iteration whose iteration-best length is <= stop_at (stop_at < 0: never);
+ * later iterations are skipped on the device and the host reads the state
+ * once per `chunk` iterations.  iterations_run / halted (may be NULL) report
+ * what happened; the colony continues normally afterwards. */
+int aco_run_until(aco_ctx *c, int32_t max_iterations, int64_t stop_at, int32_t chunk, int32_t *iterations_run,
+                  int32_t *halted);
+
 /* getIterBestDist / getGlobBestDist / getNumValidAnts (src/Colony.cu:
  * 1096-1104, 447-467). */
 int aco_get_stats(aco_ctx *c, int64_t *iter_best, int64_t *glob_best, uint32_t *valid_ants);

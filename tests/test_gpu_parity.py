@@ -152,7 +152,7 @@ def test_heuristic_override_replay(path):
                             construct_path=path)
     with make(p, xy=xy) as col:
         default = col.heuristic_nn()  # eta^beta; pinned by the replay after the restore below
-        h =np.exp(rng.uniform(np.log(1e-6), np.log(1.0), size=(300, 16))).astype(np.float32)
+        h = np.exp(rng.uniform(np.log(1e-6), np.log(1.0), size=(300, 16))).astype(np.float32)
         h[:, 0] = 0.0  # zero weights: never chosen by roulette
         col.set_heuristic_nn(h)
         np.testing.assert_array_equal(col.heuristic_nn(), h)
@@ -293,3 +293,36 @@ def test_c2_full_size_properties():
         col.iterate()
         ref = replay(p, dist, snap)
         check_iteration(col, p, dist, snap, ref)
+
+
+@pytest.mark.parametrize("mode", [capi.MODE_CANDIDATE, capi.MODE_FULLROW])
+def test_run_until_halts_on_device_like_the_reference_loop(mode):
+    """aco_run_until: the optimum test on the iteration best (src/TSP.cpp:
+    168-171) stops the run after the first iteration that reaches it; the
+    state equals a colony run for exactly that many iterations, and the colony
+    continues normally afterwards.  Rank-based w = 1 makes runs deterministic."""
+    say(f"case: run_until RBAS w=1 n=120 m=30 mode={mode}")
+    xy = instances.uniform(120, 1000, seed=15)
+    p = capi.default_params(n=120, m=30, k=12, w=1, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS, mode=mode,
+                            seed=6, tau0=0.01)
+    with make(p, xy=xy) as a:
+        ib = []
+        for _ in range(30):
+            a.iterate()
+            ib.append(a.stats()[0])
+    target = ib[11]
+    first = next(t for t, v in enumerate(ib) if v <= target)
+    with make(p, xy=xy) as b, make(p, xy=xy) as c:
+        ran, halted = b.run_until(30, target, chunk=8)
+        assert (ran, halted) == (first + 1, True)
+        c.run(first + 1)
+        assert b.stats() == c.stats()
+        np.testing.assert_array_equal(b.pheromone(), c.pheromone())
+        np.testing.assert_array_equal(b.best_tour(), c.best_tour())
+        b.run(3)
+        c.run(3)
+        assert b.stats() == c.stats()
+        np.testing.assert_array_equal(b.pheromone(), c.pheromone())
+    with make(p, xy=xy) as d:
+        assert d.run_until(10, 0, chunk=4) == (10, False)
+        assert d.stats()[0] == ib[9]
