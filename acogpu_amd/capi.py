@@ -85,6 +85,8 @@ SIGNATURES = {
     "aco_batch_get_config": (ctypes.c_int, [_vp, _vp]),
     "aco_batch_autotune": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_batch_tune_k": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _i32, _i32, ctypes.c_float, _vp, _vp]),
+    "aco_batch_exchange_best": (ctypes.c_int, [_vp]),
+    "aco_batch_run_islands": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
     "aco_batch_destroy": (ctypes.c_int, [_vp]),
     "aco_batch_last_error": (ctypes.c_char_p, [_vp]),
     "aco_last_error": (ctypes.c_char_p, [_vp]),
@@ -385,6 +387,14 @@ class Batch:
         w = ctypes.c_int32()
         self._chk(self._L.aco_batch_autotune(self._ctx, int(reps), ctypes.byref(w)))
         return w.value
+
+    def exchange_best(self):
+        self._chk(self._L.aco_batch_exchange_best(self._ctx))
+
+    def run_islands(self, iterations, exchange_every):
+        ms = ctypes.c_float()
+        self._chk(self._L.aco_batch_run_islands(self._ctx, int(iterations), int(exchange_every), ctypes.byref(ms)))
+        return ms.value
 
     def best_lengths(self):
         gb = np.zeros(self.count, dtype=np.int64)

@@ -562,3 +562,35 @@ extern "C" int aco_batch_tune_k(const aco_params* p, int32_t count, const int32_
         }
     return ACO_OK;
 }
+
+// Island model (the reference's broker build is an empty extension point,
+// inc/Colony.h:142-149): the batch holds copies of one instance, island i on
+// Philox stream i; every `exchange_every` iterations the shortest best-so-far
+// tour over all islands becomes every island's best-so-far (the rank-based
+// deposit then reinforces it everywhere).
+extern "C" int aco_batch_exchange_best(aco_batch* b) {
+    if (!b) return ACO_E_ARG;
+    if (!b->initialized) return bfail(b, ACO_E_STATE, "aco_batch_exchange_best before aco_batch_initialize");
+    for (int32_t i = 1; i < b->count; ++i)
+        if (b->n[i] != b->n[0]) return bfail(b, ACO_E_ARG, "islands must be copies of one instance (n differs)");
+    BHIP(b, aco::launch_batch_exchange(b->buf, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return ACO_OK;
+}
+
+extern "C" int aco_batch_run_islands(aco_batch* b, int32_t iterations, int32_t exchange_every, float* elapsed_ms) {
+    if (!b) return ACO_E_ARG;
+    if (iterations < 0 || exchange_every < 1) return bfail(b, ACO_E_ARG, "bad iteration count or exchange interval");
+    float total = 0.0f;
+    for (int32_t left = iterations; left > 0;) {
+        const int32_t step = left < exchange_every ? left : exchange_every;
+        float ms = 0.0f;
+        int rc = aco_batch_run(b, step, &ms);
+        if (rc) return rc;
+        total += ms;
+        left -= step;
+        if ((rc = aco_batch_exchange_best(b))) return rc;
+    }
+    if (elapsed_ms) *elapsed_ms = total;
+    return ACO_OK;
+}

@@ -157,6 +157,7 @@ hipError_t launch_batch_init(const BatchBuffers& b, int32_t pmax, int32_t nmax, 
 hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, int32_t nmax, int32_t grid,
                                          hipStream_t s);
 size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block);
+hipError_t launch_batch_exchange(const BatchBuffers& b, hipStream_t s);
 int32_t batch_construct_blocks_per_cu(int32_t ants_per_block, size_t lds_bytes);
 
 hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s);

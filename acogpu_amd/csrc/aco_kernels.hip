@@ -1373,6 +1373,27 @@ __global__ __launch_bounds__(256) void kb_stats(const BatchBuffers b) {
     if (inst == 0 && threadIdx.x == 0) *b.queue = 0u;
 }
 
+// Island exchange (batched copies of one instance): every island whose
+// best-so-far is longer than the shortest one over all islands (ties: the
+// lowest island) takes that tour and length.  One block per island; the
+// winner is never written, so the copies race with nothing.
+__global__ __launch_bounds__(256) void kb_exchange(const BatchBuffers b) {
+    __shared__ LenKey red[4];
+    const int32_t me = blockIdx.x;
+    LenKey v{INT64_MAX, 0x7fffffff};
+    for (int32_t i = threadIdx.x; i < b.count; i += blockDim.x) {
+        const LenKey k{b.state[i].glob_best, i};
+        if (key_less(k, v)) v = k;
+    }
+    const LenKey win = block_min_key(v, red);
+    if (win.ant == me || b.state[me].glob_best <= win.len) return;
+    const InstMeta wi = b.meta[win.ant];
+    const InstMeta mi = b.meta[me];
+    for (int32_t t = threadIdx.x; t < mi.n; t += blockDim.x)
+        b.best_tours[mi.city_off + t] = b.best_tours[wi.city_off + t];
+    if (threadIdx.x == 0) b.state[me].glob_best = win.len;
+}
+
 // AS: one block per ant of every instance; RBAS: w + 1 blocks per instance.
 __global__ __launch_bounds__(256) void kb_adj_scatter(const BatchBuffers b) {
     const int32_t inst = b.ant_inst[blockIdx.x];
@@ -1535,6 +1556,11 @@ hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, i
                                    0, s, b);
             }
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_exchange(const BatchBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(kb_exchange, dim3(b.count), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

@@ -219,6 +219,16 @@ int aco_batch_autotune(aco_batch *b, int32_t reps, int32_t *chosen);
 int aco_batch_tune_k(const aco_params *p, int32_t count, const int32_t *n, const int32_t *m, const int32_t *xy,
                      const int32_t *ks, int32_t nks, int32_t trial_iterations, float tolerance, int32_t *chosen_k,
                      float *report);
+/* Island model on a batch whose instances are copies of one TSP (island i
+ * draws from Philox stream i).  The reference's broker build variant is an
+ * empty extension point (inc/Colony.h:142-149); this is its multi-colony
+ * reading: aco_batch_exchange_best gives every island the shortest
+ * best-so-far tour over all islands (ties: lowest island); run_islands runs
+ * `iterations` iterations exchanging after every `exchange_every`.  The
+ * exchange matters for the rank-based variant, whose best-so-far tour
+ * deposits. */
+int aco_batch_exchange_best(aco_batch *b);
+int aco_batch_run_islands(aco_batch *b, int32_t iterations, int32_t exchange_every, float *elapsed_ms);
 int aco_batch_destroy(aco_batch *b);
 const char *aco_batch_last_error(const aco_batch *b);
 

@@ -152,3 +152,58 @@ def test_tune_k_picks_fastest_k_within_tolerance():
         b.run(8)
         gb, _ = b.best()
     assert abs(gb.mean() - rep[ks.index(k), 1]) <= 0.02 * gb.mean()
+
+
+def test_islands_exchange_equals_single_colonies_given_the_shared_best():
+    """Island model: 6 copies of one instance (island i = stream i).  After an
+    exchange every island holds the shortest best-so-far tour; the following
+    iterations equal single colonies whose best-so-far was set to it
+    (rank-based w = 1: deterministic)."""
+    say("case: 6 islands of n=150, RBAS w=1, exchange after 5 iterations, 4 more")
+    xy = instances.uniform(150, 1000, seed=77)
+    p = batch_params(variant=capi.VARIANT_RBAS, w=1, k=12, tau0_policy=capi.TAU0_CONST, tau0=0.01, rho=0.2)
+    with capi.Batch(p, [150] * 6, [xy] * 6, m_list=[40] * 6) as b:
+        b.initialize()
+        b.run(5)
+        gb0, best0 = b.best()
+        win = int(np.argmin(gb0))
+        b.exchange_best()
+        gb1, best1 = b.best()
+        assert (gb1 == gb0[win]).all()
+        for t in best1:
+            np.testing.assert_array_equal(t, best0[win])
+        b.run(4)
+        gb2, best2 = b.best()
+        for i in (0, 2, 5):
+            q = capi.default_params(n=150, m=40, k=12, variant=capi.VARIANT_RBAS, w=1, beta=2.0, rho=0.2,
+                                    tau0_policy=capi.TAU0_CONST, tau0=0.01, seed=77, stream_id=i)
+            with capi.Colony(q, xy=xy) as c:
+                c.initialize()
+                c.run(5)
+                assert c.stats()[1] == gb0[i]
+                c.set_best(int(gb0[win]), best0[win])
+                c.run(4)
+                np.testing.assert_array_equal(b.tours(i), c.tours())
+                assert c.stats()[1] == gb2[i]
+                np.testing.assert_array_equal(c.best_tour(), best2[i])
+
+
+def test_run_islands_ends_with_a_shared_best():
+    say("case: run_islands 8 islands n=120, 12 iterations, exchange every 4")
+    xy = instances.uniform(120, 1000, seed=78)
+    p = batch_params(variant=capi.VARIANT_RBAS, w=6, k=12, tau0_policy=capi.TAU0_RBAS, rho=0.1)
+    with capi.Batch(p, [120] * 8, [xy] * 8) as b:
+        b.initialize()
+        b.run_islands(12, 4)
+        gb, best = b.best()
+        assert (gb == gb.min()).all()
+        dist = orc.dist_from_int_coords(xy)
+        for t in best:
+            assert sorted(t.tolist()) == list(range(120))
+            assert instances.tour_length(dist, t) == gb[0]
+    sizes, xys = make_batch_inputs(3, 50, 80, seed=9)
+    with capi.Batch(batch_params(), sizes, xys) as b:
+        b.initialize()
+        if len(set(sizes.tolist())) > 1:
+            with pytest.raises(capi.AcoError):
+                b.exchange_best()
