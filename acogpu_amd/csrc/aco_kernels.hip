@@ -588,12 +588,15 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
     int32_t tbuf = start;
     int32_t c = start;
     unsigned long long fb = 0, scanned = 0;
+    // the candidate row of the current city is loaded one step ahead: the
+    // LDS reads for step s+1 are issued as soon as step s has chosen, before
+    // its bookkeeping
+    int32_t e = c * a.k * lmul + loff;
+    int32_t j = nl[e];
+    float cw = wl[e];
     for (int32_t s = 1; s < a.n; ++s) {
         rng_step(rng, s, (uint32_t)ant, iter, a, lane);
         const float u = word_to_uniform(rng_word(rng, s));
-        const int32_t e = c * a.k * lmul + loff;
-        const int32_t j = nl[e];
-        const float cw = wl[e];
         bool vj;
         if (VREG_VIS) {
             const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((j >> 5) << 2, (int)vreg);
@@ -602,23 +605,26 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             vj = is_visited(vis_lds, j);
         }
         const float w = vj ? 0.0f : cw;
+        // ballots straight from the compare masks (no round trip through a
+        // VGPR); the common case falls through, the rare ones are laid apart
+        const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
         const float p = K32 ? wave_inclusive_scan32(w) : wave_inclusive_scan(w);
         const float total = readlane_f(p, K32 ? 31 : 63);
         const float rr = u * total;
-        // the compare mask is the ballot (no round trip through a VGPR);
-        // the common case falls through, the rare ones are laid out apart
-        const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr && w > 0.0f);
+        const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
         int32_t next;
         if (__builtin_expect(hit != 0ull, 1)) {
             next = readlane_i(j, __ffsll((long long)hit) - 1);
         } else if (total > 0.0f) {
-            const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
             next = readlane_i(j, 63 - __clzll((long long)pos));
         } else {
             ++fb;
             next = VREG_VIS ? fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned)
                             : fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned);
         }
+        e = next * a.k * lmul + loff;
+        const int32_t jn = nl[e];
+        const float cwn = wl[e];
         if (VREG_VIS) {
             vreg |= (lane == (next >> 5)) ? (1u << (next & 31)) : 0u;
         } else {
@@ -629,6 +635,8 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             if (lane <= (s & 63)) t16[(s & ~63) + lane] = (uint16_t)tbuf;
         }
         c = next;
+        j = jn;
+        cw = cwn;
     }
     finish_counters(a, fb, scanned, lane);
     // tour out (coalesced) and exact length, edge i -> i+1 per lane
