@@ -940,7 +940,8 @@ __device__ __forceinline__ bool key_less(const LenKey& x, const LenKey& y) {
     return x.len < y.len || (x.len == y.len && x.ant < y.ant);
 }
 
-__device__ LenKey block_min_key(LenKey v, LenKey* red) {
+// (len, ant) minimum over the wave; every lane gets it.
+__device__ __forceinline__ LenKey wave_min_key(LenKey v) {
     for (int off = 32; off >= 1; off >>= 1) {
         LenKey o;
         const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)v.len, off);
@@ -949,6 +950,11 @@ __device__ LenKey block_min_key(LenKey v, LenKey* red) {
         o.ant = __shfl_xor(v.ant, off);
         if (key_less(o, v)) v = o;
     }
+    return v;
+}
+
+__device__ __forceinline__ LenKey block_min_key(LenKey v, LenKey* red) {
+    v = wave_min_key(v);
     const int nw = blockDim.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -959,7 +965,7 @@ __device__ LenKey block_min_key(LenKey v, LenKey* red) {
     return b;
 }
 
-__device__ int32_t block_min_i32(int32_t v, int32_t* red) {
+__device__ __forceinline__ int32_t block_min_i32(int32_t v, int32_t* red) {
     for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off));
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -969,9 +975,13 @@ __device__ int32_t block_min_i32(int32_t v, int32_t* red) {
     return b;
 }
 
-// `taken` holds (m+31)/32 words (RBAS ranking); red/redi one slot per wave.
-__device__ void stats_block(const StatsArgs& a, LenKey* red, int32_t* redi, uint32_t* taken, int32_t& s_improved,
-                            int32_t& s_recruit) {
+// Largest w-1 of the two-round rank selection in stats_block.
+constexpr int32_t RANK_MAX = 16;
+
+// `taken` holds (m+31)/32 words (RBAS ranking); red/redi one slot per wave;
+// red_rank (16 x RANK_MAX) the per-wave rank candidates.
+__device__ __forceinline__ void stats_block_general(const StatsArgs& a, LenKey* red, int32_t* redi, uint32_t* taken,
+                                    int32_t& s_improved, int32_t& s_recruit, LenKey* red_rank) {
     DevState* st = a.state;
     const LenKey none{(int64_t)ACO_MAX_DISTANCE - 1, 0x7fffffff};
 
@@ -1013,7 +1023,7 @@ __device__ void stats_block(const StatsArgs& a, LenKey* red, int32_t* redi, uint
             a.dep_val[k] = v ? (float)(1.0 / (double)a.lengths[k]) : 0.0f;
         }
         if (threadIdx.x == 0) st->ndep = a.m;
-    } else {  // RBAS
+    } else {  // RBAS, general w and m: w-1 block-wide passes
         const int32_t words = (a.m + 31) / 32;
         for (int32_t w = threadIdx.x; w < words; w += blockDim.x) taken[w] = 0u;
         __syncthreads();
@@ -1082,13 +1092,229 @@ __device__ void stats_block(const StatsArgs& a, LenKey* red, int32_t* redi, uint
     }
 }
 
+// Wave minimum of 64-bit keys through DPP (row shifts, then the two row
+// broadcasts: the scan pattern of wave_inclusive_scan with min); invalid
+// sources keep the identity.  The result is read from lane 63.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_u64_or_max(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)v, CTRL, ROW_MASK, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(v >> 32), CTRL, ROW_MASK, 0xf, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+    v = min(v, dpp_u64_or_max<0x111, 0xf>(v));
+    v = min(v, dpp_u64_or_max<0x112, 0xf>(v));
+    v = min(v, dpp_u64_or_max<0x114, 0xf>(v));
+    v = min(v, dpp_u64_or_max<0x118, 0xf>(v));
+    v = min(v, dpp_u64_or_max<0x142, 0xa>(v));
+    v = min(v, dpp_u64_or_max<0x143, 0xc>(v));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Block minimum of 64-bit keys: wave minima through LDS (one slot per wave).
+__device__ __forceinline__ uint64_t block_min_u64(uint64_t v, uint64_t* red) {
+    v = wave_min_u64(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t b = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = min(b, red[w]);
+    return b;
+}
+
+// (len, ant) keys packed as len << 20 | ant: the packed order is the
+// (len, ant) order for ant < 2^20 and len < 2^44 (n <= 2^18 cities of
+// distance <= 2^26).  NO_KEY marks "no ant".
+constexpr uint64_t NO_KEY = ~0ull;
+__device__ __forceinline__ uint64_t pack_key(int64_t len, int32_t ant) { return ((uint64_t)len << 20) | (uint32_t)ant; }
+__device__ __forceinline__ int64_t key_len(uint64_t k) { return (int64_t)(k >> 20); }
+__device__ __forceinline__ int32_t key_ant(uint64_t k) { return (int32_t)(k & 0xfffff); }
+
+// The same statistics with every ant's length, validity and age loaded into
+// registers once (at most four ants per thread) and the iteration state kept
+// in LDS: the dependent chain is one round of global loads instead of one
+// per phase.  Used when m <= 4 x block size and (rank-based) w - 1 <= RANK_MAX.
+struct StatsShared {
+    int64_t glob_best;
+    int32_t gperf, num_valid, improved, recruit;
+};
+
+__device__ __forceinline__ void stats_block_fast(const StatsArgs& a, uint64_t* red, uint64_t* red_rank,
+                                                 StatsShared& sh) {
+    DevState* st = a.state;
+    const int32_t B = blockDim.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) {
+        sh.glob_best = st->glob_best;
+        sh.gperf = st->gperf;
+        sh.num_valid = st->num_valid;
+    }
+    int64_t L[4], PL[4];
+    int32_t AG[4], PF[4];
+    bool IN[4], V[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int32_t k = threadIdx.x + q * B;
+        IN[q] = k < a.m;
+        L[q] = IN[q] ? a.lengths[k] : 0;
+        V[q] = IN[q] && (!a.aging || a.valid[k]);
+        AG[q] = (IN[q] && a.aging) ? a.age[k] : 0;
+        PL[q] = (IN[q] && a.aging) ? a.prev_len[k] : 0;
+        PF[q] = (IN[q] && a.aging) ? a.perf[k] : 0;
+    }
+    uint64_t mine = NO_KEY;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (!V[q]) continue;
+        const int32_t k = threadIdx.x + q * B;
+        if (a.aging) {
+            a.perf[k] = (L[q] == PL[q]) ? PF[q] + 1 : 0;
+            a.prev_len[k] = L[q];
+        }
+        mine = min(mine, pack_key(L[q], k));
+    }
+    const uint64_t ib = block_min_u64(mine, red);
+    // iterBestDist starts at MAX_DISTANCE - 1 and only a strictly smaller
+    // length is taken (src/Colony.cu:872,1015).
+    const bool found = ib != NO_KEY && key_len(ib) < (int64_t)ACO_MAX_DISTANCE - 1;
+    const int64_t ibest = found ? key_len(ib) : (int64_t)ACO_MAX_DISTANCE - 1;
+    if (threadIdx.x == 0) {
+        st->iter_best = ibest;
+        st->iter_best_ant = found ? key_ant(ib) : -1;
+        const bool improved = found && ibest < sh.glob_best;
+        if (improved) {
+            sh.glob_best = ibest;
+            sh.gperf = 0;
+        } else {
+            sh.gperf += 1;
+        }
+        st->glob_best = sh.glob_best;
+        st->gperf = sh.gperf;
+        sh.improved = improved ? key_ant(ib) : -1;
+    }
+    __syncthreads();
+    const int32_t imp = sh.improved;
+    if (imp >= 0)
+        for (int32_t i = threadIdx.x; i < a.n; i += B) a.best_tour[i] = a.tours[(size_t)imp * a.n + i];
+
+    if (a.variant == 0) {  // AS: dense, one entry per ant
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (IN[q]) a.dep_val[threadIdx.x + q * B] = V[q] ? (float)(1.0 / (double)L[q]) : 0.0f;
+        if (threadIdx.x == 0) st->ndep = a.m;
+    } else {
+        // rank-based: the w-1 smallest keys in two rounds (every wave its
+        // own, then wave 0 over the waves' candidates)
+        const int32_t t = a.w - 1;
+        uint64_t ks[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ks[q] = V[q] ? pack_key(L[q], (int32_t)(threadIdx.x + q * B)) : NO_KEY;
+        for (int32_t r = 0; r < t; ++r) {
+            const uint64_t wmin = wave_min_u64(min(min(ks[0], ks[1]), min(ks[2], ks[3])));
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (ks[q] == wmin) ks[q] = NO_KEY;
+            if (lane == 0) red_rank[wave * RANK_MAX + r] = wmin;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const int32_t nq = (int32_t)(B >> 6) * RANK_MAX / 64;  // 1..4 candidates per lane
+            uint64_t cs[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int32_t x = lane + 64 * q;
+                cs[q] = (q < nq && (x % RANK_MAX) < t) ? red_rank[x] : NO_KEY;
+            }
+            int32_t nd = 0;
+            for (int32_t r = 0; r < t; ++r) {
+                const uint64_t wmin = wave_min_u64(min(min(cs[0], cs[1]), min(cs[2], cs[3])));
+                if (wmin == NO_KEY) break;  // fewer valid ants than w-1
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (cs[q] == wmin) cs[q] = NO_KEY;
+                if (lane == 0) {
+                    a.dep_ant[nd] = key_ant(wmin);
+                    a.dep_val[nd] = (float)((double)(a.w - (r + 1)) / (double)key_len(wmin));
+                }
+                ++nd;
+            }
+            if (lane == 0) {
+                if (sh.glob_best < (int64_t)ACO_MAX_DISTANCE) {
+                    a.dep_ant[nd] = -1;
+                    a.dep_val[nd] = (float)((double)a.w / (double)sh.glob_best);
+                    ++nd;
+                }
+                st->ndep = nd;
+            }
+        }
+    }
+
+    if (a.aging) {
+        // retireAnts: first valid ant among the first m/2 at max_age.
+        uint64_t cand = NO_KEY;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t k = threadIdx.x + q * B;
+            if (k < a.m / 2 && V[q] && AG[q] >= a.max_age) cand = min(cand, (uint64_t)k);
+        }
+        const uint64_t ret = block_min_u64(cand, red);
+        if (ret != NO_KEY) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if ((uint64_t)(threadIdx.x + q * B) == ret) V[q] = false;
+            if (threadIdx.x == 0) {
+                a.valid[ret] = 0;
+                sh.num_valid -= 1;
+                st->converged = 1;
+            }
+        }
+        // recruiteAnts: first invalid ant, when the colony stagnates or is small.
+        if (threadIdx.x == 0) sh.recruit = (sh.gperf >= a.global_perf || sh.num_valid < a.min_pop);
+        __syncthreads();
+        if (sh.recruit) {
+            uint64_t inv = NO_KEY;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (IN[q] && !V[q]) inv = min(inv, (uint64_t)(threadIdx.x + q * B));
+            const uint64_t rc = block_min_u64(inv, red);
+            if (threadIdx.x == 0 && rc != NO_KEY) {
+                a.valid[rc] = 1;
+                a.age[rc] = 0;
+                sh.num_valid += 1;
+            }
+        }
+        if (threadIdx.x == 0) st->num_valid = sh.num_valid;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->iter += 1;
+        // TSP::terminationCondition's optimum test (src/TSP.cpp:168-171) on the
+        // iteration best, evaluated on the device
+        if (st->stop_at >= 0 && ibest <= st->stop_at) st->halted = 1;
+    }
+}
+
+__device__ __forceinline__ void stats_block(const StatsArgs& a, LenKey* red, int32_t* redi, uint32_t* taken, int32_t& s_improved,
+                            int32_t& s_recruit, LenKey* red_rank, StatsShared& sh) {
+    if (a.m <= 4 * (int32_t)blockDim.x && a.m < (1 << 20) && a.n <= (1 << 18) &&
+        (a.variant == 0 || a.w - 1 <= RANK_MAX))
+        stats_block_fast(a, reinterpret_cast<uint64_t*>(red), reinterpret_cast<uint64_t*>(red_rank), sh);
+    else
+        stats_block_general(a, red, redi, taken, s_improved, s_recruit, red_rank);
+}
+
 __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
     __shared__ LenKey red[16];
     __shared__ int32_t redi[16];
     __shared__ int32_t s_improved, s_recruit;
     extern __shared__ uint32_t taken[];  // RBAS: ants already ranked
+    __shared__ LenKey red_rank[16 * RANK_MAX];
+    __shared__ StatsShared sh;
     if (a.state->skip) return;
-    stats_block(a, red, redi, taken, s_improved, s_recruit);
+    stats_block(a, red, redi, taken, s_improved, s_recruit, red_rank, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -1359,6 +1585,8 @@ __global__ __launch_bounds__(256) void kb_stats(const BatchBuffers b) {
     __shared__ int32_t redi[16];
     __shared__ int32_t s_improved, s_recruit;
     extern __shared__ uint32_t taken[];
+    __shared__ LenKey red_rank[16 * RANK_MAX];
+    __shared__ StatsShared sh;
     const int32_t inst = blockIdx.x;
     const InstMeta im = b.meta[inst];
     StatsArgs a{};
@@ -1377,7 +1605,7 @@ __global__ __launch_bounds__(256) void kb_stats(const BatchBuffers b) {
     a.variant = b.variant;
     a.w = b.w < im.n ? b.w : im.n;
     a.aging = 0;
-    stats_block(a, red, redi, taken, s_improved, s_recruit);
+    stats_block(a, red, redi, taken, s_improved, s_recruit, red_rank, sh);
     if (inst == 0 && threadIdx.x == 0) *b.queue = 0u;
 }
 

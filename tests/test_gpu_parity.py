@@ -174,6 +174,23 @@ def test_heuristic_override_replay(path):
         check_iteration(col, p, dist, snap, replay(p, dist, snap))
 
 
+@pytest.mark.parametrize("m,w", [(5, 8), (64, 6), (64, 17), (64, 30), (4500, 6)])
+def test_rbas_rank_selection_replay(m, w):
+    """Rank-based deposit lists: two-round selection (w - 1 <= 16) and the
+    general w - 1 passes, including fewer ants than w - 1."""
+    say(f"case: RBAS ranking n=200 m={m} w={w}")
+    rng = np.random.default_rng(50 + w)
+    xy = instances.uniform(200, 2000, seed=50 + w)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=200, m=m, k=16, w=w, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS, seed=w)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, 200))
+        for _ in range(2):
+            snap = snapshot(col)
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+
+
 @pytest.mark.parametrize("aging", [0, 1])
 def test_as_deposit_paths_agree_and_replay(aging):
     """Ant System deposit through fp32 atomics (path 1) and through the
