@@ -92,6 +92,61 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// 64-bit wave reductions through DPP: the scan pattern of
+// wave_inclusive_scan (row shifts 1, 2, 4, 8, then the two row broadcasts)
+// with min / max; sources outside a row keep the identity.  Lane 63 ends
+// with the result, which every lane receives through v_readlane.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v, uint32_t identity) {
+    const uint32_t lo =
+        (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)(uint32_t)v, CTRL, ROW_MASK, 0xf, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)(uint32_t)(v >> 32), CTRL, ROW_MASK, 0xf, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t lane63_u64(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+    v = min(v, dpp_u64<0x111, 0xf>(v, ~0u));
+    v = min(v, dpp_u64<0x112, 0xf>(v, ~0u));
+    v = min(v, dpp_u64<0x114, 0xf>(v, ~0u));
+    v = min(v, dpp_u64<0x118, 0xf>(v, ~0u));
+    v = min(v, dpp_u64<0x142, 0xa>(v, ~0u));
+    v = min(v, dpp_u64<0x143, 0xc>(v, ~0u));
+    return lane63_u64(v);
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+    v = max(v, dpp_u64<0x111, 0xf>(v, 0u));
+    v = max(v, dpp_u64<0x112, 0xf>(v, 0u));
+    v = max(v, dpp_u64<0x114, 0xf>(v, 0u));
+    v = max(v, dpp_u64<0x118, 0xf>(v, 0u));
+    v = max(v, dpp_u64<0x142, 0xa>(v, 0u));
+    v = max(v, dpp_u64<0x143, 0xc>(v, 0u));
+    return lane63_u64(v);
+}
+
+// Wave argmax of per-lane (best, index) pairs: the largest value, ties to the
+// lowest index; best = -1 means "no candidate" (weights above -1 only; NaN
+// never becomes a lane's best).  Packed as (order-preserving float bits,
+// inverted index) so one 64-bit max decides both.  0x7fffffff if no lane has
+// a candidate.
+__device__ __forceinline__ int32_t wave_argmax_index(float best, int32_t idx) {
+    uint64_t key = 0;
+    if (best > -1.0f) {
+        const uint32_t u = __float_as_uint(best + 0.0f);  // -0 -> +0: equal floats, equal keys
+        const uint32_t ord = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        key = ((uint64_t)ord << 32) | (uint32_t)(0xffffffffu - (uint32_t)idx);
+    }
+    key = wave_max_u64(key);
+    return key ? (int32_t)(0xffffffffu - (uint32_t)key) : 0x7fffffff;
+}
+
 // Heuristic lookup: table by integer distance, 0 outside (diagonal).
 __device__ __forceinline__ float eta_of(const float* __restrict__ eta, int32_t eta_len, int32_t d) {
     return ((uint32_t)d < (uint32_t)eta_len) ? eta[d] : 0.0f;

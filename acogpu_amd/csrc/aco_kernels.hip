@@ -352,15 +352,7 @@ __device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const
             }
         }
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        const float ob = __shfl_xor(best, off);
-        const int32_t oj = __shfl_xor(bj, off);
-        if (ob > best || (ob == best && oj < bj)) {
-            best = ob;
-            bj = oj;
-        }
-    }
-    int32_t next = uniform_i(bj);
+    int32_t next = wave_argmax_index(best, bj);
     if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
         for (int32_t j0 = 0; j0 < a.n; j0 += 64) {
             const int32_t j = j0 + lane;
@@ -543,15 +535,7 @@ __device__ __forceinline__ int32_t fallback_argmax_vreg(const ConstructArgs& a, 
             }
         }
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        const float ob = __shfl_xor(best, off);
-        const int32_t oj = __shfl_xor(bj, off);
-        if (ob > best || (ob == best && oj < bj)) {
-            best = ob;
-            bj = oj;
-        }
-    }
-    int32_t next = uniform_i(bj);
+    int32_t next = wave_argmax_index(best, bj);
     if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
         uint32_t free_bits = (32 * lane < a.n) ? ~vword : 0u;
         if (32 * lane < a.n && a.n - 32 * lane < 32) free_bits &= (1u << (a.n - 32 * lane)) - 1u;
@@ -1090,28 +1074,6 @@ __device__ __forceinline__ void stats_block_general(const StatsArgs& a, LenKey* 
         // iteration best, evaluated on the device
         if (st->stop_at >= 0 && st->iter_best <= st->stop_at) st->halted = 1;
     }
-}
-
-// Wave minimum of 64-bit keys through DPP (row shifts, then the two row
-// broadcasts: the scan pattern of wave_inclusive_scan with min); invalid
-// sources keep the identity.  The result is read from lane 63.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ uint64_t dpp_u64_or_max(uint64_t v) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)v, CTRL, ROW_MASK, 0xf, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(v >> 32), CTRL, ROW_MASK, 0xf, false);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-    v = min(v, dpp_u64_or_max<0x111, 0xf>(v));
-    v = min(v, dpp_u64_or_max<0x112, 0xf>(v));
-    v = min(v, dpp_u64_or_max<0x114, 0xf>(v));
-    v = min(v, dpp_u64_or_max<0x118, 0xf>(v));
-    v = min(v, dpp_u64_or_max<0x142, 0xa>(v));
-    v = min(v, dpp_u64_or_max<0x143, 0xc>(v));
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
-    return ((uint64_t)hi << 32) | lo;
 }
 
 // Block minimum of 64-bit keys: wave minima through LDS (one slot per wave).
