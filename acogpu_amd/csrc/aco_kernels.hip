@@ -583,7 +583,9 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
         const float u = word_to_uniform(rng_word(rng, s));
         bool vj;
         if (VREG_VIS) {
-            const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((j >> 5) << 2, (int)vreg);
+            // ds_bpermute takes a byte address and drops its two low bits:
+            // j >> 3 addresses lane j >> 5, the owner of j's bitmap word
+            const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute(j >> 3, (int)vreg);
             vj = (word >> (j & 31)) & 1u;
         } else {
             vj = is_visited(vis_lds, j);
