@@ -159,27 +159,56 @@ __global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dis
 // and per row the mean of tau (dev_compute_pheromone_mean, inc/Colony.cuh:164-190),
 // lane l summing columns 256s + 4l + q in order, lanes combined by the DPP scan.
 // Algorithmic traffic: 16 bytes per element (read tau, dist; write choice, tau_next).
+// One row, four 256-element chunks at a time: all tau/dist loads of the
+// group are issued first, then the heuristic gathers, then the stores (the
+// output pointers may alias nothing the group still reads).  The row-mean
+// order is unchanged: lane l adds its columns 256s + 4l + q in s, q order.
 __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lane) {
     const size_t row = (size_t)i * a.P;
     const float* __restrict__ trow = a.tau + row;
     const int32_t* __restrict__ drow = a.dist + row;
+    float* __restrict__ crow = a.choice + row;
+    float* __restrict__ nrow = a.tau_next + row;
     float acc = 0.0f;
-    for (int32_t j0 = 4 * lane; j0 < a.n; j0 += 256) {
-        const float4 t = *reinterpret_cast<const float4*>(trow + j0);
-        const int4 d = *reinterpret_cast<const int4*>(drow + j0);
-        const float tv[4] = {t.x, t.y, t.z, t.w};
-        const int32_t dv[4] = {d.x, d.y, d.z, d.w};
-        float cv[4], ev[4];
+    for (int32_t g0 = 0; g0 < a.n; g0 += 1024) {
+        float4 t[4];
+        int4 d[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const bool in = (j0 + q) < a.n;
-            const float x = in ? tv[q] : 0.0f;
-            acc = acc + x;
-            cv[q] = in ? apply_alpha(x, a.alpha) * eta_of(a.eta, a.eta_len, dv[q]) : 0.0f;
-            ev[q] = in ? fmaxf(a.one_minus_rho * x, ACO_MIN_PHEROMONE) : 0.0f;
+        for (int s = 0; s < 4; ++s) {
+            const int32_t j0 = g0 + 256 * s + 4 * lane;
+            if (j0 < a.n) {
+                t[s] = *reinterpret_cast<const float4*>(trow + j0);
+                d[s] = *reinterpret_cast<const int4*>(drow + j0);
+            } else {
+                t[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+                d[s] = make_int4(0, 0, 0, 0);
+            }
         }
-        *reinterpret_cast<float4*>(a.choice + row + j0) = make_float4(cv[0], cv[1], cv[2], cv[3]);
-        *reinterpret_cast<float4*>(a.tau_next + row + j0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
+        float e[4][4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            e[s][0] = eta_of(a.eta, a.eta_len, d[s].x);
+            e[s][1] = eta_of(a.eta, a.eta_len, d[s].y);
+            e[s][2] = eta_of(a.eta, a.eta_len, d[s].z);
+            e[s][3] = eta_of(a.eta, a.eta_len, d[s].w);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int32_t j0 = g0 + 256 * s + 4 * lane;
+            if (j0 >= a.n) continue;
+            const float tv[4] = {t[s].x, t[s].y, t[s].z, t[s].w};
+            float cv[4], ev[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool in = (j0 + q) < a.n;
+                const float x = in ? tv[q] : 0.0f;
+                acc = acc + x;
+                cv[q] = in ? apply_alpha(x, a.alpha) * e[s][q] : 0.0f;
+                ev[q] = in ? fmaxf(a.one_minus_rho * x, ACO_MIN_PHEROMONE) : 0.0f;
+            }
+            *reinterpret_cast<float4*>(crow + j0) = make_float4(cv[0], cv[1], cv[2], cv[3]);
+            *reinterpret_cast<float4*>(nrow + j0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
+        }
     }
     if (a.want_mean) {
         const float total = readlane_f(wave_inclusive_scan(acc), 63);
