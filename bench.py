@@ -189,6 +189,39 @@ def bench_large(args):
     }
 
 
+def bench_ant_scaling(args):
+    """Latency check of the construction kernel: the n=1000 rank-based
+    workload (no aging, so every ant is an ordinary staged ant) with m = 64,
+    256, 512 and 1000 ants; construction time per launch (HIP events, mean of
+    `profile_iters` launches after 30 iterations).  If the kernel were
+    throughput-bound the time would grow with m; it stays near one ant's
+    tour time (a chain of n dependent steps)."""
+    n = 1000
+    xy = instances.clustered(n, clusters=10, side=10000, sigma=400.0, seed=1)
+    rows = {}
+    for m in (64, 256, 512, 1000):
+        p = c2_params(n, m)
+        p.aging = 0
+        col = capi.Colony(p, xy=xy)
+        col.initialize()
+        col.run(30)
+        k = [float(x) for x in col.profile(args.profile_iters)]
+        cfg = col.config()
+        col.close()
+        rows[str(m)] = {"construct_ms": k[1], "us_per_step": k[1] * 1e3 / (n - 1),
+                        "ants_per_block": cfg["ants_per_block"]}
+        log(f"bench: m={m}: construct {k[1] * 1e3:.1f} us ({cfg})")
+    base = rows["64"]["construct_ms"]
+    return {
+        "metric": "construction time vs ants, n=1000", "value": rows["1000"]["construct_ms"] / base, "unit": "x",
+        "n_gpus": 1, "steps": args.profile_iters, "warmup": 30, "ms_per_step": rows["1000"]["construct_ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "rank-based AS, n=1000 clustered, k=20, w=6, beta 3, rho 0.1, no aging; m varied",
+                   "value_is": "construct time at m=1000 / at m=64"},
+        "per_m": rows,
+    }
+
+
 def time_to_gap(curve, ref, gap=0.02):
     """Per instance, the solver wall time at the first checkpoint whose
     best-so-far length is within `gap` of ref (inf if never)."""
@@ -315,12 +348,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=3)
-    ap.add_argument("--fullrow-iters", type=int, default=10)
+    ap.add_argument("--fullrow-iters", type=int, default=10, help="0 skips the full-row GPU figure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ants-per-block", type=int, default=0)
-    ap.add_argument("--workload", choices=["c2", "batched", "large", "prior"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "batched", "large", "prior", "ant-scaling"], default="c2",
                     help="c2: the headline n=1000 tours/s line (default); batched: 4096 uneven instances; "
-                         "large: n=20000; prior: learned edge prior time to 2% gap")
+                         "large: n=20000; prior: learned edge prior time to 2% gap; ant-scaling: "
+                         "construction time vs number of ants")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--prior-instances", type=int, default=256)
     ap.add_argument("--tune-k", type=int, default=1, help="batched: also run the candidate-width tuner (0 = off)")
@@ -334,7 +368,8 @@ def main():
         return 2
 
     if args.workload != "c2":
-        line = {"batched": bench_batched, "large": bench_large, "prior": bench_prior}[args.workload](args)
+        line = {"batched": bench_batched, "large": bench_large, "prior": bench_prior,
+                "ant-scaling": bench_ant_scaling}[args.workload](args)
         print(json.dumps(line), flush=True)
         return 0
 
@@ -407,17 +442,19 @@ def main():
 
     # The same colony with reference full-row semantics on the GPU (every
     # city in NN order each step: apples-to-apples work per tour).
-    pf = c2_params(n, m)
-    pf.mode = capi.MODE_FULLROW
-    colf = capi.Colony(pf, xy=xy)
-    colf.initialize()
-    colf.run(2)
-    t0 = time.perf_counter()
-    colf.run(args.fullrow_iters)
-    wall_f = time.perf_counter() - t0
-    colf.close()
-    fullrow = {"value": m * args.fullrow_iters / wall_f, "unit": "tours/s", "iterations": args.fullrow_iters}
-    log(f"bench: full-row GPU {fullrow['value']:.4g} tours/s")
+    fullrow = None
+    if args.fullrow_iters > 0:
+        pf = c2_params(n, m)
+        pf.mode = capi.MODE_FULLROW
+        colf = capi.Colony(pf, xy=xy)
+        colf.initialize()
+        colf.run(2)
+        t0 = time.perf_counter()
+        colf.run(args.fullrow_iters)
+        wall_f = time.perf_counter() - t0
+        colf.close()
+        fullrow = {"value": m * args.fullrow_iters / wall_f, "unit": "tours/s", "iterations": args.fullrow_iters}
+        log(f"bench: full-row GPU {fullrow['value']:.4g} tours/s")
 
     line = {
         "metric": "tours/sec at n=1000",
