@@ -342,17 +342,26 @@ __device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const
     int32_t bj = 0x7fffffff;
     const float* crow = a.choice + (size_t)c * a.P;
     if (!young) {
-        // 16-byte coalesced loads, 8 per lane in flight (2048 cities per
-        // batch); the row is padded to a multiple of 64 floats.
-        for (int32_t j0 = 0; j0 < a.n; j0 += 2048) {
-            float4 x[8];
+        // 16-byte coalesced loads, 16 per lane in flight (4096 cities per
+        // batch, one memory round trip each); the row is padded to a
+        // multiple of 64 floats.
+        // Non-temporal loads: a fallback row is read once, and streaming it
+        // through the caches would evict the candidate rows every step reads.
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        for (int32_t j0 = 0; j0 < a.n; j0 += 4096) {
+            float4 x[16];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < 16; ++q) {
                 const int32_t j = j0 + 256 * q + 4 * lane;
-                x[q] = j < a.n ? *reinterpret_cast<const float4*>(crow + j) : make_float4(0.f, 0.f, 0.f, 0.f);
+                if (j < a.n) {
+                    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(crow + j));
+                    x[q] = make_float4(v.x, v.y, v.z, v.w);
+                } else {
+                    x[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
             }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < 16; ++q) {
                 const int32_t j = j0 + 256 * q + 4 * lane;
                 if (j < a.n) {
                     const uint32_t word = vis[j >> 5] >> (j & 31);  // 4 cities in one word
