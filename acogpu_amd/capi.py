@@ -35,7 +35,7 @@ class Params(ctypes.Structure):
         ("global_perf", ctypes.c_int32), ("min_pop", ctypes.c_int32), ("max_pop", ctypes.c_int32),
         ("broker", ctypes.c_int32), ("ants_per_block", ctypes.c_int32), ("seed", ctypes.c_uint64),
         ("stream_id", ctypes.c_uint32), ("construct_path", ctypes.c_int32),
-        ("deposit_path", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
+        ("deposit_path", ctypes.c_int32), ("ants_per_wave", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -83,6 +83,7 @@ SIGNATURES = {
     "aco_batch_get_pheromone": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_batch_set_pheromone": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_batch_get_config": (ctypes.c_int, [_vp, _vp]),
+    "aco_batch_get_counters": (ctypes.c_int, [_vp, _vp]),
     "aco_batch_autotune": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_batch_tune_k": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _i32, _i32, ctypes.c_float, _vp, _vp]),
     "aco_batch_exchange_best": (ctypes.c_int, [_vp]),
@@ -430,10 +431,17 @@ class Batch:
         t = np.ascontiguousarray(h, dtype=np.float32)
         self._chk(self._L.aco_batch_set_heuristic_nn(self._ctx, int(i), _p(t)))
 
+    def counters(self):
+        """(count, 2) uint64: fallback steps and cities scanned by them, per instance."""
+        t = np.zeros((self.count, 2), dtype=np.uint64)
+        self._chk(self._L.aco_batch_get_counters(self._ctx, _p(t)))
+        return t
+
     def config(self):
-        t = np.zeros(4, dtype=np.int32)
+        t = np.zeros(5, dtype=np.int32)
         self._chk(self._L.aco_batch_get_config(self._ctx, _p(t)))
-        return {"ants_per_block": int(t[0]), "blocks": int(t[1]), "items": int(t[2]), "lds_bytes": int(t[3])}
+        return {"ants_per_block": int(t[0]), "blocks": int(t[1]), "items": int(t[2]), "lds_bytes": int(t[3]),
+                "ants_per_wave": int(t[4])}
 
 
 def tune_k(params: Params, n_list, xy_list, ks, trial_iterations=20, tolerance=0.005, m_list=None):

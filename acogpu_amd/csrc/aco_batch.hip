@@ -22,7 +22,7 @@ using aco::InstMeta;
 
 struct aco_batch {
     aco_params p;
-    int32_t count = 0, k = 0, nmax = 0, mmax = 0, pmax = 0, apb = 8, grid = 0;
+    int32_t count = 0, k = 0, nmax = 0, mmax = 0, pmax = 0, apb = 8, apw = 1, grid = 0;
     int64_t total_rows = 0, total_ants = 0, mat_elems = 0, nk_elems = 0, tour_elems = 0, dep_elems = 0;
     std::vector<InstMeta> meta;
     std::vector<int32_t> n, m;
@@ -79,23 +79,29 @@ static void bfree(aco_batch* b) {
 extern "C" const char* aco_batch_last_error(const aco_batch* b) { return b ? b->err : g_batch_err; }
 
 // Work items (instance, first ant) for W ants per block, longest instances first.
-static int build_items(aco_batch* b, int32_t W) {
+// Work items of W waves x G ants (one instance each), longest instances
+// first, and the persistent grid that fills the device with them.
+static int build_items(aco_batch* b, int32_t W, int32_t G) {
+    if (G == 1 && b->nmax > 2048 && W > 8)
+        return bfail(b, ACO_E_ARG, "more than 8 ants per block need n <= 2048 (largest n=%d)", b->nmax);
     std::vector<int32_t> order(b->count);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return b->n[x] > b->n[y]; });
     std::vector<int2> it;
     for (int32_t i : order)
-        for (int32_t a0 = 0; a0 < b->m[i]; a0 += W) it.push_back(make_int2(i, a0));
+        for (int32_t a0 = 0; a0 < b->m[i]; a0 += W * G) it.push_back(make_int2(i, a0));
     BHIP(b, hipMemcpy(b->items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice));
     b->buf.nitems = (int32_t)it.size();
     b->buf.ants_per_block = W;
+    b->buf.ants_per_wave = G;
     b->apb = W;
+    b->apw = G;
     int cus = 256, dev = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
     (void)hipGetLastError();
-    const size_t lds = aco::batch_construct_lds_bytes(b->nmax, b->k, W);
-    const int32_t per_cu = aco::batch_construct_blocks_per_cu(W, lds);
+    const size_t lds = aco::batch_construct_lds_bytes(b->nmax, b->k, W, G);
+    const int32_t per_cu = aco::batch_construct_blocks_per_cu(b->nmax, W, G, lds);
     b->grid = std::min<int32_t>(cus * per_cu, b->buf.nitems);
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
@@ -115,8 +121,10 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
         return bfail(nullptr, ACO_E_ARG, "unknown variant %d", p->variant);
     if (!(p->rho >= 0.0f && p->rho <= 1.0f) || !(p->alpha >= 0.0f) || !(p->beta >= 0.0f))
         return bfail(nullptr, ACO_E_ARG, "rho must be in [0,1], alpha and beta >= 0");
-    if (p->ants_per_block < 0 || p->ants_per_block > 8)
-        return bfail(nullptr, ACO_E_ARG, "ants_per_block must be in [0, 8]");
+    if (p->ants_per_block < 0 || p->ants_per_block > 16)
+        return bfail(nullptr, ACO_E_ARG, "ants_per_block must be in [0, 16]");
+    if (p->ants_per_wave != 0 && p->ants_per_wave != 1 && p->ants_per_wave != 2 && p->ants_per_wave != 4)
+        return bfail(nullptr, ACO_E_ARG, "ants_per_wave must be 0, 1, 2 or 4");
     if (p->deposit_path < 0 || p->deposit_path > 2 || (p->deposit_path == 2 && p->variant != ACO_VARIANT_AS))
         return bfail(nullptr, ACO_E_ARG, "deposit_path must be 0, 1 or 2 (2: Ant System only)");
 
@@ -189,6 +197,13 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     if (ants > INT32_MAX / 2 || rows > INT32_MAX / 2) {
         delete b;
         return bfail(nullptr, ACO_E_RANGE, "too many ants or cities in one batch");
+    }
+    const int32_t gmax = aco::batch_max_ants_per_wave(b->nmax, p->k);
+    if (p->ants_per_wave > gmax) {
+        const int32_t nm = b->nmax;
+        delete b;
+        return bfail(nullptr, ACO_E_ARG, "ants_per_wave %d needs k <= %d and n <= %d (k=%d, largest n=%d)",
+                     p->ants_per_wave, 64 / p->ants_per_wave, 2048 / p->ants_per_wave, p->k, nm);
     }
     b->total_rows = rows;
     b->total_ants = ants;
@@ -285,7 +300,7 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     B.seed_lo = (uint32_t)p->seed;
     B.seed_hi = (uint32_t)(p->seed >> 32);
     b->dep_elems = dep;
-    TRYB(build_items(b, p->ants_per_block ? p->ants_per_block : 8));
+    TRYB(build_items(b, p->ants_per_block ? p->ants_per_block : 8, p->ants_per_wave ? p->ants_per_wave : gmax));
     *out = b;
     return ACO_OK;
 #undef TRYB
@@ -384,36 +399,49 @@ extern "C" int aco_batch_autotune(aco_batch* b, int32_t reps, int32_t* chosen) {
     // The sweep writes choice_nn from the current pheromone and the next
     // buffer's evaporated values, which the next real iteration rewrites.
     BHIP(b, aco::launch_batch_iteration_kernel(B, 0, b->nmax, 0, b->stream));
-    int best_w = b->apb;
+    int best_w = b->apb, best_g = b->apw;
     float best_ms = 1e30f;
     int max_lds = 65536, dev = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
         max_lds = 65536;
     (void)hipGetLastError();
-    for (int32_t W : {1, 2, 4, 8}) {
-        if (aco::batch_construct_lds_bytes(b->nmax, b->k, W) + 256 > (size_t)max_lds) continue;
-        int rc = build_items(b, W);
-        if (rc) return rc;
-        B.nitems = b->buf.nitems;
-        B.ants_per_block = W;
-        float total = 0.0f;
-        for (int32_t r = 0; r < reps; ++r) {
-            BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t), b->stream));
-            BHIP(b, hipEventRecord(b->ev[0], b->stream));
-            BHIP(b, aco::launch_batch_iteration_kernel(B, 1, b->nmax, b->grid, b->stream));
-            BHIP(b, hipEventRecord(b->ev[1], b->stream));
-            BHIP(b, hipEventSynchronize(b->ev[1]));
-            float ms = 0.0f;
-            BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
-            total += ms;
-        }
-        if (total < best_ms) {
-            best_ms = total;
-            best_w = W;
+    // one ant per wave and the most the batch allows (unless fixed by
+    // ants_per_wave), each with every block size
+    const int32_t gmax = aco::batch_max_ants_per_wave(b->nmax, b->k);
+    std::vector<int32_t> gs;
+    if (b->p.ants_per_wave)
+        gs.push_back(b->p.ants_per_wave);
+    else
+        gs = gmax > 1 ? std::vector<int32_t>{1, gmax} : std::vector<int32_t>{1};
+    for (int32_t G : gs) {
+        for (int32_t W : {1, 2, 4, 8, 16}) {
+            if (aco::batch_construct_lds_bytes(b->nmax, b->k, W, G) + 256 > (size_t)max_lds) continue;
+            if (G == 1 && b->nmax > 2048 && W > 8) continue;
+            int rc = build_items(b, W, G);
+            if (rc) return rc;
+            B.nitems = b->buf.nitems;
+            B.ants_per_block = W;
+            B.ants_per_wave = G;
+            float total = 0.0f;
+            for (int32_t r = 0; r < reps; ++r) {
+                BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t), b->stream));
+                BHIP(b, hipEventRecord(b->ev[0], b->stream));
+                BHIP(b, aco::launch_batch_iteration_kernel(B, 1, b->nmax, b->grid, b->stream));
+                BHIP(b, hipEventRecord(b->ev[1], b->stream));
+                BHIP(b, hipEventSynchronize(b->ev[1]));
+                float ms = 0.0f;
+                BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+                total += ms;
+            }
+            if (total < best_ms) {
+                best_ms = total;
+                best_w = W;
+                best_g = G;
+            }
         }
     }
-    int rc = build_items(b, best_w);
+    int rc = build_items(b, best_w, best_g);
     if (rc) return rc;
     BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t), b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
@@ -476,7 +504,19 @@ extern "C" int aco_batch_get_config(aco_batch* b, int32_t* config) {
     config[0] = b->apb;
     config[1] = b->grid;
     config[2] = b->buf.nitems;
-    config[3] = (int32_t)aco::batch_construct_lds_bytes(b->nmax, b->k, b->apb);
+    config[3] = (int32_t)aco::batch_construct_lds_bytes(b->nmax, b->k, b->apb, b->apw);
+    config[4] = b->apw;
+    return ACO_OK;
+}
+
+extern "C" int aco_batch_get_counters(aco_batch* b, uint64_t* counters) {
+    if (!b || !counters) return ACO_E_ARG;
+    std::vector<DevState> st(b->count);
+    BHIP(b, hipMemcpy(st.data(), b->buf.state, sizeof(DevState) * b->count, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < b->count; ++i) {
+        counters[2 * i] = st[i].fb_steps;
+        counters[2 * i + 1] = st[i].fb_scanned;
+    }
     return ACO_OK;
 }
 

@@ -149,6 +149,7 @@ struct BatchBuffers {
     float* tau0;             // per-instance initial pheromone
     float alpha, one_minus_rho;
     int32_t variant, w, ants_per_block, tau0_policy;
+    int32_t ants_per_wave;   // construction: ants sharing one wave (1, 2 or 4)
     float tau0_const, rho;
     uint32_t seed_lo, seed_hi;
 };
@@ -156,9 +157,11 @@ struct BatchBuffers {
 hipError_t launch_batch_init(const BatchBuffers& b, int32_t pmax, int32_t nmax, hipStream_t s);
 hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, int32_t nmax, int32_t grid,
                                          hipStream_t s);
-size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block);
+size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block, int32_t ants_per_wave);
+int32_t batch_max_ants_per_wave(int32_t nmax, int32_t k);
 hipError_t launch_batch_exchange(const BatchBuffers& b, hipStream_t s);
-int32_t batch_construct_blocks_per_cu(int32_t ants_per_block, size_t lds_bytes);
+int32_t batch_construct_blocks_per_cu(int32_t nmax, int32_t ants_per_block, int32_t ants_per_wave,
+                                      size_t lds_bytes);
 
 hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s);
 hipError_t launch_fix_diagonal(int32_t* dist, int32_t n, int32_t P, hipStream_t s);

@@ -584,6 +584,63 @@ __device__ __forceinline__ int32_t fallback_argmax_vreg(const ConstructArgs& a, 
     return next;
 }
 
+// Argmax fallback with the ant's bitmap in registers (lane src + w holds word
+// w), spread over the whole wave: lane l reads cities CPL l .. CPL l + CPL - 1
+// (CPL / 4 16-byte loads; n <= 64 CPL), takes their bits from the owner lane
+// with one bpermute and keeps the largest weight, lowest index on ties,
+// through a pairwise tree (the left operand holds the lower indices).  Same
+// result as the sequential scan; call with every lane active.
+template <int CPL>
+__device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a, uint32_t vreg, int src, int32_t c,
+                                                          int lane, unsigned long long& scanned) {
+    const int32_t j0 = CPL * lane;
+    const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((src + (j0 >> 5)) << 2, (int)vreg);
+    uint32_t free_bits = 0u;
+    float best = -1.0f;
+    int32_t bj = 0x7fffffff;
+    if (j0 < a.n) {
+        constexpr uint32_t FULL = CPL == 32 ? 0xffffffffu : ((1u << CPL) - 1u);
+        free_bits = ~(word >> (j0 & 31)) & FULL;
+        if (a.n - j0 < CPL) free_bits &= (1u << (a.n - j0)) - 1u;
+        scanned += __popc(free_bits);
+        float v[CPL];
+        int32_t ix[CPL];
+        const float4* src4 = reinterpret_cast<const float4*>(a.choice + (size_t)c * a.P + j0);
+#pragma unroll
+        for (int q = 0; q < CPL / 4; ++q) {
+            const float4 t = src4[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const bool ok = ((free_bits >> q) & 1u) && v[q] > -1.0f;  // NaN is never taken
+            v[q] = ok ? v[q] : -1.0f;
+            ix[q] = ok ? j0 + q : 0x7fffffff;
+        }
+#pragma unroll
+        for (int w = 1; w < CPL; w *= 2) {
+#pragma unroll
+            for (int q = 0; q + w < CPL; q += 2 * w) {
+                const bool right = v[q + w] > v[q];
+                v[q] = right ? v[q + w] : v[q];
+                ix[q] = right ? ix[q + w] : ix[q];
+            }
+        }
+        best = v[0];
+        bj = ix[0];
+    }
+    int32_t next = wave_argmax_index(best, bj);
+    if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
+        const unsigned long long m = __ballot(free_bits != 0u);
+        const int l = __ffsll((long long)m) - 1;
+        next = CPL * l + __ffs(readlane_i((int)free_bits, l)) - 1;
+    }
+    return next;
+}
+
 // One ant on LDS-staged candidate data.  VREG_VIS: the visited set lives in
 // one VGPR per lane (lane l holds cities 32l..32l+31; n <= 2048) and is read
 // across lanes with ds_bpermute; otherwise an LDS bitmap.  K32: k <= 32, the
@@ -643,8 +700,10 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             next = readlane_i(j, 63 - __clzll((long long)pos));
         } else {
             ++fb;
-            next = VREG_VIS ? fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned)
-                            : fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned);
+            if (!VREG_VIS)
+                next = fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned);
+            else
+                next = fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned);
         }
         e = next * a.k * lmul + loff;
         const int32_t jn = nl[e];
@@ -1551,9 +1610,8 @@ __global__ __launch_bounds__(256) void kb_sweep(const BatchBuffers b) {
 // One persistent launch: blocks take work items (instance, W ants) from an
 // atomic queue, longest instances first, stage the instance's candidate lists
 // and weights in LDS and construct W tours (one per wave).
-__global__ __launch_bounds__(512) void kb_construct(const BatchBuffers b) {
-    extern __shared__ __align__(16) uint32_t lds[];
-    __shared__ int32_t s_item;
+template <bool VREG_VIS>
+__device__ __forceinline__ void kb_construct_body(const BatchBuffers& b, uint32_t* lds, int32_t& s_item) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     for (;;) {
@@ -1576,7 +1634,234 @@ __global__ __launch_bounds__(512) void kb_construct(const BatchBuffers b) {
         staged_zero_entry(L, nk);
         __syncthreads();
         const int32_t ant = it.y + wave;
-        if (ant < im.m) construct_staged_dispatch(a, ant, false, 0.0f, wl, nl, vis, t16, a.state->iter, lane);
+        if (ant < im.m) {
+            if (a.k <= 32)
+                construct_staged_ant<VREG_VIS, true>(a, ant, false, 0.0f, wl, nl, vis, t16, a.state->iter, lane);
+            else
+                construct_staged_ant<VREG_VIS, false>(a, ant, false, 0.0f, wl, nl, vis, t16, a.state->iter, lane);
+        }
+        __syncthreads();
+    }
+}
+
+// Register bitmap (n <= 2048), at most 64 VGPRs (eight waves per SIMD):
+// with the candidate data staged the loop is VALU-issue bound once the SIMDs
+// are full, and the few spills this costs sit on the Philox refill path.
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void kb_construct_vreg(
+    const BatchBuffers b) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    __shared__ int32_t s_item;
+    kb_construct_body<true>(b, lds, s_item);
+}
+
+// LDS bitmap (any n; at most 8 ants per block): its fallback keeps 16 row
+// loads in flight per lane, so the register budget stays at 256.
+__global__ __launch_bounds__(512) void kb_construct_lds(const BatchBuffers b) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    __shared__ int32_t s_item;
+    kb_construct_body<false>(b, lds, s_item);
+}
+
+// ---------------------------------------------------------------------------
+// Several ants per wave (batched instances with small k and n): G ants share
+// a wave, each on its own group of L = 64 / G lanes, when k <= L and n <= 32 L.
+// Lane i of group g holds candidate i of ant g's current row and word i of
+// its visited bitmap, so every step does for G ants at once what the one-ant
+// loop does for one: the scan, ballots and compares stay inside each group
+// (the DPP row shifts never cross a 16-lane row, and for L = 32 the
+// row_bcast:15 step joins rows 0-1 and 2-3 only), which gives every group's
+// prefixes the same bits as the one-ant scan.  The random words are the
+// one-ant stream's: step s uses Philox block 64 (s / 256) + (s mod 64),
+// word (s / 64) mod 4; lane i of group g computes blocks i, i + L, ... of its
+// ant's 256-step chunk, and the step's word is read from lane s mod L of the
+// current queue entry.  Tours go to global memory in L-step segments, and
+// the edge lengths of each segment are loaded then and summed a segment
+// later, so no tour is staged in LDS.
+__device__ __forceinline__ int32_t bperm_i(int32_t src_lane, int32_t v) {
+    return __builtin_amdgcn_ds_bpermute(src_lane << 2, v);
+}
+
+template <int G>
+__device__ __forceinline__ float group_scan(float v) {
+    if constexpr (G == 4) {
+        v = v + dpp_src<0x111, 0xf>(v);
+        v = v + dpp_src<0x112, 0xf>(v);
+        v = v + dpp_src<0x114, 0xf>(v);
+        v = v + dpp_src<0x118, 0xf>(v);
+        return v;
+    } else {
+        return wave_inclusive_scan32(v);
+    }
+}
+
+// The group's last lane's value in every lane of the group.
+template <int G>
+__device__ __forceinline__ float group_last(float p, int lane) {
+    if constexpr (G == 4) {
+        // row_newbcast:15: each 16-lane row reads its own lane 15
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(p), 0x15F, 0xf, 0xf, false));
+    } else {
+        const float t0 = readlane_f(p, 31);
+        const float t1 = readlane_f(p, 63);
+        return lane < 32 ? t0 : t1;
+    }
+}
+
+template <int G>
+__device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int32_t ant0, const float* wl,
+                                                     const uint16_t* nl, uint32_t iter, int lane) {
+    constexpr int L = 64 / G;
+    constexpr uint32_t GMASK = L == 32 ? 0xffffffffu : ((1u << L) - 1u);
+    const int g = lane / L;
+    const int i = lane & (L - 1);
+    const int gb = g * L;
+    const int32_t ant = ant0 + g;
+    const bool valid = ant < a.m;
+    const uint32_t n = (uint32_t)a.n;
+
+    // The words of the current 128-step half chunk in step order, consumed
+    // from q[0]: each half recomputes the chunk's blocks and keeps two of
+    // their four words, which halves the registers the queue holds.
+    uint32_t q[2 * G];
+    auto refill = [&](int32_t s) {
+        const uint32_t blk0 = ((uint32_t)s >> 8) << 6;
+        const bool hi = (s & 128) != 0;
+#pragma unroll
+        for (int b = 0; b < G; ++b) {
+            const Philox4 r = philox4x32_10(blk0 + b * L + i, (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
+            q[b] = hi ? r.x2 : r.x0;
+            q[G + b] = hi ? r.x3 : r.x1;
+        }
+    };
+    auto advance = [&]() {
+#pragma unroll
+        for (int t = 0; t + 1 < 2 * G; ++t) q[t] = q[t + 1];
+    };
+    refill(0);
+    const int32_t start = word_to_city((uint32_t)bperm_i(gb, (int32_t)q[0]), a.n);
+    uint32_t vreg = (i == (start >> 5)) ? (1u << (start & 31)) : 0u;
+    // row stride per lane (0 for lanes >= k): a 24-bit multiply, n k < 2^24
+    const uint32_t kk = i < a.k ? (uint32_t)a.k : 0u;
+    const int32_t loff = i < a.k ? i : a.n * a.k;
+    int32_t c = start;
+    int32_t e = (int32_t)__umul24((uint32_t)c, kk) + loff;
+    int32_t j = nl[e];
+    float cw = wl[e];
+    int32_t tbuf = start;
+    int32_t prev_last = 0;
+    int64_t len = 0;
+    int32_t pend = 0;
+    unsigned long long fb = 0, scanned = 0;
+    int32_t* tour = a.tours + (size_t)ant * a.n;
+    for (int32_t s = 1; s < a.n; ++s) {
+        if (__builtin_expect((s & (L - 1)) == 0, 0)) {
+            if ((s & 127) == 0)
+                refill(s);
+            else
+                advance();
+        }
+        const float u = word_to_uniform((uint32_t)bperm_i(gb + (s & (L - 1)), (int32_t)q[0]));
+        // j >> 3 addresses lane j >> 5 of the group (bpermute drops the two
+        // low address bits; gb * 4 and j >> 3 share no bits for n <= 32 L)
+        const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((gb << 2) + (j >> 3), (int)vreg);
+        const bool vj = (word >> (j & 31)) & 1u;
+        const float w = vj ? 0.0f : cw;
+        const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
+        const float p = group_scan<G>(w);
+        const float tot = group_last<G>(p, lane);
+        const float rr = u * tot;
+        const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
+        const uint32_t bits = (uint32_t)(hit >> gb) & GMASK;
+        int32_t next = bperm_i(gb + __ffs(bits) - 1, j);
+        const unsigned long long nohit = __builtin_amdgcn_ballot_w64(bits == 0u);
+        if (__builtin_expect(nohit != 0ull, 0)) {
+#pragma unroll 1
+            for (int gg = 0; gg < G; ++gg) {
+                if (!((nohit >> (gg * L)) & 1ull)) continue;
+                const float tg = readlane_f(tot, gg * L);
+                const uint32_t pg = (uint32_t)(pos >> (gg * L)) & GMASK;
+                int32_t ng;
+                if (tg > 0.0f && pg != 0u) {  // rounding: the group's last positive lane
+                    ng = readlane_i(j, gg * L + 31 - __clz((int)pg));
+                } else {
+                    const bool vg = ant0 + gg < a.m;
+                    fb += vg ? 1 : 0;
+                    unsigned long long sc = 0;
+                    if constexpr (G == 4) {  // eight cities per lane over the whole wave
+                        ng = fallback_argmax_spread<8>(a, vreg, gg * L, readlane_i(c, gg * L), lane, sc);
+                    } else {  // 32 cities on each of the group's lanes' words
+                        const uint32_t vword = (uint32_t)bperm_i((gg * L + lane) & 63, (int32_t)vreg);
+                        ng = fallback_argmax_vreg(a, vword, readlane_i(c, gg * L), false, 0.0f, lane, sc);
+                    }
+                    scanned += vg ? sc : 0ull;
+                }
+                next = (g == gg) ? ng : next;
+            }
+        }
+        e = (int32_t)__umul24((uint32_t)next, kk) + loff;
+        const int32_t jn = nl[e];
+        const float cwn = wl[e];
+        vreg |= (i == (next >> 5)) ? (1u << (next & 31)) : 0u;
+        tbuf = (i == (s & (L - 1))) ? next : tbuf;
+        if (__builtin_expect((s & (L - 1)) == L - 1 || s == a.n - 1, 0)) {
+            // segment s0 .. s: write it, add the previous segment's edge
+            // lengths and load this one's (edge into entry s0 + i)
+            const int32_t s0 = s & ~(L - 1);
+            const int32_t cnt = s - s0 + 1;
+            // with every lane active: bpermute reads 0 from an inactive lane,
+            // and a conditional operand would run it only where taken
+            const int32_t left = bperm_i(lane - 1, tbuf);
+            const int32_t from = i == 0 ? prev_last : left;
+            len += pend;
+            pend = 0;
+            if (valid && i < cnt) {
+                tour[s0 + i] = tbuf;
+                if (s0 + i > 0) pend = a.dist[(size_t)from * a.P + tbuf];
+            }
+            prev_last = bperm_i(gb + L - 1, tbuf);
+        }
+        c = next;
+        j = jn;
+        cw = cwn;
+    }
+    (void)n;
+    len += pend;
+    if (valid && i == 0) len += a.dist[(size_t)c * a.P + start];
+#pragma unroll
+    for (int off = L / 2; off >= 1; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)len, off);
+        const uint32_t hi = __shfl_xor((uint32_t)((uint64_t)len >> 32), off);
+        len += (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    if (valid && i == 0) a.lengths[ant] = len;
+    finish_counters(a, fb, scanned, lane);
+}
+
+// Persistent batched construction with G ants per wave: work items of
+// W * G ants, candidate data staged as in kb_construct, no per-wave LDS.
+template <int G>
+__global__ __launch_bounds__(1024) void kb_construct_group(const BatchBuffers b) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    __shared__ int32_t s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int32_t)atomicAdd(b.queue, 1u);
+        __syncthreads();
+        const int32_t item = s_item;
+        __syncthreads();
+        if (item >= b.nitems) return;
+        const int2 it = b.items[item];
+        const InstMeta im = b.meta[it.x];
+        const ConstructArgs a = batch_construct_args(b, it.x, im);
+        const int32_t nk = im.n * b.k;
+        const StagedLayout L = staged_layout(lds, nk);
+        stage_u16(L.nl, a.nn16, nk);
+        stage_f32(L.wl, a.choice_nn, nk);
+        staged_zero_entry(L, nk);
+        __syncthreads();
+        const int32_t ant0 = it.y + wave * G;
+        if (ant0 < im.m) construct_group_wave<G>(a, ant0, L.wl, L.nl, a.state->iter, lane);
         __syncthreads();
     }
 }
@@ -1747,14 +2032,26 @@ hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block) {
-    return staged_lds_bytes(nmax, k, (nmax + 31) / 32, ants_per_block);
+size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block, int32_t ants_per_wave) {
+    // several ants per wave keep bitmaps in registers and tours in global
+    // memory: only the staged candidate data
+    return staged_lds_bytes(nmax, k, (nmax + 31) / 32, ants_per_wave > 1 ? 0 : ants_per_block);
 }
 
-int32_t batch_construct_blocks_per_cu(int32_t ants_per_block, size_t lds_bytes) {
+int32_t batch_max_ants_per_wave(int32_t nmax, int32_t k) {
+    if (k <= 16 && nmax <= 512) return 4;
+    if (k <= 32 && nmax <= 1024) return 2;
+    return 1;
+}
+
+int32_t batch_construct_blocks_per_cu(int32_t nmax, int32_t ants_per_block, int32_t ants_per_wave,
+                                      size_t lds_bytes) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kb_construct, 64 * ants_per_block, lds_bytes) != hipSuccess)
-        nb = 1;
+    const void* f = ants_per_wave == 4   ? reinterpret_cast<const void*>(kb_construct_group<4>)
+                    : ants_per_wave == 2 ? reinterpret_cast<const void*>(kb_construct_group<2>)
+                    : nmax <= 2048       ? reinterpret_cast<const void*>(kb_construct_vreg)
+                                         : reinterpret_cast<const void*>(kb_construct_lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * ants_per_block, lds_bytes) != hipSuccess) nb = 1;
     return nb > 0 ? nb : 1;
 }
 
@@ -1778,10 +2075,22 @@ hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, i
         case 0:
             hipLaunchKernelGGL(kb_sweep, dim3(cdiv(b.total_rows, 4)), dim3(256), 0, s, b);
             break;
-        case 1:
-            hipLaunchKernelGGL(kb_construct, dim3(grid), dim3(64 * b.ants_per_block),
-                               batch_construct_lds_bytes(nmax, b.k, b.ants_per_block), s, b);
+        case 1: {
+            // several ants per wave, else one: register bitmap (n <= 2048)
+            // or LDS bitmap; one instantiation each keeps the common
+            // kernel's register count low
+            const size_t lds = batch_construct_lds_bytes(nmax, b.k, b.ants_per_block, b.ants_per_wave);
+            const dim3 blk(64 * b.ants_per_block);
+            if (b.ants_per_wave == 4)
+                hipLaunchKernelGGL(kb_construct_group<4>, dim3(grid), blk, lds, s, b);
+            else if (b.ants_per_wave == 2)
+                hipLaunchKernelGGL(kb_construct_group<2>, dim3(grid), blk, lds, s, b);
+            else if (nmax <= 2048)
+                hipLaunchKernelGGL(kb_construct_vreg, dim3(grid), blk, lds, s, b);
+            else
+                hipLaunchKernelGGL(kb_construct_lds, dim3(grid), blk, lds, s, b);
             break;
+        }
         case 2:  // grid carries the RBAS ranking bitmap words, (max m + 31) / 32
             hipLaunchKernelGGL(kb_stats, dim3(b.count), dim3(256), (size_t)grid * sizeof(uint32_t), s, b);
             break;

@@ -77,12 +77,14 @@ typedef struct aco_params {
     int32_t min_pop;        /* -x minPopulationLimit                          */
     int32_t max_pop;        /* -y maxPopulationLimit                          */
     int32_t broker;         /* "broker" build variant: accepted, no behaviour */
-    int32_t ants_per_block; /* construction launch shape 1..8, 0 = automatic  */
+    int32_t ants_per_block; /* construction launch shape 1..8 (batch 1..16; 1..8 if n > 2048), 0 = automatic */
     uint64_t seed;          /* Philox key (reference: RANDOM_SEED, inc/Colony.h:15) */
     uint32_t stream_id;     /* Philox counter word 3 (instance id)            */
     int32_t construct_path; /* candidate mode: 0 auto, 1 global memory, 2 LDS-staged */
     int32_t deposit_path;   /* Ant System: 0 auto, 1 fp32 atomics, 2 adjacency gather (n <= 32768) */
-    int32_t reserved[5];
+    int32_t ants_per_wave;  /* batch construction: ants sharing one wave, 1, 2 (k <= 32, n <= 1024)
+                               or 4 (k <= 16, n <= 512); 0 = the most that fit */
+    int32_t reserved[4];
 } aco_params;
 
 /* Reference defaults (inc/default.h:8-39, src/Colony.cu:32-36). */
@@ -202,13 +204,19 @@ int aco_batch_get_tours(aco_batch *b, int32_t instance, int32_t *tours);     /* 
 int aco_batch_get_lengths(aco_batch *b, int32_t instance, int64_t *lengths); /* m[i]       */
 int aco_batch_get_pheromone(aco_batch *b, int32_t instance, float *tau);      /* n[i] x n[i] */
 int aco_batch_set_pheromone(aco_batch *b, int32_t instance, const float *tau);
-/* config = {ants per block, persistent blocks, work items, LDS bytes per block} */
+/* config[5] = {ants per block, persistent blocks, work items, LDS bytes per
+ * block, ants per wave} */
 int aco_batch_get_config(aco_batch *b, int32_t *config);
+/* counters[2 * count] = per instance {fallback steps, unvisited cities
+ * scanned by fallback steps} since create, as aco_get_counters */
+int aco_batch_get_counters(aco_batch *b, uint64_t *counters);
 /* candidate-edge heuristic of one instance (n[i] * k values), as aco_set_heuristic_nn */
 int aco_batch_set_heuristic_nn(aco_batch *b, int32_t instance, const float *h);
-/* Times construction launches for each ants-per-block choice (a launch
- * parameter only: tours do not change) and keeps the fastest; reps launches
- * per choice; chosen (may be NULL) gets the choice. */
+/* Times construction launches for each ants-per-block choice, with one ant
+ * per wave and with the most ants per wave the batch allows (unless
+ * ants_per_wave fixes it): launch parameters only, tours do not change.
+ * Keeps the fastest; reps launches per choice; chosen (may be NULL) gets the
+ * ants per block (aco_batch_get_config reports both). */
 int aco_batch_autotune(aco_batch *b, int32_t reps, int32_t *chosen);
 /* Candidate-list width tuner: builds a batch of the same instances for each
  * k in ks[nks], runs trial_iterations iterations (after aco_batch_autotune)

@@ -26,18 +26,23 @@ def batch_params(**kw):
     return capi.default_params(**base)
 
 
-@pytest.mark.parametrize("apb", [1, 4, 8])
-def test_batch_first_iterations_equal_single_colonies(apb):
-    say(f"case: batch of 10 vs single colonies, ants_per_block={apb}")
+# (ants per block, ants per wave, k): one ant per wave, and 2 or 4 ants
+# sharing a wave (k <= 32 or 16), where k below the group width leaves lanes
+# idle
+@pytest.mark.parametrize("apb,apw,k", [(1, 1, 16), (4, 1, 16), (8, 1, 16), (4, 2, 16), (8, 4, 16), (16, 4, 16),
+                                       (2, 2, 19), (8, 4, 9)])
+def test_batch_first_iterations_equal_single_colonies(apb, apw, k):
+    say(f"case: batch of 10 vs single colonies, ants_per_block={apb} ants_per_wave={apw} k={k}")
     sizes, xys = make_batch_inputs(10, 20, 300, seed=1)
-    p = batch_params(ants_per_block=apb)
+    p = batch_params(ants_per_block=apb, ants_per_wave=apw, k=k)
     with capi.Batch(p, sizes, xys) as b:
         b.initialize()
+        assert b.config()["ants_per_wave"] == apw
         b.run(1)
         gb, best = b.best()
         for i in (0, 3, 7, 9):
             n = int(sizes[i])
-            q = batch_params(n=n, m=n, stream_id=i)
+            q = batch_params(n=n, m=n, stream_id=i, k=k)
             with capi.Colony(q, xy=xys[i]) as c:
                 c.initialize()
                 c.iterate()
@@ -46,6 +51,28 @@ def test_batch_first_iterations_equal_single_colonies(apb):
                 assert gb[i] == c.stats()[1]
                 np.testing.assert_array_equal(best[i], c.best_tour())
                 np.testing.assert_allclose(b.pheromone(i), c.pheromone(), rtol=1e-6)
+
+
+def test_batch_above_2048_cities_equals_single_colonies():
+    """n > 2048: the visited set no longer fits one register per lane and
+    the batch takes its LDS-bitmap construction kernel."""
+    say("case: batch of 2, n in {2100, 2300}, m = 8, k = 8 vs single colonies")
+    sizes = np.array([2100, 2300], dtype=np.int32)
+    xys = [instances.uniform(int(n), 10000, seed=700 + i) for i, n in enumerate(sizes)]
+    p = batch_params(k=8)
+    with capi.Batch(p, sizes, xys, [8, 8]) as b:
+        b.initialize()
+        assert b.config()["ants_per_wave"] == 1 and b.config()["ants_per_block"] <= 8
+        b.run(1)
+        for i in range(2):
+            n = int(sizes[i])
+            with capi.Colony(batch_params(n=n, m=8, k=8, stream_id=i), xy=xys[i]) as c:
+                c.initialize()
+                c.iterate()
+                np.testing.assert_array_equal(b.tours(i), c.tours())
+                np.testing.assert_array_equal(b.lengths(i), c.lengths())
+        with pytest.raises(capi.AcoError):
+            capi.Batch(batch_params(k=8, ants_per_block=16), sizes, xys, [8, 8])
 
 
 @pytest.mark.parametrize("deposit_path", [1, 2])
@@ -102,18 +129,51 @@ def test_batch_deterministic_rank_based_run_matches_oracle():
 
 
 def test_batch_autotune_changes_only_the_launch():
-    say("case: autotune on a batch of 24")
+    say("case: autotune on a batch of 24 vs one ant per wave")
     sizes, xys = make_batch_inputs(24, 50, 500, seed=4)
-    p = batch_params(ants_per_block=1)
-    with capi.Batch(p, sizes, xys) as a, capi.Batch(p, sizes, xys) as b:
+    with capi.Batch(batch_params(ants_per_block=1, ants_per_wave=1), sizes, xys) as a, \
+            capi.Batch(batch_params(ants_per_block=1), sizes, xys) as b:
         a.initialize()
         b.initialize()
         w = b.autotune(reps=2)
-        assert w in (1, 2, 4, 8) and b.config()["ants_per_block"] == w
+        assert w in (1, 2, 4, 8, 16) and b.config()["ants_per_block"] == w
+        assert b.config()["ants_per_wave"] in (1, 4)
         a.run(1)
         b.run(1)
         for i in range(24):
             np.testing.assert_array_equal(a.tours(i), b.tours(i))
+
+
+@pytest.mark.parametrize("apw", [2, 4])
+def test_batch_ants_sharing_a_wave_through_fallbacks(apw):
+    """k = 3 leaves most steps with every candidate visited: the argmax
+    fallback and the rounding path run for some groups of a wave while the
+    others take the roulette.  Tours, lengths and the fallback counters equal
+    the one-ant-per-wave launch's; n from 5 (fewer steps than a group has
+    lanes) up, m not a multiple of the ants per item."""
+    say(f"case: batch of 40, k=3, n in [5, 200], ants_per_wave={apw} vs 1, 3 iterations")
+    sizes, xys = make_batch_inputs(40, 5, 200, seed=11)
+    ms = [max(1, int(n) // 3 + 1) for n in sizes]
+    with capi.Batch(batch_params(k=3, ants_per_wave=1, ants_per_block=4), sizes, xys, ms) as a, \
+            capi.Batch(batch_params(k=3, ants_per_wave=apw, ants_per_block=4), sizes, xys, ms) as b:
+        assert b.config()["ants_per_wave"] == apw
+        a.initialize()
+        b.initialize()
+        for it in range(3):
+            a.run(1)
+            b.run(1)
+            for i in range(40):
+                np.testing.assert_array_equal(a.tours(i), b.tours(i))
+                np.testing.assert_array_equal(a.lengths(i), b.lengths(i))
+            np.testing.assert_array_equal(a.counters(), b.counters())
+            # the next iteration starts from the same pheromone in both
+            for i in range(40):
+                b.set_pheromone(i, a.pheromone(i))
+        i = int(np.argmax(sizes))
+        dist = orc.dist_from_int_coords(xys[i])
+        for t, l in zip(b.tours(i), b.lengths(i)):
+            assert sorted(t.tolist()) == list(range(int(sizes[i])))
+            assert instances.tour_length(dist, t) == l
 
 
 def test_batch_4096_uneven_instances_properties():
