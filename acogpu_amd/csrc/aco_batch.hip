@@ -183,7 +183,7 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
         im.dep_off = (int32_t)dep;
         im.pad = 0;
         im.adj_off = adj;
-        adj += (int64_t)im.n * 2 * im.m;
+        adj += (int64_t)im.n * aco::adj_pitch(im.m);
         mat += (int64_t)im.n * im.P;
         nk += (int64_t)im.n * p->k;
         tours += (int64_t)im.m * im.n;
@@ -197,6 +197,12 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     if (ants > INT32_MAX / 2 || rows > INT32_MAX / 2) {
         delete b;
         return bfail(nullptr, ACO_E_RANGE, "too many ants or cities in one batch");
+    }
+    if (p->deposit_path == 2 && b->nmax > aco::ADJ_BATCH_MAX_N) {
+        const int32_t nm = b->nmax;
+        delete b;
+        return bfail(nullptr, ACO_E_ARG, "the batched adjacency deposit needs n <= %d (largest n=%d)",
+                     aco::ADJ_BATCH_MAX_N, nm);
     }
     const int32_t gmax = aco::batch_max_ants_per_wave(b->nmax, p->k);
     if (p->ants_per_wave > gmax) {
@@ -249,8 +255,18 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     TRYB(balloc(b, &B.dep_ant, (size_t)dep));
     TRYB(balloc(b, &B.dep_val, (size_t)dep));
     B.adj = nullptr;
-    if (p->variant == ACO_VARIANT_AS && p->deposit_path != 1 && b->nmax <= aco::ADJ_MAX_N)
+    B.adj_items = nullptr;
+    std::vector<int2> adj_items;
+    if (p->variant == ACO_VARIANT_AS && p->deposit_path != 1 && b->nmax <= aco::ADJ_BATCH_MAX_N) {
         TRYB(balloc(b, &B.adj, (size_t)adj));
+        B.adj_ants = aco::adj_stage_ants(b->nmax);
+        for (int32_t i = 0; i < count; ++i)
+            for (int32_t a0 = 0; a0 < b->m[i]; a0 += B.adj_ants) adj_items.push_back(make_int2(i, a0));
+        int2* d_items = nullptr;
+        TRYB(balloc(b, &d_items, adj_items.size()));
+        B.adj_items = d_items;
+        B.nadj_items = (int32_t)adj_items.size();
+    }
     TRYB(balloc(b, &B.state, (size_t)count));
     TRYB(balloc(b, &b->items, (size_t)ants));
     TRYB(balloc(b, &B.queue, 1));
@@ -272,6 +288,9 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     HIPB(hipMemcpy(d_ant_inst, ant_inst.data(), sizeof(int32_t) * ants, hipMemcpyHostToDevice));
     HIPB(hipMemcpy(d_xy, xy, sizeof(int32_t) * 2 * rows, hipMemcpyHostToDevice));
     HIPB(hipMemcpy(d_eta, eta.data(), sizeof(float) * eta.size(), hipMemcpyHostToDevice));
+    if (!adj_items.empty())
+        HIPB(hipMemcpy(const_cast<int2*>(B.adj_items), adj_items.data(), sizeof(int2) * adj_items.size(),
+                       hipMemcpyHostToDevice));
     HIPB(hipMemset(B.dist, 0, sizeof(int32_t) * mat));
     HIPB(hipMemset(B.choice, 0, sizeof(float) * mat));
     HIPB(hipMemset(B.tours, 0, sizeof(int32_t) * tours));

@@ -45,7 +45,7 @@ struct aco_ctx {
     uint8_t* d_valid = nullptr;
     int32_t* d_best_tour = nullptr;
     int32_t* d_dep_ant = nullptr;
-    int32_t* d_adj = nullptr;  // Ant System adjacency slots, n x 2m (null: atomic deposit)
+    uint16_t* d_adj = nullptr;  // Ant System adjacency slots, n x adj_pitch(m) (null: atomic deposit)
     float* d_dep_val = nullptr;
     DevState* d_state = nullptr;
     int64_t* d_scratch64 = nullptr;
@@ -270,7 +270,7 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     TRYB(dalloc(c, &c->d_dep_ant, (size_t)c->m + c->p.w + 2));
     TRYB(dalloc(c, &c->d_dep_val, (size_t)c->m + c->p.w + 2));
     if (c->p.variant == ACO_VARIANT_AS && c->n <= aco::ADJ_MAX_N && c->p.deposit_path != 1)
-        TRYB(dalloc(c, &c->d_adj, (size_t)c->n * 2 * c->m));
+        TRYB(dalloc(c, &c->d_adj, (size_t)c->n * aco::adj_pitch(c->m)));
     TRYB(dalloc(c, &c->d_state, 1));
     TRYB(dalloc(c, &c->d_scratch64, 4));
     HIPB(hipMemsetAsync(c->d_dist, 0, nP * sizeof(int32_t), c->stream));

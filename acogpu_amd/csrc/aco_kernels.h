@@ -95,14 +95,29 @@ struct DepositArgs {
     const DevState* state;
     float* tau_next;
     int32_t n, P, dense;
-    // Ant System adjacency deposit (dense list): row x of adj (2m slots)
-    // holds, for ant a, its successor (slot 2a) and predecessor (2a+1) of x.
-    int32_t* adj;
+    // Ant System adjacency deposit (dense list): row x of adj (adj_pitch(m)
+    // 16-bit slots) holds, for ant a, its successor (slot 2a) and
+    // predecessor (2a+1) of x.
+    uint16_t* adj;
     int32_t m;
 };
 
 // Largest n whose deposit row (n fp32) is accumulated in LDS.
 constexpr int32_t ADJ_MAX_N = 32768;
+
+// Slots per adjacency row: 2m rounded up to 16, so rows are 32-byte aligned
+// and the gather's 16-slot loads never leave their row.
+__host__ __device__ inline int32_t adj_pitch(int32_t m) { return (2 * m + 15) & ~15; }
+
+// Largest n of a batch whose rows gather in LDS four at a time.
+constexpr int32_t ADJ_BATCH_MAX_N = 10240;
+
+// Ants per staged scatter block for rows of n cities: the block's columns of
+// every row (n x 2A 16-bit slots) fit in LDS.  0: n too large, slots are
+// stored one by one.
+__host__ __device__ inline int32_t adj_stage_ants(int32_t n) {
+    return n <= 1280 ? 32 : n <= 2560 ? 16 : n <= 5120 ? 8 : n <= 10240 ? 4 : 0;
+}
 
 // ---------------------------------------------------------------------------
 // Batched instances: all matrices of all instances live in shared buffers;
@@ -116,7 +131,7 @@ struct InstMeta {
     int32_t city_off;  // offset in per-city arrays (global row of city 0)
     int32_t dep_off;   // offset in the deposit lists (m + w + 2 per instance)
     int32_t pad;
-    int64_t adj_off;   // offset in the adjacency slots (n x 2m per instance)
+    int64_t adj_off;   // offset in the adjacency slots (n x adj_pitch(m) per instance)
 };
 
 struct BatchBuffers {
@@ -140,7 +155,9 @@ struct BatchBuffers {
     int32_t* best_tours;     // per-city layout (city_off)
     int32_t* dep_ant;
     float* dep_val;
-    int32_t* adj;            // Ant System adjacency slots (null: atomic deposit)
+    uint16_t* adj;           // Ant System adjacency slots (null: atomic deposit)
+    const int2* adj_items;   // staged scatter blocks (instance, first ant), adj_ants ants each
+    int32_t nadj_items, adj_ants;
     DevState* state;         // one per instance
     const int2* items;       // construction work items (instance, first ant), longest first
     int32_t nitems;

@@ -1404,50 +1404,143 @@ __global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
 // Ant System deposit in two passes with no global atomics; the same sum as
 // deposit_entry (src/AntSystem.cu:295-317): tau_xy += dep_val[a] for every
 // ant a whose tour uses edge (x, y), in both directions.
-// Pass 1, one block per ant: ant a writes succ_a(x) to slot 2a and pred_a(y)
-// to slot 2a+1 of rows x and y of its tour edges (x, y).  Plain stores: every
-// slot has exactly one writer.
-__device__ __forceinline__ void adj_scatter_ant(const DepositArgs& a, int32_t ant) {
+// Pass 1: ant a writes succ_a(x) to slot 2a and pred_a(y) to slot 2a+1 of
+// rows x and y of its tour edges (x, y), one wave per ant, 64 edges per
+// store.  Plain 16-bit stores: every slot has exactly one writer.  A block
+// takes 32 consecutive ants, whose slots share each row's cache lines, so
+// the partial-line stores meet in one L2.
+__device__ __forceinline__ void adj_scatter_ant(const DepositArgs& a, int32_t ant, int lane) {
     if (ant >= a.state->ndep || a.dep_val[ant] == 0.0f) return;
     const int32_t* t = a.tours + (size_t)ant * a.n;
-    const size_t pitch = 2 * (size_t)a.m;
-    for (int32_t i = threadIdx.x; i < a.n; i += blockDim.x) {
+    const size_t pitch = (size_t)adj_pitch(a.m);
+    for (int32_t i = lane; i < a.n; i += 64) {
         const int32_t x = t[i];
         const int32_t y = t[(i + 1 == a.n) ? 0 : i + 1];
-        a.adj[(size_t)x * pitch + 2 * ant] = y;
-        a.adj[(size_t)y * pitch + 2 * ant + 1] = x;
+        a.adj[(size_t)x * pitch + 2 * ant] = (uint16_t)y;
+        a.adj[(size_t)y * pitch + 2 * ant + 1] = (uint16_t)x;
     }
 }
 
-// Pass 2, one block per pheromone row x: the row's 2m slots are summed into
-// an LDS row by their ants' dep_val (slots of ants with dep_val 0 are stale
-// and skipped), then added once to tau_next where nonzero.
-__device__ __forceinline__ void adj_gather_row(const DepositArgs& a, int32_t x, float* delta) {
-    for (int32_t j = threadIdx.x; j < a.n; j += blockDim.x) delta[j] = 0.0f;
-    __syncthreads();
-    const int32_t* row = a.adj + (size_t)x * 2 * a.m;
-    const int32_t slots = 2 * min(a.m, a.state->ndep);
-    for (int32_t s = threadIdx.x; s < slots; s += blockDim.x) {
-        const float d = a.dep_val[s >> 1];
-        if (d != 0.0f) atomicAdd(&delta[row[s]], d);
+// Pass 1, staged (A = adj_stage_ants(n) ants a0 .. a0 + A - 1 of one
+// instance per block): the ants write their slots into an LDS copy of their
+// 2A columns of every row, which the block then stores row by row in whole
+// 16-byte chunks, so each row's slots of these ants reach memory in one
+// piece instead of 2A scattered 2-byte stores that each cost a line write.
+// Columns of ants past m or without a deposit hold garbage, which the
+// gather never reads (dep_val 0 or beyond ndep).
+__device__ __forceinline__ void adj_scatter_staged(const DepositArgs& a, int32_t a0, int32_t A, uint16_t* st) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int32_t w2 = 2 * A;
+    for (int32_t j = wave; j < A; j += nw) {
+        const int32_t ant = a0 + j;
+        if (ant >= a.m || ant >= a.state->ndep || a.dep_val[ant] == 0.0f) continue;
+        const int32_t* t = a.tours + (size_t)ant * a.n;
+        for (int32_t i = lane; i < a.n; i += 64) {
+            const int32_t x = t[i];
+            const int32_t y = t[(i + 1 == a.n) ? 0 : i + 1];
+            st[x * w2 + 2 * j] = (uint16_t)y;
+            st[y * w2 + 2 * j + 1] = (uint16_t)x;
+        }
     }
     __syncthreads();
+    const int32_t pitch = adj_pitch(a.m);
+    const int32_t chunks = w2 / 8;                // 16-byte chunks of a row's columns
+    const int32_t lim = min(w2, pitch - 2 * a0);  // columns inside the row (a multiple of 8)
+    for (int32_t idx = threadIdx.x; idx < a.n * chunks; idx += blockDim.x) {
+        const int32_t r = idx / chunks, c = idx - r * chunks;
+        if (8 * c >= lim) continue;
+        *reinterpret_cast<uint4*>(a.adj + (size_t)r * pitch + 2 * a0 + 8 * c) =
+            *reinterpret_cast<const uint4*>(st + r * w2 + 8 * c);
+    }
+}
+
+// Pass 2, one row x per `nthr` threads (thread t of the row): the row's
+// slots are summed into the LDS row `delta` (zero on entry, zero again on
+// return) by their ants' dep_val (slots of ants with dep_val 0 are stale and
+// skipped), then added once to tau_next where nonzero.  Each thread reads
+// the slots of 8 consecutive ants as two 16-byte loads and runs its
+// successors and its predecessors into one pending sum each, which it adds
+// to LDS only when the target changes: once the colony agrees on a row's
+// neighbours, most of the row's LDS atomics disappear.  (Float sums then
+// associate differently from a per-slot order; both orders are within the
+// fp32 rounding the deposit's tests allow.)
+__device__ __forceinline__ void adj_gather_row(const DepositArgs& a, int32_t x, float* delta, int t, int nthr) {
+    const uint16_t* row = a.adj + (size_t)x * adj_pitch(a.m);
+    const int32_t nd = min(a.m, a.state->ndep);
+    for (int32_t a0 = 8 * t; a0 < nd; a0 += 8 * nthr) {
+        uint16_t sl[16];
+        const uint4 v0 = *reinterpret_cast<const uint4*>(row + 2 * a0);
+        const uint4 v1 = *reinterpret_cast<const uint4*>(row + 2 * a0 + 8);
+        const uint32_t w32[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            sl[2 * q] = (uint16_t)(w32[q] & 0xffffu);
+            sl[2 * q + 1] = (uint16_t)(w32[q] >> 16);
+        }
+        int32_t ts = -1, tp = -1;
+        float as = 0.0f, ap = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (a0 + q >= nd) break;
+            const float d = a.dep_val[a0 + q];
+            if (d == 0.0f) continue;
+            const int32_t su = sl[2 * q], pr = sl[2 * q + 1];
+            if (su == ts) {
+                as += d;
+            } else {
+                if (ts >= 0) atomicAdd(&delta[ts], as);
+                ts = su;
+                as = d;
+            }
+            if (pr == tp) {
+                ap += d;
+            } else {
+                if (tp >= 0) atomicAdd(&delta[tp], ap);
+                tp = pr;
+                ap = d;
+            }
+        }
+        if (ts >= 0) atomicAdd(&delta[ts], as);
+        if (tp >= 0) atomicAdd(&delta[tp], ap);
+    }
+}
+
+// Second half of pass 2: the row's sums go to tau_next and the LDS row is
+// cleared for the next row.
+__device__ __forceinline__ void adj_apply_row(const DepositArgs& a, int32_t x, float* delta, int t, int nthr) {
     float* tn = a.tau_next + (size_t)x * a.P;
-    for (int32_t j = threadIdx.x; j < a.n; j += blockDim.x) {
+    for (int32_t j = t; j < a.n; j += nthr) {
         const float v = delta[j];
-        if (v != 0.0f) tn[j] = tn[j] + v;
+        if (v != 0.0f) {
+            tn[j] = tn[j] + v;
+            delta[j] = 0.0f;
+        }
     }
 }
 
+// n > 10240: 32 ants per block, one wave per ant, slots stored one by one.
 __global__ __launch_bounds__(256) void k_adj_scatter(const DepositArgs a) {
     if (a.state->skip) return;
-    adj_scatter_ant(a, blockIdx.x);
+    const int32_t ant = blockIdx.x * 32 + (threadIdx.x >> 6);
+    for (int r = 0; r < 8; ++r)
+        if (ant + 4 * r < a.m) adj_scatter_ant(a, ant + 4 * r, threadIdx.x & 63);
 }
 
+__global__ __launch_bounds__(256) void k_adj_scatter_staged(const DepositArgs a, int32_t A) {
+    extern __shared__ __align__(16) uint16_t adj_stage[];
+    if (a.state->skip) return;
+    adj_scatter_staged(a, blockIdx.x * A, A, adj_stage);
+}
+
+// One block per row: the LDS row holds n floats (n <= ADJ_MAX_N).
 __global__ __launch_bounds__(256) void k_adj_gather(const DepositArgs a) {
     extern __shared__ float delta_row[];
     if (a.state->skip) return;
-    adj_gather_row(a, blockIdx.x, delta_row);
+    for (int32_t j = threadIdx.x; j < a.n; j += blockDim.x) delta_row[j] = 0.0f;
+    __syncthreads();
+    adj_gather_row(a, blockIdx.x, delta_row, threadIdx.x, blockDim.x);
+    __syncthreads();
+    adj_apply_row(a, blockIdx.x, delta_row, threadIdx.x, blockDim.x);
 }
 
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t count) {
@@ -1917,18 +2010,30 @@ __global__ __launch_bounds__(256) void kb_exchange(const BatchBuffers b) {
     if (threadIdx.x == 0) b.state[me].glob_best = win.len;
 }
 
-// AS: one block per ant of every instance; RBAS: w + 1 blocks per instance.
+// One staged scatter block per (instance, first ant) item.
 __global__ __launch_bounds__(256) void kb_adj_scatter(const BatchBuffers b) {
-    const int32_t inst = b.ant_inst[blockIdx.x];
-    const InstMeta im = b.meta[inst];
-    adj_scatter_ant(batch_deposit_args(b, inst, im), blockIdx.x - im.ant_off);
+    extern __shared__ __align__(16) uint16_t adj_stage_b[];
+    const int2 it = b.adj_items[blockIdx.x];
+    const InstMeta im = b.meta[it.x];
+    adj_scatter_staged(batch_deposit_args(b, it.x, im), it.y, b.adj_ants, adj_stage_b);
 }
 
-__global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b) {
-    extern __shared__ float delta_row[];
-    const int32_t inst = b.row_inst[blockIdx.x];
+// One wave per row, four rows per block; each wave's LDS row holds nmax
+// floats (dynamic LDS 4 nmax floats).
+__global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b, int32_t nmax) {
+    extern __shared__ float delta_rows[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* delta = delta_rows + (size_t)wave * nmax;
+    const int32_t r = blockIdx.x * 4 + wave;
+    if (r >= b.total_rows) return;
+    const int32_t inst = b.row_inst[r];
     const InstMeta im = b.meta[inst];
-    adj_gather_row(batch_deposit_args(b, inst, im), blockIdx.x - im.city_off, delta_row);
+    const DepositArgs a = batch_deposit_args(b, inst, im);
+    for (int32_t j = lane; j < a.n; j += 64) delta[j] = 0.0f;
+    wave_lds_sync();
+    adj_gather_row(a, r - im.city_off, delta, lane, 64);
+    wave_lds_sync();
+    adj_apply_row(a, r - im.city_off, delta, lane, 64);
 }
 
 __global__ __launch_bounds__(256) void kb_deposit(const BatchBuffers b) {
@@ -2027,7 +2132,11 @@ hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s) {
 
 hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s) {
     if (!a.adj || !a.dense || a.n > ADJ_MAX_N) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_adj_scatter, dim3(a.m), dim3(256), 0, s, a);
+    const int32_t A = adj_stage_ants(a.n);
+    if (A)
+        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(256), (size_t)4 * a.n * A, s, a, A);
+    else
+        hipLaunchKernelGGL(k_adj_scatter, dim3(cdiv(a.m, 32)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_adj_gather, dim3(a.n), dim3(256), (size_t)a.n * sizeof(float), s, a);
     return hipGetLastError();
 }
@@ -2096,8 +2205,9 @@ hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, i
             break;
         default:
             if (b.variant == 0 && b.adj) {
-                hipLaunchKernelGGL(kb_adj_scatter, dim3(b.total_ants), dim3(256), 0, s, b);
-                hipLaunchKernelGGL(kb_adj_gather, dim3(b.total_rows), dim3(256), (size_t)nmax * sizeof(float), s, b);
+                hipLaunchKernelGGL(kb_adj_scatter, dim3(b.nadj_items), dim3(256), (size_t)4 * nmax * b.adj_ants, s, b);
+                hipLaunchKernelGGL(kb_adj_gather, dim3(cdiv(b.total_rows, 4)), dim3(256),
+                                   4 * (size_t)nmax * sizeof(float), s, b, nmax);
             } else {
                 hipLaunchKernelGGL(kb_deposit, dim3(b.variant == 0 ? b.total_ants : b.count * (b.w + 1)), dim3(256),
                                    0, s, b);

@@ -81,7 +81,8 @@ typedef struct aco_params {
     uint64_t seed;          /* Philox key (reference: RANDOM_SEED, inc/Colony.h:15) */
     uint32_t stream_id;     /* Philox counter word 3 (instance id)            */
     int32_t construct_path; /* candidate mode: 0 auto, 1 global memory, 2 LDS-staged */
-    int32_t deposit_path;   /* Ant System: 0 auto, 1 fp32 atomics, 2 adjacency gather (n <= 32768) */
+    int32_t deposit_path;   /* Ant System: 0 auto, 1 fp32 atomics, 2 adjacency gather (n <= 32768;
+                               batch n <= 10240) */
     int32_t ants_per_wave;  /* batch construction: ants sharing one wave, 1, 2 (k <= 32, n <= 1024)
                                or 4 (k <= 16, n <= 512); 0 = the most that fit */
     int32_t reserved[4];
