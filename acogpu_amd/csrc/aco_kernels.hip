@@ -1801,29 +1801,29 @@ __device__ __forceinline__ float group_last(float p, int lane) {
 }
 
 template <int G>
-__device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int32_t ant0, const float* wl,
-                                                     const uint16_t* nl, uint32_t iter, int lane) {
+__device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int32_t ant0, const int2* cand,
+                                                     uint32_t iter, int lane) {
     constexpr int L = 64 / G;
-    constexpr uint32_t GMASK = L == 32 ? 0xffffffffu : ((1u << L) - 1u);
     const int g = lane / L;
     const int i = lane & (L - 1);
     const int gb = g * L;
     const int32_t ant = ant0 + g;
     const bool valid = ant < a.m;
-    const uint32_t n = (uint32_t)a.n;
 
-    // The words of the current 128-step half chunk in step order, consumed
-    // from q[0]: each half recomputes the chunk's blocks and keeps two of
-    // their four words, which halves the registers the queue holds.
-    uint32_t q[2 * G];
+    // The uniforms of the current 128-step half chunk in step order,
+    // consumed from q[0]: each half recomputes the chunk's blocks and keeps
+    // two of their four words, which halves the registers the queue holds.
+    float q[2 * G];
+    uint32_t first = 0;  // step 0's word (lane i = 0 of the first block)
     auto refill = [&](int32_t s) {
         const uint32_t blk0 = ((uint32_t)s >> 8) << 6;
         const bool hi = (s & 128) != 0;
 #pragma unroll
         for (int b = 0; b < G; ++b) {
             const Philox4 r = philox4x32_10(blk0 + b * L + i, (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
-            q[b] = hi ? r.x2 : r.x0;
-            q[G + b] = hi ? r.x3 : r.x1;
+            if (b == 0) first = r.x0;
+            q[b] = word_to_uniform(hi ? r.x2 : r.x0);
+            q[G + b] = word_to_uniform(hi ? r.x3 : r.x1);
         }
     };
     auto advance = [&]() {
@@ -1831,15 +1831,13 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
         for (int t = 0; t + 1 < 2 * G; ++t) q[t] = q[t + 1];
     };
     refill(0);
-    const int32_t start = word_to_city((uint32_t)bperm_i(gb, (int32_t)q[0]), a.n);
+    const int32_t start = word_to_city((uint32_t)bperm_i(gb, (int32_t)first), a.n);
     uint32_t vreg = (i == (start >> 5)) ? (1u << (start & 31)) : 0u;
     // row stride per lane (0 for lanes >= k): a 24-bit multiply, n k < 2^24
     const uint32_t kk = i < a.k ? (uint32_t)a.k : 0u;
     const int32_t loff = i < a.k ? i : a.n * a.k;
     int32_t c = start;
-    int32_t e = (int32_t)__umul24((uint32_t)c, kk) + loff;
-    int32_t j = nl[e];
-    float cw = wl[e];
+    int2 cj = cand[(int32_t)__umul24((uint32_t)c, kk) + loff];
     int32_t tbuf = start;
     int32_t prev_last = 0;
     int64_t len = 0;
@@ -1853,26 +1851,36 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
             else
                 advance();
         }
-        const float u = word_to_uniform((uint32_t)bperm_i(gb + (s & (L - 1)), (int32_t)q[0]));
+        const int32_t j = cj.x;
+        const float cw = __int_as_float(cj.y);
+        const float u = __int_as_float(
+            __builtin_amdgcn_ds_bpermute((gb << 2) + ((s & (L - 1)) << 2), __float_as_int(q[0])));
         // j >> 3 addresses lane j >> 5 of the group (bpermute drops the two
-        // low address bits; gb * 4 and j >> 3 share no bits for n <= 32 L)
+        // low address bits; gb * 4 and j >> 3 share no bits for n <= 32 L);
+        // the bit field extract reads bit j mod 32
         const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((gb << 2) + (j >> 3), (int)vreg);
-        const bool vj = (word >> (j & 31)) & 1u;
+        const bool vj = __builtin_amdgcn_ubfe(word, (uint32_t)j, 1u) != 0u;
         const float w = vj ? 0.0f : cw;
         const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
         const float p = group_scan<G>(w);
         const float tot = group_last<G>(p, lane);
         const float rr = u * tot;
         const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
-        const uint32_t bits = (uint32_t)(hit >> gb) & GMASK;
-        int32_t next = bperm_i(gb + __ffs(bits) - 1, j);
-        const unsigned long long nohit = __builtin_amdgcn_ballot_w64(bits == 0u);
+        // first hit of the group; no hit gives L (another group's lane, the
+        // result is then replaced below)
+        int32_t f;
+        if constexpr (L == 16)
+            f = __builtin_ctz(((uint32_t)(hit >> gb) & 0xffffu) | 0x10000u);
+        else
+            f = __builtin_ctzll((hit >> gb) | (1ull << 32));
+        int32_t next = bperm_i(gb + f, j);
+        const unsigned long long nohit = __builtin_amdgcn_ballot_w64(f == L);
         if (__builtin_expect(nohit != 0ull, 0)) {
 #pragma unroll 1
             for (int gg = 0; gg < G; ++gg) {
                 if (!((nohit >> (gg * L)) & 1ull)) continue;
                 const float tg = readlane_f(tot, gg * L);
-                const uint32_t pg = (uint32_t)(pos >> (gg * L)) & GMASK;
+                const uint32_t pg = (uint32_t)((pos >> (gg * L)) & (L == 32 ? 0xffffffffull : 0xffffull));
                 int32_t ng;
                 if (tg > 0.0f && pg != 0u) {  // rounding: the group's last positive lane
                     ng = readlane_i(j, gg * L + 31 - __clz((int)pg));
@@ -1891,9 +1899,7 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
                 next = (g == gg) ? ng : next;
             }
         }
-        e = (int32_t)__umul24((uint32_t)next, kk) + loff;
-        const int32_t jn = nl[e];
-        const float cwn = wl[e];
+        cj = cand[(int32_t)__umul24((uint32_t)next, kk) + loff];
         vreg |= (i == (next >> 5)) ? (1u << (next & 31)) : 0u;
         tbuf = (i == (s & (L - 1))) ? next : tbuf;
         if (__builtin_expect((s & (L - 1)) == L - 1 || s == a.n - 1, 0)) {
@@ -1914,10 +1920,7 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
             prev_last = bperm_i(gb + L - 1, tbuf);
         }
         c = next;
-        j = jn;
-        cw = cwn;
     }
-    (void)n;
     len += pend;
     if (valid && i == 0) len += a.dist[(size_t)c * a.P + start];
 #pragma unroll
@@ -1928,6 +1931,15 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
     }
     if (valid && i == 0) a.lengths[ant] = len;
     finish_counters(a, fb, scanned, lane);
+}
+
+// Candidate table of the grouped construction: entry e = (city, weight bits)
+// of candidate e mod k of row e / k, then one (0, 0) entry that lanes >= k
+// read; one 8-byte LDS read per step.
+__device__ __forceinline__ void stage_cand(int2* cand, const uint16_t* __restrict__ nn16,
+                                           const float* __restrict__ w, int32_t nk) {
+    for (int32_t x = threadIdx.x; x < nk; x += blockDim.x) cand[x] = make_int2((int32_t)nn16[x], __float_as_int(w[x]));
+    if (threadIdx.x == 0) cand[nk] = make_int2(0, 0);
 }
 
 // Persistent batched construction with G ants per wave: work items of
@@ -1948,13 +1960,11 @@ __global__ __launch_bounds__(1024) void kb_construct_group(const BatchBuffers b)
         const InstMeta im = b.meta[it.x];
         const ConstructArgs a = batch_construct_args(b, it.x, im);
         const int32_t nk = im.n * b.k;
-        const StagedLayout L = staged_layout(lds, nk);
-        stage_u16(L.nl, a.nn16, nk);
-        stage_f32(L.wl, a.choice_nn, nk);
-        staged_zero_entry(L, nk);
+        int2* cand = reinterpret_cast<int2*>(lds);
+        stage_cand(cand, a.nn16, a.choice_nn, nk);
         __syncthreads();
         const int32_t ant0 = it.y + wave * G;
-        if (ant0 < im.m) construct_group_wave<G>(a, ant0, L.wl, L.nl, a.state->iter, lane);
+        if (ant0 < im.m) construct_group_wave<G>(a, ant0, cand, a.state->iter, lane);
         __syncthreads();
     }
 }
@@ -2143,8 +2153,9 @@ hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s) {
 
 size_t batch_construct_lds_bytes(int32_t nmax, int32_t k, int32_t ants_per_block, int32_t ants_per_wave) {
     // several ants per wave keep bitmaps in registers and tours in global
-    // memory: only the staged candidate data
-    return staged_lds_bytes(nmax, k, (nmax + 31) / 32, ants_per_wave > 1 ? 0 : ants_per_block);
+    // memory: only the (city, weight) candidate table
+    if (ants_per_wave > 1) return 8 * ((size_t)nmax * k + 1);
+    return staged_lds_bytes(nmax, k, (nmax + 31) / 32, ants_per_block);
 }
 
 int32_t batch_max_ants_per_wave(int32_t nmax, int32_t k) {
