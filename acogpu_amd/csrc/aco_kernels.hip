@@ -586,10 +586,11 @@ __device__ __forceinline__ int32_t fallback_argmax_vreg(const ConstructArgs& a, 
 
 // Argmax fallback with the ant's bitmap in registers (lane src + w holds word
 // w), spread over the whole wave: lane l reads cities CPL l .. CPL l + CPL - 1
-// (CPL / 4 16-byte loads; n <= 64 CPL), takes their bits from the owner lane
-// with one bpermute and keeps the largest weight, lowest index on ties,
-// through a pairwise tree (the left operand holds the lower indices).  Same
-// result as the sequential scan; call with every lane active.
+// (CPL / 4 16-byte loads; n <= 64 CPL) and takes their bits from the owner
+// lane with one bpermute.  Visited cities become -1, the lane's maximum
+// comes from fmaxf (maxNum: a NaN weight never wins, as in the sequential
+// scan), and the lowest index holding it is the lane's candidate: the same
+// result as the sequential scan.  Call with every lane active.
 template <int CPL>
 __device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a, uint32_t vreg, int src, int32_t c,
                                                           int lane, unsigned long long& scanned) {
@@ -604,7 +605,6 @@ __device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a
         if (a.n - j0 < CPL) free_bits &= (1u << (a.n - j0)) - 1u;
         scanned += __popc(free_bits);
         float v[CPL];
-        int32_t ix[CPL];
         const float4* src4 = reinterpret_cast<const float4*>(a.choice + (size_t)c * a.P + j0);
 #pragma unroll
         for (int q = 0; q < CPL / 4; ++q) {
@@ -615,22 +615,13 @@ __device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a
             v[4 * q + 3] = t.w;
         }
 #pragma unroll
-        for (int q = 0; q < CPL; ++q) {
-            const bool ok = ((free_bits >> q) & 1u) && v[q] > -1.0f;  // NaN is never taken
-            v[q] = ok ? v[q] : -1.0f;
-            ix[q] = ok ? j0 + q : 0x7fffffff;
-        }
+        for (int q = 0; q < CPL; ++q) v[q] = ((free_bits >> q) & 1u) ? v[q] : -1.0f;
+        float m = v[0];
 #pragma unroll
-        for (int w = 1; w < CPL; w *= 2) {
+        for (int q = 1; q < CPL; ++q) m = fmaxf(m, v[q]);
 #pragma unroll
-            for (int q = 0; q + w < CPL; q += 2 * w) {
-                const bool right = v[q + w] > v[q];
-                v[q] = right ? v[q + w] : v[q];
-                ix[q] = right ? ix[q + w] : ix[q];
-            }
-        }
-        best = v[0];
-        bj = ix[0];
+        for (int q = CPL - 1; q >= 0; --q) bj = (v[q] == m) ? j0 + q : bj;
+        best = m;
     }
     int32_t next = wave_argmax_index(best, bj);
     if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
@@ -1810,25 +1801,26 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
     const int32_t ant = ant0 + g;
     const bool valid = ant < a.m;
 
-    // The uniforms of the current 128-step half chunk in step order,
-    // consumed from q[0]: each half recomputes the chunk's blocks and keeps
-    // two of their four words, which halves the registers the queue holds.
-    float q[2 * G];
+    // The uniforms of the current 256-step chunk in step order, consumed
+    // from q[0] (4 G registers; the VGPR budget at this kernel's occupancy
+    // has room, and each Philox block is computed once).
+    float q[4 * G];
     uint32_t first = 0;  // step 0's word (lane i = 0 of the first block)
     auto refill = [&](int32_t s) {
         const uint32_t blk0 = ((uint32_t)s >> 8) << 6;
-        const bool hi = (s & 128) != 0;
 #pragma unroll
         for (int b = 0; b < G; ++b) {
             const Philox4 r = philox4x32_10(blk0 + b * L + i, (uint32_t)ant, iter, a.stream, a.seed_lo, a.seed_hi);
             if (b == 0) first = r.x0;
-            q[b] = word_to_uniform(hi ? r.x2 : r.x0);
-            q[G + b] = word_to_uniform(hi ? r.x3 : r.x1);
+            q[b] = word_to_uniform(r.x0);
+            q[G + b] = word_to_uniform(r.x1);
+            q[2 * G + b] = word_to_uniform(r.x2);
+            q[3 * G + b] = word_to_uniform(r.x3);
         }
     };
     auto advance = [&]() {
 #pragma unroll
-        for (int t = 0; t + 1 < 2 * G; ++t) q[t] = q[t + 1];
+        for (int t = 0; t + 1 < 4 * G; ++t) q[t] = q[t + 1];
     };
     refill(0);
     const int32_t start = word_to_city((uint32_t)bperm_i(gb, (int32_t)first), a.n);
@@ -1846,7 +1838,7 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
     int32_t* tour = a.tours + (size_t)ant * a.n;
     for (int32_t s = 1; s < a.n; ++s) {
         if (__builtin_expect((s & (L - 1)) == 0, 0)) {
-            if ((s & 127) == 0)
+            if ((s & 255) == 0)
                 refill(s);
             else
                 advance();
