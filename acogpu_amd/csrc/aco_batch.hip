@@ -79,18 +79,37 @@ static void bfree(aco_batch* b) {
 extern "C" const char* aco_batch_last_error(const aco_batch* b) { return b ? b->err : g_batch_err; }
 
 // Work items (instance, first ant) for W ants per block, longest instances first.
-// Work items of W waves x G ants (one instance each), longest instances
-// first, and the persistent grid that fills the device with them.
+// Work items of W waves x G ants (one instance each) in QUEUE_BINS bins,
+// one per XCD: instances by decreasing work (ant slots x steps) go to the
+// least loaded bin, and each instance's items stay together in its bin.
+// Also sizes the persistent grid that fills the device with blocks.
 static int build_items(aco_batch* b, int32_t W, int32_t G) {
     if (G == 1 && b->nmax > 2048 && W > 8)
         return bfail(b, ACO_E_ARG, "more than 8 ants per block need n <= 2048 (largest n=%d)", b->nmax);
+    const int32_t per = W * G;
+    std::vector<int64_t> work(b->count);
+    for (int32_t i = 0; i < b->count; ++i) work[i] = (int64_t)((b->m[i] + per - 1) / per) * per * b->n[i];
     std::vector<int32_t> order(b->count);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return b->n[x] > b->n[y]; });
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return work[x] > work[y]; });
+    std::vector<std::vector<int32_t>> bins(aco::QUEUE_BINS);
+    std::vector<int64_t> load(aco::QUEUE_BINS, 0);
+    for (int32_t i : order) {
+        const int32_t h = (int32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        bins[h].push_back(i);
+        load[h] += work[i];
+    }
     std::vector<int2> it;
-    for (int32_t i : order)
-        for (int32_t a0 = 0; a0 < b->m[i]; a0 += W * G) it.push_back(make_int2(i, a0));
+    std::vector<int32_t> off(aco::QUEUE_BINS + 1, 0);
+    for (int32_t h = 0; h < aco::QUEUE_BINS; ++h) {
+        off[h] = (int32_t)it.size();
+        for (int32_t i : bins[h])
+            for (int32_t a0 = 0; a0 < b->m[i]; a0 += per) it.push_back(make_int2(i, a0));
+    }
+    off[aco::QUEUE_BINS] = (int32_t)it.size();
     BHIP(b, hipMemcpy(b->items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(const_cast<int32_t*>(b->buf.bin_off), off.data(), sizeof(int32_t) * off.size(),
+                      hipMemcpyHostToDevice));
     b->buf.nitems = (int32_t)it.size();
     b->buf.ants_per_block = W;
     b->buf.ants_per_wave = G;
@@ -269,7 +288,10 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     }
     TRYB(balloc(b, &B.state, (size_t)count));
     TRYB(balloc(b, &b->items, (size_t)ants));
-    TRYB(balloc(b, &B.queue, 1));
+    TRYB(balloc(b, &B.queue, (size_t)aco::QUEUE_BINS));
+    int32_t* d_bin_off = nullptr;
+    TRYB(balloc(b, &d_bin_off, (size_t)aco::QUEUE_BINS + 1));
+    B.bin_off = d_bin_off;
     TRYB(balloc(b, &B.cnn, (size_t)count));
     TRYB(balloc(b, &B.tau0, (size_t)count));
     float* d_eta = nullptr;
@@ -349,7 +371,7 @@ extern "C" int aco_batch_initialize(aco_batch* b) {
         s.stop_at = -1;
     }
     BHIP(b, hipMemcpyAsync(b->buf.state, st.data(), sizeof(DevState) * b->count, hipMemcpyHostToDevice, b->stream));
-    BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t), b->stream));
+    BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t) * aco::QUEUE_BINS, b->stream));
     BHIP(b, hipMemsetAsync(b->buf.best_tours, 0, sizeof(int32_t) * b->total_rows, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     if (b->graph) {
@@ -444,7 +466,7 @@ extern "C" int aco_batch_autotune(aco_batch* b, int32_t reps, int32_t* chosen) {
             B.ants_per_wave = G;
             float total = 0.0f;
             for (int32_t r = 0; r < reps; ++r) {
-                BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t), b->stream));
+                BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t) * aco::QUEUE_BINS, b->stream));
                 BHIP(b, hipEventRecord(b->ev[0], b->stream));
                 BHIP(b, aco::launch_batch_iteration_kernel(B, 1, b->nmax, b->grid, b->stream));
                 BHIP(b, hipEventRecord(b->ev[1], b->stream));
@@ -462,7 +484,7 @@ extern "C" int aco_batch_autotune(aco_batch* b, int32_t reps, int32_t* chosen) {
     }
     int rc = build_items(b, best_w, best_g);
     if (rc) return rc;
-    BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t), b->stream));
+    BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t) * aco::QUEUE_BINS, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     if (chosen) *chosen = best_w;
     return ACO_OK;

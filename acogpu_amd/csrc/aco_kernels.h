@@ -161,7 +161,8 @@ struct BatchBuffers {
     DevState* state;         // one per instance
     const int2* items;       // construction work items (instance, first ant), longest first
     int32_t nitems;
-    uint32_t* queue;         // work-queue counter, reset by the statistics kernel
+    uint32_t* queue;         // QUEUE_BINS work-queue counters, reset by the statistics kernel
+    const int32_t* bin_off;  // items of bin h: [bin_off[h], bin_off[h + 1])
     int64_t* cnn;            // per-instance nearest-neighbour tour length
     float* tau0;             // per-instance initial pheromone
     float alpha, one_minus_rho;
@@ -170,6 +171,9 @@ struct BatchBuffers {
     float tau0_const, rho;
     uint32_t seed_lo, seed_hi;
 };
+
+// Construction work-queue bins, one per XCD of the MI355X.
+constexpr int32_t QUEUE_BINS = 8;
 
 hipError_t launch_batch_init(const BatchBuffers& b, int32_t pmax, int32_t nmax, hipStream_t s);
 hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, int32_t nmax, int32_t grid,

@@ -1691,19 +1691,38 @@ __global__ __launch_bounds__(256) void kb_sweep(const BatchBuffers b) {
     sweep_row(batch_sweep_args(b, im), r - im.city_off, threadIdx.x & 63);
 }
 
-// One persistent launch: blocks take work items (instance, W ants) from an
-// atomic queue, longest instances first, stage the instance's candidate lists
-// and weights in LDS and construct W tours (one per wave).
+// The persistent construction's work queue: QUEUE_BINS bins of items, each
+// bin holding whole instances (longest first, bins balanced by work).
+// Blocks are dispatched round-robin over the XCDs, so block b takes items
+// from bin b mod 8 first: an instance's candidate data, choice rows and
+// distances are then read through one XCD's L2.  A block whose bin is empty
+// moves on to the next bins.  Thread 0 only; `bin` starts at the block's
+// home bin and only grows.  Returns the item index, or -1 once every bin is
+// empty.
+__device__ __forceinline__ int32_t batch_next_item(const BatchBuffers& b, int32_t& bin) {
+    const int32_t home = blockIdx.x % QUEUE_BINS;
+    for (; bin < home + QUEUE_BINS; ++bin) {
+        const int32_t h = bin % QUEUE_BINS;
+        const int32_t k = (int32_t)atomicAdd(&b.queue[h], 1u);
+        if (k < b.bin_off[h + 1] - b.bin_off[h]) return b.bin_off[h] + k;
+    }
+    return -1;
+}
+
+// One persistent launch: blocks take work items (instance, W ants) from the
+// binned queue, stage the instance's candidate lists and weights in LDS and
+// construct W tours (one per wave).
 template <bool VREG_VIS>
 __device__ __forceinline__ void kb_construct_body(const BatchBuffers& b, uint32_t* lds, int32_t& s_item) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    int32_t bin = blockIdx.x % QUEUE_BINS;
     for (;;) {
-        if (threadIdx.x == 0) s_item = (int32_t)atomicAdd(b.queue, 1u);
+        if (threadIdx.x == 0) s_item = batch_next_item(b, bin);
         __syncthreads();
         const int32_t item = s_item;
         __syncthreads();
-        if (item >= b.nitems) return;
+        if (item < 0) return;
         const int2 it = b.items[item];
         const InstMeta im = b.meta[it.x];
         const ConstructArgs a = batch_construct_args(b, it.x, im);
@@ -1942,12 +1961,13 @@ __global__ __launch_bounds__(1024) void kb_construct_group(const BatchBuffers b)
     __shared__ int32_t s_item;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    int32_t bin = blockIdx.x % QUEUE_BINS;
     for (;;) {
-        if (threadIdx.x == 0) s_item = (int32_t)atomicAdd(b.queue, 1u);
+        if (threadIdx.x == 0) s_item = batch_next_item(b, bin);
         __syncthreads();
         const int32_t item = s_item;
         __syncthreads();
-        if (item >= b.nitems) return;
+        if (item < 0) return;
         const int2 it = b.items[item];
         const InstMeta im = b.meta[it.x];
         const ConstructArgs a = batch_construct_args(b, it.x, im);
@@ -1961,7 +1981,7 @@ __global__ __launch_bounds__(1024) void kb_construct_group(const BatchBuffers b)
     }
 }
 
-// One block per instance; block 0 also re-arms the construction queue.
+// One block per instance; block 0 also re-arms the construction queue bins.
 __global__ __launch_bounds__(256) void kb_stats(const BatchBuffers b) {
     __shared__ LenKey red[16];
     __shared__ int32_t redi[16];
@@ -1988,7 +2008,7 @@ __global__ __launch_bounds__(256) void kb_stats(const BatchBuffers b) {
     a.w = b.w < im.n ? b.w : im.n;
     a.aging = 0;
     stats_block(a, red, redi, taken, s_improved, s_recruit, red_rank, sh);
-    if (inst == 0 && threadIdx.x == 0) *b.queue = 0u;
+    if (inst == 0 && threadIdx.x < QUEUE_BINS) b.queue[threadIdx.x] = 0u;
 }
 
 // Island exchange (batched copies of one instance): every island whose
