@@ -110,7 +110,8 @@ def bench_batched(args):
     b.initialize()
     w = b.autotune(reps=2)
     b.initialize()
-    log(f"bench: autotuned ants per block {w}, config {b.config()}")
+    cfg = b.config()
+    log(f"bench: autotuned ants per block {w}, config {cfg}")
     b.run(args.warmup)
     b.initialize()
     t0 = time.perf_counter()
@@ -150,7 +151,7 @@ def bench_batched(args):
         "n_gpus": 1, "steps": iters, "warmup": args.warmup, "ms_per_step": wall * 1e3 / iters,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{count} instances, n log-uniform [50,500], m=n, k=16, AS, beta 2, rho 0.5",
-                   "iterations": iters, "ants_per_block": w},
+                   "iterations": iters, "ants_per_block": w, "ants_per_wave": cfg["ants_per_wave"]},
         "sequential": {"value": seq_rate, "unit": "instances/s", "sample": int(len(sample)),
                        "kind": "single-instance solver, one instance after another"},
         "speedup_vs_sequential": batch_rate / seq_rate,
