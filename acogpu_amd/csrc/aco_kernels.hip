@@ -319,20 +319,6 @@ __device__ __forceinline__ uint32_t rng_word(const AntRng& r, int32_t s) {
     return (uint32_t)readlane_i((int)r.cur, s & 63);
 }
 
-// Roulette over one wave of candidate weights: inclusive DPP scan, r = u *
-// total, first lane with prefix > r and positive weight (else the last
-// positive lane).  Returns the lane, or -1 when the total is not positive.
-__device__ __forceinline__ int select_lane(float w, float u) {
-    const float p = wave_inclusive_scan(w);
-    const float total = readlane_f(p, 63);
-    if (!(total > 0.0f)) return -1;
-    const float rr = u * total;
-    const unsigned long long hit = __ballot(p > rr && w > 0.0f);
-    if (hit) return __ffsll((long long)hit) - 1;
-    const unsigned long long pos = __ballot(w > 0.0f);
-    return 63 - __clzll((long long)pos);
-}
-
 // choose_best_next: argmax of the selection weight over every unvisited city,
 // ties to the lowest index.  Lanes stride the row four loads at a time; exact
 // comparisons make the result independent of the reduction order.
@@ -417,15 +403,115 @@ __device__ __forceinline__ bool ant_is_young(const ConstructArgs& a, int32_t ant
     return a.aging && converged && a.valid[ant] && a.age[ant] < a.min_age;
 }
 
+// Wave-64 prefix sum restricted to 32 lanes: the first five steps of
+// wave_inclusive_scan.  When lanes 32..63 hold zeros it yields the same bits
+// in lanes 0..31 as the full scan, and lane 31 equals the full scan's lane 63.
+__device__ __forceinline__ float wave_inclusive_scan32(float v) {
+    v = v + dpp_src<0x111, 0xf>(v);
+    v = v + dpp_src<0x112, 0xf>(v);
+    v = v + dpp_src<0x114, 0xf>(v);
+    v = v + dpp_src<0x118, 0xf>(v);
+    v = v + dpp_src<0x142, 0xa>(v);
+    return v;
+}
+
+// Exact tour length of one ant whose tour the wave has just stored: lane l
+// sums the edges i -> i+1 for i = l, l + 64, ...; sixteen distance loads per
+// lane are in flight at a time.
+__device__ __forceinline__ void store_tour_length(const ConstructArgs& a, const int32_t* tour, int32_t ant,
+                                                  int lane) {
+    __threadfence_block();  // the wave's tour stores before its lanes read them back
+    int64_t len = 0;
+    for (int32_t i0 = 0; i0 < a.n; i0 += 64 * 16) {
+        int32_t x[16], y[16], d[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int32_t i = i0 + 64 * q + lane;
+            x[q] = i < a.n ? tour[i] : 0;
+            y[q] = i < a.n ? tour[i + 1 == a.n ? 0 : i + 1] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int32_t i = i0 + 64 * q + lane;
+            d[q] = i < a.n ? a.dist[(size_t)x[q] * a.P + y[q]] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) len += d[q];
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)len, off);
+        const uint32_t hi = __shfl_xor((uint32_t)((uint64_t)len >> 32), off);
+        len += (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    if (lane == 0) a.lengths[ant] = len;
+}
+
 // ---------------------------------------------------------------------------
 // Tour construction, candidate-list mode, global-memory variant (any n).
 // One ant per wave64 for all n-1 steps (the reference launches 3(n-1)
 // kernels per iteration, src/Colony.cu:713-755).  Per step: lane r < k holds
 // candidate r of the current city and its weight (0 if visited, per the LDS
 // bitmap); the DPP scan gives the roulette prefix and a ballot the pick; a
-// zero total falls back to the argmax over the unvisited cities.  The tour
-// length is accumulated exactly in int64, one step behind (the distance load
-// completes under the next step's candidate loads).
+// zero total falls back to the argmax over the unvisited cities.  The next
+// city's candidate row is requested as soon as the pick is known, so its
+// memory latency overlaps the step's bookkeeping, and no other vector load
+// sits in the per-step chain: vector memory returns in issue order, so a
+// distance load per step would make every candidate load wait for it.  The
+// tour length is summed after the last step from the stored tour.
+// K32: k <= 32, the roulette scan stops at 32 lanes.
+template <bool K32>
+__device__ __forceinline__ void construct_cand_ant(const ConstructArgs& a, int32_t ant, bool young, float ratio,
+                                                   uint32_t* vis, uint32_t iter, int lane) {
+    for (int32_t w = lane; w < a.words; w += 64) vis[w] = 0u;
+    AntRng rng;
+    rng_refill(rng, 0, (uint32_t)ant, iter, a, lane);
+    const int32_t start = word_to_city(rng_word(rng, 0), a.n);
+    mark_visited(vis, start, lane);
+    const bool in = lane < a.k;
+    // lanes >= k keep candidate 0 with weight 0
+    auto load_row = [&](int32_t city, int32_t& j, float& cw) {
+        j = 0;
+        cw = 0.0f;
+        if (in) {
+            const size_t e = (size_t)city * a.k + lane;
+            j = a.nn16 != nullptr ? (int32_t)a.nn16[e] : a.nn[e];
+            cw = young ? young_formula(a.tau_nn[e], a.mean[city], ratio, a.eta_nn[e], a.alpha) : a.choice_nn[e];
+        }
+    };
+    int32_t* tour = a.tours + (size_t)ant * a.n;
+    int32_t tbuf = start;
+    int32_t c = start;
+    int32_t j;
+    float cw;
+    load_row(start, j, cw);
+    unsigned long long fb = 0, scanned = 0;
+    for (int32_t s = 1; s < a.n; ++s) {
+        rng_step(rng, s, (uint32_t)ant, iter, a, lane);
+        const float u = word_to_uniform(rng_word(rng, s));
+        const float w = is_visited(vis, j) ? 0.0f : cw;
+        const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
+        const float p = K32 ? wave_inclusive_scan32(w) : wave_inclusive_scan(w);
+        const float total = readlane_f(p, K32 ? 31 : 63);
+        const float rr = u * total;
+        const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
+        int32_t next;
+        if (__builtin_expect(hit != 0ull, 1)) {
+            next = readlane_i(j, __ffsll((long long)hit) - 1);
+        } else if (total > 0.0f) {
+            next = readlane_i(j, 63 - __clzll((long long)pos));
+        } else {
+            ++fb;
+            next = fallback_argmax(a, vis, c, young, ratio, lane, scanned);
+        }
+        load_row(next, j, cw);
+        mark_visited(vis, next, lane);
+        stage_city(tour, tbuf, s, next, a.n, lane);
+        c = next;
+    }
+    finish_counters(a, fb, scanned, lane);
+    store_tour_length(a, tour, ant, lane);
+}
+
 __global__ __launch_bounds__(512) void k_construct_cand(const ConstructArgs a) {
     extern __shared__ __align__(16) uint32_t lds[];
     if (a.state->skip) return;
@@ -438,51 +524,10 @@ __global__ __launch_bounds__(512) void k_construct_cand(const ConstructArgs a) {
     const uint32_t iter = a.state->iter;
     const bool young = ant_is_young(a, ant, a.state->converged);
     const float ratio = young ? (float)a.age[ant] / (float)a.min_age : 0.0f;
-
-    for (int32_t w = lane; w < a.words; w += 64) vis[w] = 0u;
-    AntRng rng;
-    rng_refill(rng, 0, (uint32_t)ant, iter, a, lane);
-    const int32_t start = word_to_city(rng_word(rng, 0), a.n);
-    mark_visited(vis, start, lane);
-
-    int32_t* tour = a.tours + (size_t)ant * a.n;
-    int32_t tbuf = start;
-    int64_t len = 0;
-    int32_t pend = 0;
-    int32_t c = start;
-    unsigned long long fb = 0, scanned = 0;
-    for (int32_t s = 1; s < a.n; ++s) {
-        rng_step(rng, s, (uint32_t)ant, iter, a, lane);
-        const float u = word_to_uniform(rng_word(rng, s));
-        int32_t j = 0;
-        float w = 0.0f;
-        if (lane < a.k) {
-            const size_t e = (size_t)c * a.k + lane;
-            j = a.nn16 != nullptr ? (int32_t)a.nn16[e] : a.nn[e];
-            const float cw = young ? young_formula(a.tau_nn[e], a.mean[c], ratio, a.eta_nn[e], a.alpha)
-                                   : a.choice_nn[e];
-            w = is_visited(vis, j) ? 0.0f : cw;
-        }
-        len += pend;
-        const int sel = select_lane(w, u);
-        int32_t next;
-        if (sel >= 0) {
-            next = readlane_i(j, sel);
-        } else {
-            ++fb;
-            next = fallback_argmax(a, vis, c, young, ratio, lane, scanned);
-        }
-        mark_visited(vis, next, lane);
-        pend = a.dist[(size_t)c * a.P + next];
-        stage_city(tour, tbuf, s, next, a.n, lane);
-        c = next;
-    }
-    len += pend;
-    finish_counters(a, fb, scanned, lane);
-    if (lane == 0) {
-        len += a.dist[(size_t)c * a.P + start];
-        a.lengths[ant] = len;
-    }
+    if (a.k <= 32)
+        construct_cand_ant<true>(a, ant, young, ratio, vis, iter, lane);
+    else
+        construct_cand_ant<false>(a, ant, young, ratio, vis, iter, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -494,17 +539,6 @@ __global__ __launch_bounds__(512) void k_construct_cand(const ConstructArgs a) {
 // stage that ant's pulled-toward-the-mean weights instead (they differ per
 // age), so young ants also run at LDS speed.  Tours are kept in LDS (16-bit)
 // and written, with their exact lengths, after the last step.
-// Wave-64 prefix sum restricted to 32 lanes: the first five steps of
-// wave_inclusive_scan.  When lanes 32..63 hold zeros it yields the same bits
-// in lanes 0..31 as the full scan, and lane 31 equals the full scan's lane 63.
-__device__ __forceinline__ float wave_inclusive_scan32(float v) {
-    v = v + dpp_src<0x111, 0xf>(v);
-    v = v + dpp_src<0x112, 0xf>(v);
-    v = v + dpp_src<0x114, 0xf>(v);
-    v = v + dpp_src<0x118, 0xf>(v);
-    v = v + dpp_src<0x142, 0xa>(v);
-    return v;
-}
 
 // Argmax fallback for the register bitmap layout: lane l owns cities
 // 32l .. 32l+31 (its bitmap word), reads them as eight 16-byte loads and

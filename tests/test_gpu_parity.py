@@ -96,6 +96,60 @@ def test_as_iteration_replay_fallback_heavy(path):
         assert steps > 0
 
 
+@pytest.mark.parametrize("n,m,k,path,seed", [
+    (5000, 24, 32, 1, 6),   # global-memory candidate rows, 32-lane scan, LDS bitmap
+    (2500, 16, 40, 1, 8),   # global-memory candidate rows, 64-lane scan
+    (3000, 16, 8, 2, 7),    # LDS-staged candidate rows with the LDS bitmap (n > 2048), fallback heavy
+])
+def test_as_iteration_replay_large_n(n, m, k, path, seed):
+    say(f"case: AS large n={n} m={m} k={k} path={path} seed={seed}")
+    rng = np.random.default_rng(seed)
+    xy = instances.uniform(n, 100000, seed=seed)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=k, beta=2.0, rho=0.2, seed=500 + seed, construct_path=path)
+    with make(p, xy=xy) as col:
+        assert col.config()["path"] == (0 if path == 1 else 2)
+        col.set_pheromone(random_tau(rng, n))
+        for it in range(2):
+            snap = snapshot(col)
+            col.iterate()
+            ref = replay(p, dist, snap)
+            check_iteration(col, p, dist, snap, ref)
+        assert ref["fallback"][0] > 0
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("n,k,m", [(2, 1, 1), (3, 2, 3), (5, 4, 7), (33, 32, 5), (65, 64, 4)])
+def test_tiny_instances_replay(n, k, m, path):
+    """Smallest instances: one candidate, n below the wave width, k = n - 1."""
+    say(f"case: tiny AS n={n} k={k} m={m} path={path}")
+    rng = np.random.default_rng(n)
+    xy = instances.uniform(n, 1000, seed=60 + n)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=k, beta=2.0, rho=0.5, seed=n, construct_path=path)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, n))
+        for it in range(3):
+            snap = snapshot(col)
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+
+
+@pytest.mark.parametrize("n,m", [(2, 1), (3, 4), (64, 3)])
+def test_tiny_fullrow_replay(n, m):
+    say(f"case: tiny full-row AS n={n} m={m}")
+    rng = np.random.default_rng(n)
+    xy = instances.uniform(n, 1000, seed=80 + n)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, mode=capi.MODE_FULLROW, beta=2.0, rho=0.5, seed=n)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, n))
+        for it in range(2):
+            snap = snapshot(col)
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_rbas_aging_replay_c2_shape(path):
     say(f"case: RBAS+aging clustered n=500 m=160 k=20 w=6 beta=3 rho=0.1 (C2 shape, small) path={path}")
