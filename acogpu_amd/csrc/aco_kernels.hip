@@ -687,7 +687,8 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
         mark_visited(vis_lds, start, lane);
     }
     // lanes >= k read the zero weight staged after the n*k candidate entries
-    const int32_t lmul = lane < a.k ? 1 : 0;
+    // row stride per lane (0 for lanes >= k): a 24-bit multiply, n k < 2^24
+    const uint32_t kk = lane < a.k ? (uint32_t)a.k : 0u;
     const int32_t loff = lane < a.k ? lane : a.n * a.k;
     int32_t tbuf = start;
     int32_t c = start;
@@ -695,22 +696,28 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
     // the candidate row of the current city is loaded one step ahead: the
     // LDS reads for step s+1 are issued as soon as step s has chosen, before
     // its bookkeeping
-    int32_t e = c * a.k * lmul + loff;
-    int32_t j = nl[e];
-    float cw = wl[e];
+    // per-lane row bases; an entry address is one 24-bit multiply-add
+    const char* nlb = reinterpret_cast<const char*>(nl + loff);
+    const char* wlb = reinterpret_cast<const char*>(wl + loff);
+    const uint32_t kk2 = 2u * kk, kk4 = 4u * kk;
+    int32_t j = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)c, kk2));
+    float cw = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)c, kk4));
     for (int32_t s = 1; s < a.n; ++s) {
         rng_step(rng, s, (uint32_t)ant, iter, a, lane);
         const float u = word_to_uniform(rng_word(rng, s));
-        bool vj;
+        __builtin_assume((uint32_t)j < 65536u);  // a 16-bit LDS entry
+        float w;
         if (VREG_VIS) {
             // ds_bpermute takes a byte address and drops its two low bits:
-            // j >> 3 addresses lane j >> 5, the owner of j's bitmap word
-            const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute(j >> 3, (int)vreg);
-            vj = (word >> (j & 31)) & 1u;
+            // j >> 3 addresses lane j >> 5, the owner of j's bitmap word; the
+            // signed bit field extract of bit j mod 32 gives an all-ones mask
+            // for a visited city, which clears the weight's bits
+            const int32_t word = __builtin_amdgcn_ds_bpermute(j >> 3, (int)vreg);
+            const int32_t vmask = __builtin_amdgcn_sbfe(word, (uint32_t)j, 1u);
+            w = __int_as_float(__float_as_int(cw) & ~vmask);
         } else {
-            vj = is_visited(vis_lds, j);
+            w = is_visited(vis_lds, j) ? 0.0f : cw;
         }
-        const float w = vj ? 0.0f : cw;
         // ballots straight from the compare masks (no round trip through a
         // VGPR); the common case falls through, the rare ones are laid apart
         const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
@@ -730,9 +737,8 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             else
                 next = fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned);
         }
-        e = next * a.k * lmul + loff;
-        const int32_t jn = nl[e];
-        const float cwn = wl[e];
+        const int32_t jn = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)next, kk2));
+        const float cwn = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)next, kk4));
         if (VREG_VIS) {
             vreg |= (lane == (next >> 5)) ? (1u << (next & 31)) : 0u;
         } else {
