@@ -332,9 +332,10 @@ class Colony:
                 "max_lds": int(t[3])}
 
     def counters(self):
-        t = np.zeros(2, dtype=np.uint64)
+        """(fallback steps, unvisited cities scanned by fallback steps, tours constructed)"""
+        t = np.zeros(3, dtype=np.uint64)
         self._chk(self._L.aco_get_counters(self._ctx, _p(t)))
-        return int(t[0]), int(t[1])
+        return int(t[0]), int(t[1]), int(t[2])
 
 
 class Batch:

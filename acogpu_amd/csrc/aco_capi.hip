@@ -858,6 +858,7 @@ extern "C" int aco_get_counters(aco_ctx* c, uint64_t* counters) {
     if (rc) return rc;
     counters[0] = st.fb_steps;
     counters[1] = st.fb_scanned;
+    counters[2] = st.tours;
     return ACO_OK;
 }
 

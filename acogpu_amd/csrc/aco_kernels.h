@@ -24,6 +24,7 @@ struct DevState {
     int64_t stop_at;         // halt after the first iteration whose best <= stop_at (-1: never)
     int32_t halted;          // set by the statistics kernel; cleared by the host
     int32_t skip;            // the current iteration is skipped (the sweep copies halted)
+    unsigned long long tours;       // tours constructed (valid ants of every iteration run)
 };
 
 struct SweepArgs {

@@ -1406,6 +1406,9 @@ __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
     __shared__ LenKey red_rank[16 * RANK_MAX];
     __shared__ StatsShared sh;
     if (a.state->skip) return;
+    // the ants that built a tour this iteration: the valid ones (validity
+    // changes only below, after the deposit list is made)
+    if (threadIdx.x == 0) a.state->tours += a.aging ? (unsigned long long)a.state->num_valid : (unsigned long long)a.m;
     stats_block(a, red, redi, taken, s_improved, s_recruit, red_rank, sh);
 }
 

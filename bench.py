@@ -5,9 +5,11 @@ Workload (default): rank-based Ant System with ant aging, n = 1000 cities
 from 10 Gaussian clusters (centres uniform in [0,10000]^2, sigma 400, clipped,
 rounded), EUC_2D integer distances, m = 1000 ants, candidate lists k = 20,
 w = 6 ranked ants, alpha 1, beta 3, rho 0.1, fp32 pheromone / choice_info.
-One step = one ACO iteration = m tours (sweep, construction, statistics,
-deposit), all on the device; the timed loop replays a hipGraph of the
-iteration with the inputs resident in HBM.
+One step = one ACO iteration (sweep, construction, statistics, deposit), all
+on the device; the timed loop replays a hipGraph of the iteration with the
+inputs resident in HBM.  tours/s counts the tours actually constructed: every
+valid ant builds one per iteration, and aging retires ants (about 840 of the
+1000 stay valid once the colony has converged).
 
 Prints one JSON line.  `--gpus N` with N > 1 is refused: this build measures
 one GPU.
@@ -414,11 +416,14 @@ def main():
     log(f"bench: warmup {args.warmup} iterations")
     col.run(args.warmup)
     log(f"bench: timed {args.steps} iterations (hipGraph replay)")
+    tours0 = col.counters()[2]
     t0 = time.perf_counter()
     ev_ms = col.run(args.steps)
     wall = time.perf_counter() - t0
     ib, gb, nv = col.stats()
-    tours_per_s = m * args.steps / wall
+    # tours actually constructed: with aging, retired (invalid) ants build none
+    tours = col.counters()[2] - tours0
+    tours_per_s = tours / wall
     log(f"bench: {wall * 1e3 / args.steps:.3f} ms/iteration (events {ev_ms / args.steps:.3f}), "
         f"{tours_per_s:.4g} tours/s, best {gb}")
 
@@ -450,11 +455,13 @@ def main():
         colf = capi.Colony(pf, xy=xy)
         colf.initialize()
         colf.run(2)
+        tf0 = colf.counters()[2]
         t0 = time.perf_counter()
         colf.run(args.fullrow_iters)
         wall_f = time.perf_counter() - t0
+        tours_f = colf.counters()[2] - tf0
         colf.close()
-        fullrow = {"value": m * args.fullrow_iters / wall_f, "unit": "tours/s", "iterations": args.fullrow_iters}
+        fullrow = {"value": tours_f / wall_f, "unit": "tours/s", "iterations": args.fullrow_iters}
         log(f"bench: full-row GPU {fullrow['value']:.4g} tours/s")
 
     line = {
@@ -473,6 +480,7 @@ def main():
         "config": {"workload": "rank-based AS + aging, n=1000 clustered (10 Gaussian clusters, "
                                "sigma 400), m=1000, k=20, w=6, alpha 1, beta 3, rho 0.1",
                    "n": n, "m": m, "k": k, "iterations_timed": args.steps,
+                   "tours_per_iteration": tours / args.steps,
                    "construct_path": cfg["path"], "ants_per_block": cfg["ants_per_block"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,

@@ -168,7 +168,9 @@ int aco_set_heuristic_nn(aco_ctx *c, const float *h);
 int aco_get_heuristic_nn(aco_ctx *c, float *h);
 /* best-so-far length/tour overwrite (parity replays) */
 int aco_set_best(aco_ctx *c, int64_t glob_best, const int32_t *tour);
-/* counters = {fallback steps, unvisited cities scanned by fallback steps} since create */
+/* counters[3] = {fallback steps, unvisited cities scanned by fallback steps,
+ * tours constructed (the valid ants of every iteration run)} since
+ * aco_initialize */
 int aco_get_counters(aco_ctx *c, uint64_t *counters);
 
 /* Launch configuration chosen at create: config = {construction path

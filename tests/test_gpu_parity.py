@@ -91,7 +91,8 @@ def test_as_iteration_replay_fallback_heavy(path):
         col.iterate()
         ref = replay(p, dist, snap)
         check_iteration(col, p, dist, snap, ref)
-        steps, scanned = col.counters()
+        steps, scanned, tours = col.counters()
+        assert tours == p.m
         assert steps == ref["fallback"][0] and scanned == ref["fallback"][1]
         assert steps > 0
 
@@ -162,14 +163,17 @@ def test_rbas_aging_replay_c2_shape(path):
                             tau0_policy=capi.TAU0_RBAS, seed=5, construct_path=path)
     with make(p, xy=xy) as col:
         checked_young = False
+        expect_tours = 0
         for it in range(14):
             snap = snapshot(col)
+            expect_tours += snap["num_valid"]
             col.iterate()
             ref = replay(p, dist, snap)
             check_iteration(col, p, dist, snap, ref)
             if snap["converged"] and np.any((snap["age"] + 1 < p.min_age) & (snap["valid"] > 0)):
                 checked_young = True
         assert checked_young, "no young ant after convergence was exercised"
+        assert col.counters()[2] == expect_tours  # only valid ants construct
 
 
 @pytest.mark.parametrize("path", PATHS)
