@@ -185,8 +185,7 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
         if (p->mode == ACO_MODE_FULLROW) {
             c->path = 1;
             c->apb = p->ants_per_block ? p->ants_per_block : 1;
-            const int32_t nchunk = (c->n + 63) / 64;
-            const size_t per_ant = ((size_t)c->words + 2 * (size_t)nchunk * 64 + nchunk) * 4;
+            const size_t per_ant = aco::full_lds_words(c->n, c->words) * 4;
             while (c->apb > 1 && per_ant * c->apb > (size_t)max_lds) --c->apb;
             if (per_ant > (size_t)max_lds) {
                 int rc2 = fail(nullptr, ACO_E_RANGE, "full-row mode needs %zu bytes of LDS per ant (limit %d)", per_ant,

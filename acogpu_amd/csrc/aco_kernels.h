@@ -192,6 +192,17 @@ hipError_t launch_nn_lists(const int32_t* dist, int32_t n, int32_t P, int32_t k,
 hipError_t launch_fill_rows(float* a, int32_t n, int32_t P, float v, hipStream_t s);
 hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, hipStream_t s);
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t s);
+// Full-row construction: chunks of 64 candidates whose list entries and
+// weights are loaded together; LDS words per ant (visited bitmap, the weight
+// row in city order, then per candidate its weight, prefix and city, then
+// the running total before each chunk and after the last).
+constexpr int FULL_GROUP = 16;
+// word offset of the weight row (16-byte aligned: per-ant areas are multiples of 4 words)
+__host__ __device__ inline size_t full_row_offset(int32_t words) { return ((size_t)words + 3) & ~(size_t)3; }
+__host__ __device__ inline size_t full_lds_words(int32_t n, int32_t words) {
+    const size_t nchunk = ((size_t)n + 63) / 64;
+    return ((full_row_offset(words) + 4 * nchunk * 64 + nchunk + 1) + 3) & ~(size_t)3;
+}
 // Construction paths: 0 = candidate lists with global reads, 1 = full row,
 // 2 = candidate lists staged in LDS (n*k*6 bytes + per-ant state per block).
 size_t staged_lds_bytes(int32_t n, int32_t k, int32_t words, int32_t ants_per_block);

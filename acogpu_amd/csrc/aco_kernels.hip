@@ -951,6 +951,48 @@ __global__ __launch_bounds__(256) void k_nn_aux(const int32_t* __restrict__ nn, 
 // running chunk total T (sequential over chunks).  Fallback: the farthest
 // unvisited city (select_local_best inc/Colony.cuh:33-43).  LDS per ant:
 // bitmap, weights and prefixes of the whole row, chunk offsets.
+// Selection weights of row c in city order into rowbuf (LDS), 0 for visited
+// cities: eight 16-byte loads per lane in flight (2048 cities per pass).
+// Young aging ants get the pulled-toward-the-mean weights.
+__device__ __forceinline__ void stage_weight_row(const ConstructArgs& a, int32_t c, const AntCtx& ac,
+                                                 const uint32_t* vis, float* rowbuf, int lane) {
+    const size_t row = (size_t)c * a.P;
+    for (int32_t b0 = 0; b0 < a.n; b0 += 2048) {
+        float4 x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int32_t j = b0 + 256 * q + 4 * lane;
+            x[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (j < a.n) {
+                if (!ac.young) {
+                    x[q] = *reinterpret_cast<const float4*>(a.choice + row + j);
+                } else {
+                    const float4 t = *reinterpret_cast<const float4*>(a.tau + row + j);
+                    const int4 d = *reinterpret_cast<const int4*>(a.dist + row + j);
+                    const float mc = a.mean[c];
+                    x[q].x = young_formula(t.x, mc, ac.ratio, eta_of(a.eta, a.eta_len, d.x), a.alpha);
+                    x[q].y = young_formula(t.y, mc, ac.ratio, eta_of(a.eta, a.eta_len, d.y), a.alpha);
+                    x[q].z = young_formula(t.z, mc, ac.ratio, eta_of(a.eta, a.eta_len, d.z), a.alpha);
+                    x[q].w = young_formula(t.w, mc, ac.ratio, eta_of(a.eta, a.eta_len, d.w), a.alpha);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int32_t j = b0 + 256 * q + 4 * lane;
+            if (j < a.n) {
+                const uint32_t word = vis[j >> 5] >> (j & 31);  // cities j .. j+3 share a word
+                float4 v = x[q];
+                v.x = (word & 1u) ? 0.0f : v.x;
+                v.y = (word & 2u) ? 0.0f : v.y;
+                v.z = (word & 4u) ? 0.0f : v.z;
+                v.w = (word & 8u) ? 0.0f : v.w;
+                *reinterpret_cast<float4*>(rowbuf + j) = v;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(512) void k_construct_full(const ConstructArgs a) {
     extern __shared__ __align__(16) uint32_t lds[];
     if (a.state->skip) return;
@@ -959,11 +1001,12 @@ __global__ __launch_bounds__(512) void k_construct_full(const ConstructArgs a) {
     const int32_t ant = blockIdx.x * (blockDim.x >> 6) + wave;
     if (ant >= a.m) return;
     const int32_t nchunk = (a.n + 63) / 64;
-    const size_t per_ant = (size_t)a.words + 2 * (size_t)nchunk * 64 + nchunk;
-    uint32_t* vis = lds + (size_t)wave * per_ant;
-    float* wbuf = reinterpret_cast<float*>(vis + a.words);
+    uint32_t* vis = lds + (size_t)wave * full_lds_words(a.n, a.words);
+    float* rowbuf = reinterpret_cast<float*>(vis + full_row_offset(a.words));
+    float* wbuf = rowbuf + (size_t)nchunk * 64;
     float* pbuf = wbuf + (size_t)nchunk * 64;
-    float* tp = pbuf + (size_t)nchunk * 64;
+    int32_t* jbuf = reinterpret_cast<int32_t*>(pbuf + (size_t)nchunk * 64);
+    float* tp = reinterpret_cast<float*>(jbuf + (size_t)nchunk * 64);
     const uint32_t iter = a.state->iter;
 
     if (a.aging && !a.valid[ant]) return;
@@ -978,37 +1021,66 @@ __global__ __launch_bounds__(512) void k_construct_full(const ConstructArgs a) {
 
     int32_t* tour = a.tours + (size_t)ant * a.n;
     int32_t tbuf = start;
-    int64_t len = 0;
     int32_t c = start;
     unsigned long long fb = 0;
     for (int32_t s = 1; s < a.n; ++s) {
         rng_step(rng, s, (uint32_t)ant, iter, a, lane);
         const float u = word_to_uniform(rng_word(rng, s));
         const int32_t* nrow = a.nn + (size_t)c * a.n;
-        const float* crow = a.choice + (size_t)c * a.P;
-        float T = 0.0f;
-        for (int32_t q = 0; q < nchunk; ++q) {
-            const int32_t r = q * 64 + lane;
-            float w = 0.0f;
-            if (r < a.n) {
-                const int32_t j = nrow[r];
-                if (!is_visited(vis, j)) w = ac.young ? young_weight(a, c, j, ac.ratio) : crow[j];
+        // FULL_GROUP chunks of 64 candidates at a time.  The list entries of
+        // the first group are requested together with the weight row, which
+        // is read in city order (16-byte loads), zeroed where visited and
+        // kept in LDS; the list order then gathers from LDS, so a step costs
+        // one memory round trip for n <= 64 FULL_GROUP.  The chunk scans run
+        // in chunk order; T accumulates the chunk totals sequentially.
+        int32_t jj[FULL_GROUP];
+        auto load_entries = [&](int32_t g0) {
+#pragma unroll
+            for (int i = 0; i < FULL_GROUP; ++i) {
+                const int32_t r = (g0 + i) * 64 + lane;
+                jj[i] = r < a.n ? nrow[r] : -1;
             }
-            const float p = wave_inclusive_scan(w);
-            wbuf[r] = w;
-            pbuf[r] = p;
-            if (lane == 0) tp[q] = T;
-            T = T + readlane_f(p, 63);
+        };
+        load_entries(0);
+        stage_weight_row(a, c, ac, vis, rowbuf, lane);
+        float T = 0.0f;
+        for (int32_t g0 = 0; g0 < nchunk; g0 += FULL_GROUP) {
+            if (g0 > 0) load_entries(g0);
+            float w[FULL_GROUP];
+#pragma unroll
+            for (int i = 0; i < FULL_GROUP; ++i) w[i] = jj[i] >= 0 ? rowbuf[jj[i]] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < FULL_GROUP; ++i) {
+                if (g0 + i < nchunk) {
+                    const int32_t r = (g0 + i) * 64 + lane;
+                    const float p = wave_inclusive_scan(w[i]);
+                    wbuf[r] = w[i];
+                    pbuf[r] = p;
+                    jbuf[r] = jj[i];
+                    if (lane == 0) tp[g0 + i] = T;
+                    T = T + readlane_f(p, 63);
+                }
+            }
         }
+        if (lane == 0) tp[nchunk] = T;
+        // The pick is the first (chunk, lane) with tp[q] + prefix > r and a
+        // positive weight.  tp[q + 1] is the same sum for the chunk's last
+        // lane and the sums grow along a chunk, so the first chunk whose
+        // tp[q + 1] exceeds r holds it (its first lane above r has a
+        // positive weight: the sum rose there); one ballot over the chunk
+        // ends, one over that chunk.
         int32_t next = -1;
         if (T > 0.0f) {
             const float rr = u * T;
-            for (int32_t q = 0; q < nchunk; ++q) {
-                const int32_t r = q * 64 + lane;
-                const float P = tp[q] + pbuf[r];
-                const unsigned long long hit = __ballot(P > rr && wbuf[r] > 0.0f);
-                if (hit) {
-                    next = nrow[q * 64 + __ffsll((long long)hit) - 1];
+            for (int32_t q0 = 0; q0 < nchunk; q0 += 64) {
+                const int32_t q = q0 + lane;
+                const unsigned long long qa = __ballot(q < nchunk && tp[q < nchunk ? q + 1 : 0] > rr);
+                if (qa) {
+                    const int32_t qs = q0 + __ffsll((long long)qa) - 1;
+                    const int32_t r = qs * 64 + lane;
+                    const float P = tp[qs] + pbuf[r];
+                    const unsigned long long hit = __ballot(P > rr && wbuf[r] > 0.0f);
+                    if (hit) next = jbuf[qs * 64 + __ffsll((long long)hit) - 1];
                     break;
                 }
             }
@@ -1017,25 +1089,21 @@ __global__ __launch_bounds__(512) void k_construct_full(const ConstructArgs a) {
             ++fb;
             for (int32_t q = nchunk - 1; q >= 0; --q) {
                 const int32_t r = q * 64 + lane;
-                const unsigned long long m = __ballot(r < a.n && !is_visited(vis, nrow[r < a.n ? r : 0]));
+                const unsigned long long m = __ballot(r < a.n && !is_visited(vis, jbuf[r < a.n ? r : 0]));
                 if (m) {
-                    next = nrow[q * 64 + 63 - __clzll((long long)m)];
+                    next = jbuf[q * 64 + 63 - __clzll((long long)m)];
                     break;
                 }
             }
         }
         next = uniform_i(next);
         mark_visited(vis, next, lane);
-        if (lane == 0) len += a.dist[(size_t)c * a.P + next];
         stage_city(tour, tbuf, s, next, a.n, lane);
         c = next;
     }
-    if (lane == 0) {
-        if (a.n == 1) tour[0] = start;
-        len += a.dist[(size_t)c * a.P + start];
-        a.lengths[ant] = len;
-        if (fb) atomicAdd(&a.state->fb_steps, fb);
-    }
+    if (a.n == 1 && lane == 0) tour[0] = start;
+    if (lane == 0 && fb) atomicAdd(&a.state->fb_steps, fb);
+    store_tour_length(a, tour, ant, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -2170,9 +2238,8 @@ hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_p
         const size_t lds = staged_lds_bytes(a.n, a.k, a.words, W);
         hipLaunchKernelGGL(k_construct_staged, dim3(cdiv(a.m, W) + a.young_blocks), dim3(64 * W), lds, s, a);
     } else {
-        const int32_t nchunk = (a.n + 63) / 64;
-        const size_t per_ant = (size_t)a.words + 2 * (size_t)nchunk * 64 + nchunk;
-        hipLaunchKernelGGL(k_construct_full, dim3(cdiv(a.m, W)), dim3(64 * W), (size_t)W * per_ant * 4, s, a);
+        const size_t lds = (size_t)W * full_lds_words(a.n, a.words) * 4;
+        hipLaunchKernelGGL(k_construct_full, dim3(cdiv(a.m, W)), dim3(64 * W), lds, s, a);
     }
     return hipGetLastError();
 }
