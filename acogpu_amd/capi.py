@@ -332,10 +332,11 @@ class Colony:
                 "max_lds": int(t[3])}
 
     def counters(self):
-        """(fallback steps, unvisited cities scanned by fallback steps, tours constructed)"""
-        t = np.zeros(3, dtype=np.uint64)
+        """(fallback steps, cities compared by fallback steps, tours constructed,
+        fallback steps answered by a row top list)"""
+        t = np.zeros(4, dtype=np.uint64)
         self._chk(self._L.aco_get_counters(self._ctx, _p(t)))
-        return int(t[0]), int(t[1]), int(t[2])
+        return int(t[0]), int(t[1]), int(t[2]), int(t[3])
 
 
 class Batch:

@@ -46,6 +46,8 @@ struct aco_ctx {
     int32_t* d_best_tour = nullptr;
     int32_t* d_dep_ant = nullptr;
     uint16_t* d_adj = nullptr;  // Ant System adjacency slots, n x adj_pitch(m) (null: atomic deposit)
+    uint16_t* d_top = nullptr;  // row top lists, n x TOP_K (null: none)
+    float* d_top_thr = nullptr; // per row: TOP_K-th weight of the previous sweep
     float* d_dep_val = nullptr;
     DevState* d_state = nullptr;
     int64_t* d_scratch64 = nullptr;
@@ -108,7 +110,8 @@ static void free_all(aco_ctx* c) {
     void* ptrs[] = {c->d_dist, c->d_tau[0], c->d_tau[1], c->d_choice, c->d_choice_nn, c->d_nn, c->d_eta,
                     c->d_tau_nn, c->d_eta_nn, c->d_nn16,
                     c->d_mean, c->d_tours, c->d_len, c->d_prev_len, c->d_age, c->d_perf, c->d_valid,
-                    c->d_best_tour, c->d_dep_ant, c->d_dep_val, c->d_state, c->d_scratch64, c->d_adj};
+                    c->d_best_tour, c->d_dep_ant, c->d_dep_val, c->d_state, c->d_scratch64, c->d_adj,
+                    c->d_top, c->d_top_thr};
     for (void* q : ptrs)
         if (q) hipFree(q);
     for (auto& e : c->ev)
@@ -270,6 +273,12 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     TRYB(dalloc(c, &c->d_dep_val, (size_t)c->m + c->p.w + 2));
     if (c->p.variant == ACO_VARIANT_AS && c->n <= aco::ADJ_MAX_N && c->p.deposit_path != 1)
         TRYB(dalloc(c, &c->d_adj, (size_t)c->n * aco::adj_pitch(c->m)));
+    // Row top lists for large candidate-mode instances (alpha = 1: the
+    // sweep's lower bound of the TOP_K-th weight holds for tau times eta)
+    if (c->p.mode == ACO_MODE_CANDIDATE && c->n >= aco::TOP_MIN_N && c->n <= 65535 && c->p.alpha == 1.0f) {
+        TRYB(dalloc(c, &c->d_top, (size_t)c->n * aco::TOP_K));
+        TRYB(dalloc(c, &c->d_top_thr, (size_t)c->n));
+    }
     TRYB(dalloc(c, &c->d_state, 1));
     TRYB(dalloc(c, &c->d_scratch64, 4));
     HIPB(hipMemsetAsync(c->d_dist, 0, nP * sizeof(int32_t), c->stream));
@@ -380,6 +389,7 @@ extern "C" int aco_initialize(aco_ctx* c) {
     c->next_phase = ACO_PHASE_PROBABILITIES;
     HIP_TRY(c, aco::launch_fill_rows(c->d_tau[0], c->n, c->P, tau0, c->stream));
     HIP_TRY(c, aco::launch_fill_rows(c->d_tau[1], c->n, c->P, 0.0f, c->stream));
+    if (c->d_top_thr) HIP_TRY(c, hipMemsetAsync(c->d_top_thr, 0, sizeof(float) * c->n, c->stream));
     HIP_TRY(c, aco::launch_nn_lists(c->d_dist, c->n, c->P, c->nn_cols, c->d_nn, c->stream));
     if (c->k > 0)
         HIP_TRY(c, aco::launch_nn_aux(c->d_nn, c->d_dist, c->d_eta, c->eta_len, c->n, c->P, c->k, c->d_nn16,
@@ -425,6 +435,8 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur) {
     sa.one_minus_rho = 1.0f - c->p.rho;
     sa.want_mean = c->p.aging;
     sa.state = c->d_state;
+    sa.top = c->d_top;
+    sa.top_thr = c->d_top_thr;
     if ((e = aco::launch_sweep(sa, c->stream)) != hipSuccess) return e;
     return hipSuccess;
 }
@@ -466,6 +478,7 @@ static hipError_t enqueue_construct(aco_ctx* c, int cur) {
     ca.stream = c->p.stream_id;
     ca.words = c->words;
     ca.young_blocks = c->young_blocks;
+    ca.top = c->d_top;
     if ((e = aco::launch_construct(ca, c->path, c->apb, c->stream)) != hipSuccess) return e;
     return hipSuccess;
 }
@@ -858,6 +871,7 @@ extern "C" int aco_get_counters(aco_ctx* c, uint64_t* counters) {
     counters[0] = st.fb_steps;
     counters[1] = st.fb_scanned;
     counters[2] = st.tours;
+    counters[3] = st.fb_listed;
     return ACO_OK;
 }
 

@@ -25,7 +25,18 @@ struct DevState {
     int32_t halted;          // set by the statistics kernel; cleared by the host
     int32_t skip;            // the current iteration is skipped (the sweep copies halted)
     unsigned long long tours;       // tours constructed (valid ants of every iteration run)
+    unsigned long long fb_listed;   // fallback steps answered by the row's top list
 };
+
+// Top list of a pheromone row (large instances, alpha = 1): the TOP_K cities
+// with the largest selection weight, in the order (weight descending, index
+// ascending) that the fallback's argmax uses.  Rebuilt by every sweep; a
+// fallback takes the first unvisited entry and scans the row only when all
+// TOP_K are visited.  TOP_INVALID in entry 0 marks a row without a list.
+constexpr int TOP_K = 256;
+constexpr int TOP_BUF = 512;               // per-wave LDS key buffer of the selection
+constexpr int32_t TOP_MIN_N = 4096;        // smallest n with top lists
+constexpr uint16_t TOP_INVALID = 0xffff;
 
 struct SweepArgs {
     const float* tau;        // current pheromone (pitched n x P)
@@ -46,6 +57,8 @@ struct SweepArgs {
     float alpha, one_minus_rho;
     int32_t want_mean;
     DevState* state;         // single colony: halting flags (null in a batch)
+    uint16_t* top;           // n x TOP_K row top lists (null: none)
+    float* top_thr;          // n: the TOP_K-th weight of the previous sweep
 };
 
 struct ConstructArgs {
@@ -71,6 +84,7 @@ struct ConstructArgs {
     uint32_t seed_lo, seed_hi, stream;
     int32_t words;           // visited-bitmap words per ant
     int32_t young_blocks;    // trailing blocks reserved for young aging ants (staged path)
+    const uint16_t* top;     // n x TOP_K row top lists (null: none)
 };
 
 struct StatsArgs {

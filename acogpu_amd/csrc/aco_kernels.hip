@@ -163,13 +163,93 @@ __global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dis
 // group are issued first, then the heuristic gathers, then the stores (the
 // output pointers may alias nothing the group still reads).  The row-mean
 // order is unchanged: lane l adds its columns 256s + 4l + q in s, q order.
-__device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lane) {
+// Top-list keys: a larger key is a larger weight, then a lower index.
+__device__ __forceinline__ uint64_t top_key(float v, int32_t j) {
+    return ((uint64_t)__float_as_uint(v) << 32) | (uint32_t)(0xffffffffu - (uint32_t)j);
+}
+
+// Descending bitonic sort of buf[0 .. TOP_BUF) by one wave.
+__device__ __forceinline__ void wave_sort_desc(uint64_t* buf, int lane) {
+    for (int k = 2; k <= TOP_BUF; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            wave_lds_sync();
+#pragma unroll
+            for (int t = 0; t < TOP_BUF / 64; ++t) {
+                const int i = t * 64 + lane;
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t x = buf[i], y = buf[ixj];
+                    const bool desc = (i & k) == 0;
+                    if (desc ? (x < y) : (x > y)) {
+                        buf[i] = y;
+                        buf[ixj] = x;
+                    }
+                }
+            }
+        }
+    }
+    wave_lds_sync();
+}
+
+// Keeps the TOP_K largest keys of buf[0 .. cnt) in buf[0 .. TOP_K), sorted;
+// returns the TOP_K-th weight (keys below it can no longer enter).
+__device__ __forceinline__ float top_compact(uint64_t* buf, int32_t& cnt, int lane) {
+    for (int32_t x = cnt + lane; x < TOP_BUF; x += 64) buf[x] = 0ull;
+    wave_sort_desc(buf, lane);
+    cnt = TOP_K;
+    return __uint_as_float((uint32_t)(buf[TOP_K - 1] >> 32));
+}
+
+// Streaming selection of a row's top list during the sweep.  The previous
+// sweep's TOP_K-th weight w, times (1 - rho) and a rounding margin, is a lower
+// bound of this sweep's TOP_K-th weight (evaporation scales every weight by
+// (1 - rho), deposits only add), so only weights >= that bound are kept;
+// fewer than TOP_K of them means the bound did not hold (pheromone set from
+// outside) and the row gets no list this iteration and a full selection the
+// next.
+struct TopSel {
+    uint64_t* buf;
+    float thr;
+    int32_t cnt;
+};
+
+__device__ __forceinline__ void top_offer(TopSel& t, float v, int32_t j, bool in, int lane) {
+    const bool pass = in && v >= t.thr;  // NaN never passes
+    const unsigned long long m = __ballot(pass);
+    if (!m) return;
+    if (t.cnt + __popcll(m) > TOP_BUF) t.thr = fmaxf(t.thr, top_compact(t.buf, t.cnt, lane));
+    if (pass) t.buf[t.cnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = top_key(v, j);
+    t.cnt += __popcll(m);
+    wave_lds_sync();
+}
+
+__device__ __forceinline__ void top_finish(const SweepArgs& a, TopSel& t, int32_t i, int lane) {
+    uint16_t* out = a.top + (size_t)i * TOP_K;
+    if (t.cnt >= TOP_K) {
+        const float w = top_compact(t.buf, t.cnt, lane);
+        for (int32_t r = lane; r < TOP_K; r += 64) out[r] = (uint16_t)(0xffffffffu - (uint32_t)t.buf[r]);
+        if (lane == 0) a.top_thr[i] = w;
+    } else {
+        if (lane == 0) {
+            out[0] = TOP_INVALID;
+            a.top_thr[i] = 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf) {
     const size_t row = (size_t)i * a.P;
     const float* __restrict__ trow = a.tau + row;
     const int32_t* __restrict__ drow = a.dist + row;
     float* __restrict__ crow = a.choice + row;
     float* __restrict__ nrow = a.tau_next + row;
     float acc = 0.0f;
+    TopSel top{topbuf, 0.0f, 0};
+    if (a.top != nullptr) {
+        // lower bound of this sweep's TOP_K-th weight: (1 - rho) w (1 - u)^3
+        // >= fl(fl(w (1 - rho)) (1 - 2^-20)) for the unit roundoff u = 2^-24
+        top.thr = (a.top_thr[i] * a.one_minus_rho) * 0.99999904632568359375f;
+    }
     for (int32_t g0 = 0; g0 < a.n; g0 += 1024) {
         float4 t[4];
         int4 d[4];
@@ -209,7 +289,22 @@ __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lan
             *reinterpret_cast<float4*>(crow + j0) = make_float4(cv[0], cv[1], cv[2], cv[3]);
             *reinterpret_cast<float4*>(nrow + j0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
         }
+        if (a.top != nullptr) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int32_t j0 = g0 + 256 * s + 4 * lane;
+                const float tv[4] = {t[s].x, t[s].y, t[s].z, t[s].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool in = (j0 + q) < a.n;
+                    // the weight just stored (alpha = 1: tau times eta)
+                    const float v = in ? tv[q] * e[s][q] : 0.0f;
+                    top_offer(top, v, j0 + q, in, lane);
+                }
+            }
+        }
     }
+    if (a.top != nullptr) top_finish(a, top, i, lane);
     if (a.want_mean) {
         const float total = readlane_f(wave_inclusive_scan(acc), 63);
         if (lane == 0) a.mean[i] = total / (float)a.n;
@@ -225,6 +320,7 @@ __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lan
 }
 
 __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
+    extern __shared__ __align__(16) uint64_t topbuf[];  // TOP_BUF keys per wave when a.top is set
     const int lane = threadIdx.x & 63;
     // A halted run skips whole iterations: the sweep, first kernel of an
     // iteration, publishes the decision for the other three.
@@ -239,7 +335,7 @@ __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
         for (int32_t k = blockIdx.x * 256 + threadIdx.x; k < a.m; k += gridDim.x * 256) a.age[k] += 1;
     const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= a.n) return;
-    sweep_row(a, i, lane);
+    sweep_row(a, i, lane, topbuf + (size_t)(threadIdx.x >> 6) * TOP_BUF);
 }
 
 // ---------------------------------------------------------------------------
@@ -323,10 +419,32 @@ __device__ __forceinline__ uint32_t rng_word(const AntRng& r, int32_t s) {
 // ties to the lowest index.  Lanes stride the row four loads at a time; exact
 // comparisons make the result independent of the reduction order.
 __device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const uint32_t* vis, int32_t c,
-                                                bool young, float ratio, int lane, unsigned long long& scanned) {
+                                                bool young, float ratio, int lane, unsigned long long& scanned,
+                                                unsigned long long& listed) {
     float best = -1.0f;
     int32_t bj = 0x7fffffff;
     const float* crow = a.choice + (size_t)c * a.P;
+    if (!young && a.top != nullptr) {
+        // The row's top list holds its TOP_K largest weights in argmax order
+        // (weight descending, index ascending), and every city outside it
+        // comes later in that order: its first unvisited entry is the argmax
+        // over all unvisited cities.
+        const uint16_t* tl = a.top + (size_t)c * TOP_K;
+        int32_t e[TOP_K / 64];
+#pragma unroll
+        for (int q = 0; q < TOP_K / 64; ++q) e[q] = tl[64 * q + lane];
+        if (readlane_i(e[0], 0) != (int32_t)TOP_INVALID) {
+#pragma unroll
+            for (int q = 0; q < TOP_K / 64; ++q) {
+                const unsigned long long m = __ballot(!is_visited(vis, e[q]));
+                if (m) {
+                    ++listed;
+                    if (lane == 0) ++scanned;
+                    return readlane_i(e[q], __ffsll((long long)m) - 1);
+                }
+            }
+        }
+    }
     if (!young) {
         // 16-byte coalesced loads, 16 per lane in flight (4096 cities per
         // batch, one memory round trip each); the row is padded to a
@@ -391,11 +509,12 @@ __device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const
 }
 
 __device__ __forceinline__ void finish_counters(const ConstructArgs& a, unsigned long long fb,
-                                                unsigned long long scanned, int lane) {
+                                                unsigned long long scanned, int lane, unsigned long long listed = 0) {
     for (int off = 32; off >= 1; off >>= 1) scanned += __shfl_xor(scanned, off);
     if (lane == 0) {
         if (fb) atomicAdd(&a.state->fb_steps, fb);
         if (scanned) atomicAdd(&a.state->fb_scanned, scanned);
+        if (listed) atomicAdd(&a.state->fb_listed, listed);
     }
 }
 
@@ -484,7 +603,7 @@ __device__ __forceinline__ void construct_cand_ant(const ConstructArgs& a, int32
     int32_t j;
     float cw;
     load_row(start, j, cw);
-    unsigned long long fb = 0, scanned = 0;
+    unsigned long long fb = 0, scanned = 0, listed = 0;
     for (int32_t s = 1; s < a.n; ++s) {
         rng_step(rng, s, (uint32_t)ant, iter, a, lane);
         const float u = word_to_uniform(rng_word(rng, s));
@@ -501,14 +620,14 @@ __device__ __forceinline__ void construct_cand_ant(const ConstructArgs& a, int32
             next = readlane_i(j, 63 - __clzll((long long)pos));
         } else {
             ++fb;
-            next = fallback_argmax(a, vis, c, young, ratio, lane, scanned);
+            next = fallback_argmax(a, vis, c, young, ratio, lane, scanned, listed);
         }
         load_row(next, j, cw);
         mark_visited(vis, next, lane);
         stage_city(tour, tbuf, s, next, a.n, lane);
         c = next;
     }
-    finish_counters(a, fb, scanned, lane);
+    finish_counters(a, fb, scanned, lane, listed);
     store_tour_length(a, tour, ant, lane);
 }
 
@@ -692,7 +811,7 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
     const int32_t loff = lane < a.k ? lane : a.n * a.k;
     int32_t tbuf = start;
     int32_t c = start;
-    unsigned long long fb = 0, scanned = 0;
+    unsigned long long fb = 0, scanned = 0, listed = 0;
     // the candidate row of the current city is loaded one step ahead: the
     // LDS reads for step s+1 are issued as soon as step s has chosen, before
     // its bookkeeping
@@ -733,7 +852,7 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
         } else {
             ++fb;
             if (!VREG_VIS)
-                next = fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned);
+                next = fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned, listed);
             else
                 next = fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned);
         }
@@ -752,7 +871,7 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
         j = jn;
         cw = cwn;
     }
-    finish_counters(a, fb, scanned, lane);
+    finish_counters(a, fb, scanned, lane, listed);
     // tour out (coalesced) and exact length, edge i -> i+1 per lane
     int32_t* tour = a.tours + (size_t)ant * a.n;
     int64_t len = 0;
@@ -1799,7 +1918,7 @@ __global__ __launch_bounds__(256) void kb_sweep(const BatchBuffers b) {
     const int32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= b.total_rows) return;
     const InstMeta im = b.meta[b.row_inst[r]];
-    sweep_row(batch_sweep_args(b, im), r - im.city_off, threadIdx.x & 63);
+    sweep_row(batch_sweep_args(b, im), r - im.city_off, threadIdx.x & 63, nullptr);
 }
 
 // The persistent construction's work queue: QUEUE_BINS bins of items, each
@@ -2219,7 +2338,8 @@ hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t star
 }
 
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_sweep, dim3(cdiv(a.n, 4)), dim3(256), 0, s, a);
+    const size_t lds = a.top != nullptr ? (size_t)4 * TOP_BUF * sizeof(uint64_t) : 0;
+    hipLaunchKernelGGL(k_sweep, dim3(cdiv(a.n, 4)), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

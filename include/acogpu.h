@@ -168,8 +168,10 @@ int aco_set_heuristic_nn(aco_ctx *c, const float *h);
 int aco_get_heuristic_nn(aco_ctx *c, float *h);
 /* best-so-far length/tour overwrite (parity replays) */
 int aco_set_best(aco_ctx *c, int64_t glob_best, const int32_t *tour);
-/* counters[3] = {fallback steps, unvisited cities scanned by fallback steps,
- * tours constructed (the valid ants of every iteration run)} since
+/* counters[4] = {fallback steps, cities whose weight fallback steps compared
+ * (every unvisited city of a row scan, one for a top-list answer), tours
+ * constructed (the valid ants of every iteration run), fallback steps
+ * answered by the row's top list (n >= 4096, alpha = 1)} since
  * aco_initialize */
 int aco_get_counters(aco_ctx *c, uint64_t *counters);
 

@@ -91,7 +91,8 @@ def test_as_iteration_replay_fallback_heavy(path):
         col.iterate()
         ref = replay(p, dist, snap)
         check_iteration(col, p, dist, snap, ref)
-        steps, scanned, tours = col.counters()
+        steps, scanned, tours, listed = col.counters()
+        assert listed == 0  # no top lists below 4096 cities
         assert tours == p.m
         assert steps == ref["fallback"][0] and scanned == ref["fallback"][1]
         assert steps > 0
@@ -101,6 +102,7 @@ def test_as_iteration_replay_fallback_heavy(path):
     (5000, 24, 32, 1, 6),   # global-memory candidate rows, 32-lane scan, LDS bitmap
     (2500, 16, 40, 1, 8),   # global-memory candidate rows, 64-lane scan
     (3000, 16, 8, 2, 7),    # LDS-staged candidate rows with the LDS bitmap (n > 2048), fallback heavy
+    (4500, 16, 4, 2, 9),    # LDS-staged rows, fallbacks answered by the row top lists
 ])
 def test_as_iteration_replay_large_n(n, m, k, path, seed):
     say(f"case: AS large n={n} m={m} k={k} path={path} seed={seed}")
@@ -111,12 +113,17 @@ def test_as_iteration_replay_large_n(n, m, k, path, seed):
     with make(p, xy=xy) as col:
         assert col.config()["path"] == (0 if path == 1 else 2)
         col.set_pheromone(random_tau(rng, n))
-        for it in range(2):
+        for it in range(3):
             snap = snapshot(col)
             col.iterate()
             ref = replay(p, dist, snap)
             check_iteration(col, p, dist, snap, ref)
         assert ref["fallback"][0] > 0
+        fb, _, _, listed = col.counters()
+        if n >= 4096:  # row top lists (alpha = 1) answer most fallbacks, the rest scan rows
+            assert 0 < listed < fb
+        else:
+            assert listed == 0
 
 
 @pytest.mark.parametrize("path", PATHS)
