@@ -133,6 +133,9 @@ constexpr int32_t ADJ_BATCH_MAX_N = 10240;
 __host__ __device__ inline int32_t adj_stage_ants(int32_t n) {
     return n <= 1280 ? 32 : n <= 2560 ? 16 : n <= 5120 ? 8 : n <= 10240 ? 4 : 0;
 }
+// Larger n: blocks of ADJ_BAND_ANTS ants and ADJ_BAND_ROWS rows (128 KB of LDS).
+constexpr int32_t ADJ_BAND_ANTS = 16;
+constexpr int32_t ADJ_BAND_ROWS = 2048;
 
 // ---------------------------------------------------------------------------
 // Batched instances: all matrices of all instances live in shared buffers;

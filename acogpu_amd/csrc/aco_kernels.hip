@@ -1630,47 +1630,50 @@ __global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
 // store.  Plain 16-bit stores: every slot has exactly one writer.  A block
 // takes 32 consecutive ants, whose slots share each row's cache lines, so
 // the partial-line stores meet in one L2.
-__device__ __forceinline__ void adj_scatter_ant(const DepositArgs& a, int32_t ant, int lane) {
-    if (ant >= a.state->ndep || a.dep_val[ant] == 0.0f) return;
-    const int32_t* t = a.tours + (size_t)ant * a.n;
-    const size_t pitch = (size_t)adj_pitch(a.m);
-    for (int32_t i = lane; i < a.n; i += 64) {
-        const int32_t x = t[i];
-        const int32_t y = t[(i + 1 == a.n) ? 0 : i + 1];
-        a.adj[(size_t)x * pitch + 2 * ant] = (uint16_t)y;
-        a.adj[(size_t)y * pitch + 2 * ant + 1] = (uint16_t)x;
-    }
-}
-
-// Pass 1, staged (A = adj_stage_ants(n) ants a0 .. a0 + A - 1 of one
-// instance per block): the ants write their slots into an LDS copy of their
-// 2A columns of every row, which the block then stores row by row in whole
-// 16-byte chunks, so each row's slots of these ants reach memory in one
-// piece instead of 2A scattered 2-byte stores that each cost a line write.
-// Columns of ants past m or without a deposit hold garbage, which the
+// Pass 1, staged (A ants a0 .. a0 + A - 1 of one instance per block, rows
+// x0 .. x0 + X - 1 of the adjacency): the ants write their slots of those
+// rows into an LDS copy of their 2A columns, which the block then stores row
+// by row in whole 16-byte chunks, so each row's slots of these ants reach
+// memory in one piece instead of 2A scattered 2-byte stores that each cost a
+// line write.  Small instances take every row at once (X = n); large ones
+// split the rows into bands, each block reading its ants' tours once per
+// band.  Columns of ants past m or without a deposit hold garbage, which the
 // gather never reads (dep_val 0 or beyond ndep).
-__device__ __forceinline__ void adj_scatter_staged(const DepositArgs& a, int32_t a0, int32_t A, uint16_t* st) {
+__device__ __forceinline__ void adj_scatter_staged(const DepositArgs& a, int32_t a0, int32_t A, int32_t x0, int32_t X,
+                                                   uint16_t* st) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int32_t w2 = 2 * A;
+    const int32_t rows = min(X, a.n - x0);
     for (int32_t j = wave; j < A; j += nw) {
         const int32_t ant = a0 + j;
         if (ant >= a.m || ant >= a.state->ndep || a.dep_val[ant] == 0.0f) continue;
         const int32_t* t = a.tours + (size_t)ant * a.n;
-        for (int32_t i = lane; i < a.n; i += 64) {
-            const int32_t x = t[i];
-            const int32_t y = t[(i + 1 == a.n) ? 0 : i + 1];
-            st[x * w2 + 2 * j] = (uint16_t)y;
-            st[y * w2 + 2 * j + 1] = (uint16_t)x;
+        // eight edges per lane per pass, their tour loads issued together
+        for (int32_t i0 = 0; i0 < a.n; i0 += 512) {
+            int32_t xs[8], ys[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int32_t i = i0 + 64 * q + lane;
+                xs[q] = i < a.n ? t[i] : -1;
+                ys[q] = i < a.n ? t[(i + 1 == a.n) ? 0 : i + 1] : -1;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int32_t x = xs[q], y = ys[q];
+                if (x < 0) continue;
+                if ((uint32_t)(x - x0) < (uint32_t)rows) st[(x - x0) * w2 + 2 * j] = (uint16_t)y;
+                if ((uint32_t)(y - x0) < (uint32_t)rows) st[(y - x0) * w2 + 2 * j + 1] = (uint16_t)x;
+            }
         }
     }
     __syncthreads();
     const int32_t pitch = adj_pitch(a.m);
     const int32_t chunks = w2 / 8;                // 16-byte chunks of a row's columns
     const int32_t lim = min(w2, pitch - 2 * a0);  // columns inside the row (a multiple of 8)
-    for (int32_t idx = threadIdx.x; idx < a.n * chunks; idx += blockDim.x) {
+    for (int32_t idx = threadIdx.x; idx < rows * chunks; idx += blockDim.x) {
         const int32_t r = idx / chunks, c = idx - r * chunks;
         if (8 * c >= lim) continue;
-        *reinterpret_cast<uint4*>(a.adj + (size_t)r * pitch + 2 * a0 + 8 * c) =
+        *reinterpret_cast<uint4*>(a.adj + (size_t)(x0 + r) * pitch + 2 * a0 + 8 * c) =
             *reinterpret_cast<const uint4*>(st + r * w2 + 8 * c);
     }
 }
@@ -1739,18 +1742,14 @@ __device__ __forceinline__ void adj_apply_row(const DepositArgs& a, int32_t x, f
     }
 }
 
-// n > 10240: 32 ants per block, one wave per ant, slots stored one by one.
-__global__ __launch_bounds__(256) void k_adj_scatter(const DepositArgs a) {
-    if (a.state->skip) return;
-    const int32_t ant = blockIdx.x * 32 + (threadIdx.x >> 6);
-    for (int r = 0; r < 8; ++r)
-        if (ant + 4 * r < a.m) adj_scatter_ant(a, ant + 4 * r, threadIdx.x & 63);
-}
-
-__global__ __launch_bounds__(256) void k_adj_scatter_staged(const DepositArgs a, int32_t A) {
+// Block (g, band) = blockIdx.x % groups, / groups: ants g A .. g A + A - 1,
+// rows band X .. band X + X - 1.
+__global__ __launch_bounds__(256) void k_adj_scatter_staged(const DepositArgs a, int32_t A, int32_t X) {
     extern __shared__ __align__(16) uint16_t adj_stage[];
     if (a.state->skip) return;
-    adj_scatter_staged(a, blockIdx.x * A, A, adj_stage);
+    const int32_t groups = (a.m + A - 1) / A;
+    const int32_t g = blockIdx.x % groups, band = blockIdx.x / groups;
+    adj_scatter_staged(a, g * A, A, band * X, X, adj_stage);
 }
 
 // One block per row: the LDS row holds n floats (n <= ADJ_MAX_N).
@@ -2267,7 +2266,7 @@ __global__ __launch_bounds__(256) void kb_adj_scatter(const BatchBuffers b) {
     extern __shared__ __align__(16) uint16_t adj_stage_b[];
     const int2 it = b.adj_items[blockIdx.x];
     const InstMeta im = b.meta[it.x];
-    adj_scatter_staged(batch_deposit_args(b, it.x, im), it.y, b.adj_ants, adj_stage_b);
+    adj_scatter_staged(batch_deposit_args(b, it.x, im), it.y, b.adj_ants, 0, im.n, adj_stage_b);
 }
 
 // One wave per row, four rows per block; each wave's LDS row holds nmax
@@ -2385,10 +2384,13 @@ hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s) {
 hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s) {
     if (!a.adj || !a.dense || a.n > ADJ_MAX_N) return hipErrorInvalidValue;
     const int32_t A = adj_stage_ants(a.n);
-    if (A)
-        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(256), (size_t)4 * a.n * A, s, a, A);
-    else
-        hipLaunchKernelGGL(k_adj_scatter, dim3(cdiv(a.m, 32)), dim3(256), 0, s, a);
+    if (A) {
+        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(256), (size_t)4 * a.n * A, s, a, A, a.n);
+    } else {  // bands of ADJ_BAND_ROWS rows, ADJ_BAND_ANTS ants per block
+        const int32_t bands = cdiv(a.n, ADJ_BAND_ROWS);
+        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, ADJ_BAND_ANTS) * bands), dim3(256),
+                           (size_t)4 * ADJ_BAND_ROWS * ADJ_BAND_ANTS, s, a, ADJ_BAND_ANTS, ADJ_BAND_ROWS);
+    }
     hipLaunchKernelGGL(k_adj_gather, dim3(a.n), dim3(256), (size_t)a.n * sizeof(float), s, a);
     return hipGetLastError();
 }

@@ -8,7 +8,7 @@ import pytest
 
 import oracle as orc
 from acogpu_amd import capi, instances
-from replay import check_iteration, replay, snapshot
+from replay import MAXD, check_iteration, replay, snapshot
 
 pytestmark = pytest.mark.gpu
 
@@ -285,6 +285,43 @@ def test_as_deposit_paths_agree_and_replay(aging):
     try:
         np.testing.assert_array_equal(cols[0].tours(), cols[1].tours())
         np.testing.assert_allclose(cols[0].pheromone(), cols[1].pheromone(), rtol=2e-6)
+    finally:
+        for c in cols:
+            c.close()
+
+
+def test_as_deposit_paths_agree_large_n():
+    """n > 10240: the adjacency scatter runs in row bands.  Both deposit
+    paths from the same pheromone give the same tours and pheromone within
+    fp32 rounding, and the banded one is within 1e-5 of an FP64 evaporation
+    + deposit of those tours."""
+    n, m = 10500, 48
+    say(f"case: AS deposit paths n={n} m={m} k=8 (banded adjacency scatter)")
+    rng = np.random.default_rng(43)
+    xy = instances.uniform(n, 100000, seed=43)
+    tau = random_tau(rng, n)
+    cols = []
+    try:
+        for path in (1, 2):
+            p = capi.default_params(n=n, m=m, k=8, beta=2.0, rho=0.2, variant=capi.VARIANT_AS, seed=12,
+                                    deposit_path=path)
+            col = make(p, xy=xy)
+            cols.append(col)
+            col.set_pheromone(tau)
+            col.iterate()
+        tours, lengths = cols[1].tours(), cols[1].lengths()
+        np.testing.assert_array_equal(cols[0].tours(), tours)
+        g1, g2 = cols[0].pheromone(), cols[1].pheromone()
+        np.testing.assert_allclose(g1, g2, rtol=2e-6)
+        del g1
+        st = np.array([MAXD - 1, MAXD, -1, 0], dtype=np.int64)
+        best = np.zeros(n, np.int32)
+        dep_ant, dep_w = orc.stats(p.variant, tours, lengths, 1, st, best)
+        tau64 = tau.astype(np.float64)
+        orc.evaporate_f64(tau64, p.rho)
+        orc.deposit_f64(tau64, tours, lengths, best, int(st[1]), dep_ant, dep_w)
+        rel = np.abs(g2 - tau64) / np.abs(tau64)
+        assert rel.max() <= 1e-5, f"pheromone rel err {rel.max():.3e}"
     finally:
         for c in cols:
             c.close()
