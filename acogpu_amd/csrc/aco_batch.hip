@@ -303,8 +303,13 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
         std::fill(ant_inst.begin() + b->meta[i].ant_off, ant_inst.begin() + b->meta[i].ant_off + b->m[i], i);
     }
     std::vector<float> eta((size_t)dmax + 1);
-    eta[0] = (float)std::pow(2.0, (double)p->beta);  // duplicate cities: d taken as 1/2
-    for (int32_t d = 1; d <= dmax; ++d) eta[d] = (float)std::pow(1.0 / (double)d, (double)p->beta);
+    B.eta_pow = aco::eta_integer_power((double)p->beta, dmax);
+    if (B.eta_pow) {
+        for (int32_t d = 0; d <= dmax; ++d) eta[d] = aco::eta_power_value(B.eta_pow, d);
+    } else {
+        eta[0] = (float)std::pow(2.0, (double)p->beta);  // duplicate cities: d taken as 1/2
+        for (int32_t d = 1; d <= dmax; ++d) eta[d] = (float)std::pow(1.0 / (double)d, (double)p->beta);
+    }
     HIPB(hipMemcpy(d_meta, b->meta.data(), sizeof(InstMeta) * count, hipMemcpyHostToDevice));
     HIPB(hipMemcpy(d_row_inst, row_inst.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
     HIPB(hipMemcpy(d_ant_inst, ant_inst.data(), sizeof(int32_t) * ants, hipMemcpyHostToDevice));

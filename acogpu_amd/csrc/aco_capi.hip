@@ -36,6 +36,7 @@ struct aco_ctx {
     uint16_t* d_nn16 = nullptr;
     float* d_eta = nullptr;
     int32_t eta_len = 0;
+    int32_t eta_pow = 0;  // aco::eta_integer_power of (beta, largest distance)
     float* d_mean = nullptr;
     int32_t* d_tours = nullptr;
     int64_t* d_len = nullptr;
@@ -313,7 +314,7 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
         HIPB(aco::launch_fix_diagonal(c->d_dist, c->n, c->P, c->stream));
     }
 
-    // Heuristic table eta^beta by integer distance, fp64 pow rounded once.
+    // Heuristic table eta^beta by integer distance (aco::eta_integer_power).
     int32_t* d_max = reinterpret_cast<int32_t*>(c->d_scratch64);
     HIPB(hipMemsetAsync(d_max, 0, sizeof(int32_t), c->stream));
     HIPB(aco::launch_dist_max(c->d_dist, c->n, c->P, d_max, c->stream));
@@ -324,8 +325,13 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
         return bail(fail(c, ACO_E_RANGE, "largest distance %d exceeds the heuristic table limit 2^26", dmax));
     c->eta_len = dmax + 1;
     std::vector<float> eta((size_t)c->eta_len);
-    eta[0] = (float)std::pow(2.0, (double)c->p.beta);  // d == 0: duplicate cities, taken as d = 1/2
-    for (int32_t d = 1; d <= dmax; ++d) eta[d] = (float)std::pow(1.0 / (double)d, (double)c->p.beta);
+    c->eta_pow = aco::eta_integer_power((double)c->p.beta, dmax);
+    if (c->eta_pow) {
+        for (int32_t d = 0; d <= dmax; ++d) eta[d] = aco::eta_power_value(c->eta_pow, d);
+    } else {
+        eta[0] = (float)std::pow(2.0, (double)c->p.beta);  // d == 0: duplicate cities, taken as d = 1/2
+        for (int32_t d = 1; d <= dmax; ++d) eta[d] = (float)std::pow(1.0 / (double)d, (double)c->p.beta);
+    }
     TRYB(dalloc(c, &c->d_eta, (size_t)c->eta_len));
     HIPB(hipMemcpy(c->d_eta, eta.data(), sizeof(float) * eta.size(), hipMemcpyHostToDevice));
     HIPB(hipDeviceSynchronize());
@@ -434,6 +440,7 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur) {
     sa.alpha = c->p.alpha;
     sa.one_minus_rho = 1.0f - c->p.rho;
     sa.want_mean = c->p.aging;
+    sa.eta_pow = c->eta_pow;
     sa.state = c->d_state;
     sa.top = c->d_top;
     sa.top_thr = c->d_top_thr;

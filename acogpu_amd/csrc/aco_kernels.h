@@ -56,6 +56,7 @@ struct SweepArgs {
     int32_t n, P;
     float alpha, one_minus_rho;
     int32_t want_mean;
+    int32_t eta_pow;         // 1..3: eta computed as 1 / d^eta_pow (equal to the table); 0: table
     DevState* state;         // single colony: halting flags (null in a batch)
     uint16_t* top;           // n x TOP_K row top lists (null: none)
     float* top_thr;          // n: the TOP_K-th weight of the previous sweep
@@ -116,6 +117,28 @@ struct DepositArgs {
     uint16_t* adj;
     int32_t m;
 };
+
+// Heuristic eta^beta of an integer distance d.  beta = 1, 2 or 3 with
+// dmax^beta < 2^53 (d^beta exact in fp64): 1 / d^beta, one fp64 division
+// rounded once to fp32, which a kernel evaluates on the fly instead of
+// gathering it from the table (eta_pow = beta); any other beta: fp64
+// pow(1 / d, beta) rounded once (table only, eta_pow = 0).  d = 0 (duplicate
+// cities) is taken as d = 1/2.
+__host__ __device__ inline int32_t eta_integer_power(double beta, int32_t dmax) {
+    const int32_t b = (int32_t)beta;
+    if ((double)b != beta || b < 1 || b > 3) return 0;
+    double p = 1.0;
+    for (int32_t q = 0; q < b; ++q) p *= (double)dmax;
+    return p < 9007199254740992.0 ? b : 0;
+}
+__host__ __device__ inline float eta_power_value(int32_t pw, int32_t d) {
+    if (d == 0) return (float)(1 << pw);
+    const double dd = (double)d;
+    double p = dd;
+    if (pw >= 2) p *= dd;
+    if (pw >= 3) p *= dd;
+    return (float)(1.0 / p);
+}
 
 // Largest n whose deposit row (n fp32) is accumulated in LDS.
 constexpr int32_t ADJ_MAX_N = 32768;
@@ -184,6 +207,7 @@ struct BatchBuffers {
     int64_t* cnn;            // per-instance nearest-neighbour tour length
     float* tau0;             // per-instance initial pheromone
     float alpha, one_minus_rho;
+    int32_t eta_pow;         // as SweepArgs::eta_pow
     int32_t variant, w, ants_per_block, tau0_policy;
     int32_t ants_per_wave;   // construction: ants sharing one wave (1, 2 or 4)
     float tau0_const, rho;

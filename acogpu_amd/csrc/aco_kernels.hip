@@ -267,10 +267,16 @@ __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lan
         float e[4][4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            e[s][0] = eta_of(a.eta, a.eta_len, d[s].x);
-            e[s][1] = eta_of(a.eta, a.eta_len, d[s].y);
-            e[s][2] = eta_of(a.eta, a.eta_len, d[s].z);
-            e[s][3] = eta_of(a.eta, a.eta_len, d[s].w);
+            const int32_t dv[4] = {d[s].x, d[s].y, d[s].z, d[s].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                // 1 / d^beta computed (the table's value) rather than
+                // gathered: the gathers were most of the sweep's L2 requests
+                if (a.eta_pow)
+                    e[s][q] = (uint32_t)dv[q] < (uint32_t)a.eta_len ? eta_power_value(a.eta_pow, dv[q]) : 0.0f;
+                else
+                    e[s][q] = eta_of(a.eta, a.eta_len, dv[q]);
+            }
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -1794,6 +1800,7 @@ __device__ __forceinline__ SweepArgs batch_sweep_args(const BatchBuffers& b, con
     a.alpha = b.alpha;
     a.one_minus_rho = b.one_minus_rho;
     a.want_mean = 0;
+    a.eta_pow = b.eta_pow;
     return a;
 }
 

@@ -150,11 +150,26 @@ void orc_nn_lists(const int32_t *dist, int32_t n, int32_t k, int32_t *nn) {
 /* Heuristic eta^beta.  Reference HURESTIC(d, b) = powf(1.0/(float)d, b),
  * 0 for d == MAX_DISTANCE (inc/Colony.h:17-21, src/Colony.cu:232-246).  Here
  * it is tabulated by integer distance, in fp64 and rounded to fp32 once, so
- * CPU and GPU read identical bits.  d == 0 (duplicate cities,
- * release/input/a280.tsp:178) would give +inf; it is defined as d = 1/2. */
+ * CPU and GPU read identical bits.  For beta = 1, 2 or 3 with dmax^beta <
+ * 2^53 the value is 1 / d^beta (d^beta exact, one correctly rounded fp64
+ * division), which the GPU sweep also evaluates on the fly; otherwise
+ * pow(1 / d, beta).  d == 0 (duplicate cities, release/input/a280.tsp:178)
+ * would give +inf; it is defined as d = 1/2. */
 int32_t orc_eta_table(double beta, int32_t dmax, float *table) {
-    table[0] = (float)pow(2.0, beta);
-    for (int32_t d = 1; d <= dmax; d++) table[d] = (float)pow(1.0 / (double)d, beta);
+    const int32_t b = (int32_t)beta;
+    double top = 1.0;
+    for (int32_t q = 0; q < b && q < 3; q++) top *= (double)dmax;
+    if ((double)b == beta && b >= 1 && b <= 3 && top < 9007199254740992.0) {
+        table[0] = (float)(1 << b);
+        for (int32_t d = 1; d <= dmax; d++) {
+            const double dd = (double)d;
+            const double p = b == 1 ? dd : (b == 2 ? dd * dd : dd * dd * dd);
+            table[d] = (float)(1.0 / p);
+        }
+    } else {
+        table[0] = (float)pow(2.0, beta);
+        for (int32_t d = 1; d <= dmax; d++) table[d] = (float)pow(1.0 / (double)d, beta);
+    }
     return dmax + 1;
 }
 
