@@ -379,8 +379,11 @@ extern "C" int aco_initialize(aco_ctx* c) {
     // Colony::computeParameters -> computeInitialPheromone (src/AntSystem.cu:194-203,
     // src/RankBasedAntSystem.cu:353-363).
     float tau0 = c->p.tau0;
+    HIP_TRY(c, aco::launch_nn_lists(c->d_dist, c->n, c->P, c->nn_cols, c->d_nn, c->stream));
     if (c->p.tau0_policy != ACO_TAU0_CONST) {
-        HIP_TRY(c, aco::launch_greedy(c->d_dist, c->n, c->P, 0, c->d_scratch64, c->stream));
+        const bool lists = c->nn_cols < c->n;
+        HIP_TRY(c, aco::launch_greedy(c->d_dist, c->n, c->P, 0, c->d_scratch64, lists ? c->d_nn : nullptr,
+                                      lists ? c->nn_cols : 0, c->stream));
         int64_t cnn = 0;
         HIP_TRY(c, hipMemcpyAsync(&cnn, c->d_scratch64, sizeof(cnn), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -396,7 +399,6 @@ extern "C" int aco_initialize(aco_ctx* c) {
     HIP_TRY(c, aco::launch_fill_rows(c->d_tau[0], c->n, c->P, tau0, c->stream));
     HIP_TRY(c, aco::launch_fill_rows(c->d_tau[1], c->n, c->P, 0.0f, c->stream));
     if (c->d_top_thr) HIP_TRY(c, hipMemsetAsync(c->d_top_thr, 0, sizeof(float) * c->n, c->stream));
-    HIP_TRY(c, aco::launch_nn_lists(c->d_dist, c->n, c->P, c->nn_cols, c->d_nn, c->stream));
     if (c->k > 0)
         HIP_TRY(c, aco::launch_nn_aux(c->d_nn, c->d_dist, c->d_eta, c->eta_len, c->n, c->P, c->k, c->d_nn16,
                                       c->d_eta_nn, c->stream));

@@ -231,7 +231,9 @@ hipError_t launch_fix_diagonal(int32_t* dist, int32_t n, int32_t P, hipStream_t 
 hipError_t launch_dist_max(const int32_t* dist, int32_t n, int32_t P, int32_t* out, hipStream_t s);
 hipError_t launch_nn_lists(const int32_t* dist, int32_t n, int32_t P, int32_t k, int32_t* nn, hipStream_t s);
 hipError_t launch_fill_rows(float* a, int32_t n, int32_t P, float v, hipStream_t s);
-hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, hipStream_t s);
+// nn: n x k candidate lists (k < n) to shortcut the steps, or null.
+hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, const int32_t* nn,
+                         int32_t k, hipStream_t s);
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t s);
 // Full-row construction: chunks of 64 candidates whose list entries and
 // weights are loaded together; LDS words per ant (visited bitmap, the weight

@@ -50,17 +50,21 @@ __global__ __launch_bounds__(256) void k_fix_diagonal(int32_t* __restrict__ dist
     if (i < n) dist[(size_t)i * P + i] = ACO_MAX_DISTANCE;
 }
 
+// Largest off-diagonal distance: rows strided over the grid, one atomic per
+// block (a single address takes every block's atomic, so few blocks).
 __global__ __launch_bounds__(256) void k_dist_max(const int32_t* __restrict__ dist, int32_t n, int32_t P,
                                                   int32_t* __restrict__ out) {
-    const int32_t i = blockIdx.y;
-    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    __shared__ int32_t red[4];
     int32_t v = 0;
-    if (j < n && j != i) {
-        const int32_t d = dist[(size_t)i * P + j];
-        if (d != ACO_MAX_DISTANCE) v = d;
-    }
+    for (int32_t i = blockIdx.x; i < n; i += gridDim.x)
+        for (int32_t j = threadIdx.x; j < n; j += 256) {
+            const int32_t d = dist[(size_t)i * P + j];
+            if (j != i && d != ACO_MAX_DISTANCE) v = max(v, d);
+        }
     for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, max(max(red[0], red[1]), max(red[2], red[3])));
 }
 
 // ---------------------------------------------------------------------------
@@ -106,16 +110,48 @@ __global__ __launch_bounds__(256) void k_fill_rows(float* __restrict__ a, int32_
 // Nearest-neighbour tour from `start` (ties to the lowest index): C_nn of the
 // initial-pheromone formulas (Colony::greedyDistance, a stub in the
 // reference, src/Colony.cu:1053-1055).  One block; the visited bitmap in LDS.
+// With candidate lists (nn, k < n, stable (d, j) order) wave 0 takes the
+// first unvisited candidate of the current city, the same city the full
+// (d, j) minimum would give, step after step without block barriers; only
+// when every candidate is visited does the whole block scan the row.
 __device__ void greedy_block(const int32_t* __restrict__ dist, int32_t n, int32_t P, int32_t start,
-                             int64_t* __restrict__ out, uint32_t* vis, uint64_t* red, int32_t& s_next) {
+                             int64_t* __restrict__ out, uint32_t* vis, uint64_t* red, int32_t& s_next,
+                             int32_t& s_step, const int32_t* __restrict__ nn, int32_t k) {
     const int32_t words = (n + 31) / 32;
     for (int32_t w = threadIdx.x; w < words; w += blockDim.x) vis[w] = 0;
     __syncthreads();
     if (threadIdx.x == 0) vis[start >> 5] |= 1u << (start & 31);
     __syncthreads();
-    int64_t len = 0;
+    int64_t len = 0;  // thread 0
     int32_t c = start;
-    for (int32_t s = 1; s < n; ++s) {
+    int32_t s = 1;
+    while (s < n) {
+        if (nn != nullptr) {
+            if (threadIdx.x < 64) {
+                const int lane = threadIdx.x;
+                while (s < n) {
+                    const int32_t j = lane < k ? nn[(size_t)c * k + lane] : 0;
+                    const unsigned long long m = __ballot(lane < k && !((vis[j >> 5] >> (j & 31)) & 1u));
+                    if (!m) break;
+                    const int32_t nx = readlane_i(j, __ffsll((long long)m) - 1);
+                    if (lane == 0) {
+                        len += dist[(size_t)c * P + nx];
+                        vis[nx >> 5] |= 1u << (nx & 31);
+                    }
+                    wave_lds_sync();
+                    c = nx;
+                    ++s;
+                }
+                if (lane == 0) {
+                    s_next = c;
+                    s_step = s;
+                }
+            }
+            __syncthreads();
+            c = s_next;
+            s = s_step;
+            if (s >= n) break;
+        }
         uint64_t best = ~0ull;
         for (int32_t j = threadIdx.x; j < n; j += blockDim.x) {
             if ((vis[j >> 5] >> (j & 31)) & 1u) continue;
@@ -139,16 +175,19 @@ __device__ void greedy_block(const int32_t* __restrict__ dist, int32_t n, int32_
         }
         __syncthreads();
         c = s_next;
+        ++s;
     }
     if (threadIdx.x == 0) *out = len + dist[(size_t)c * P + start];
 }
 
 __global__ __launch_bounds__(1024) void k_greedy(const int32_t* __restrict__ dist, int32_t n, int32_t P,
-                                                 int32_t start, int64_t* __restrict__ out) {
+                                                 int32_t start, int64_t* __restrict__ out,
+                                                 const int32_t* __restrict__ nn, int32_t k) {
     extern __shared__ __align__(16) uint32_t sh[];
-    __shared__ int32_t s_next;
+    __shared__ int32_t s_next, s_step;
     const int32_t words = (n + 31) / 32;
-    greedy_block(dist, n, P, start, out, sh, reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1)), s_next);
+    greedy_block(dist, n, P, start, out, sh, reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1)), s_next, s_step,
+                 nn, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -1893,12 +1932,12 @@ __global__ __launch_bounds__(256) void kb_nn_aux(const BatchBuffers b) {
 // One block per instance: C_nn, then tau0 for the instance's policy.
 __global__ __launch_bounds__(1024) void kb_greedy(const BatchBuffers b) {
     extern __shared__ __align__(16) uint32_t sh[];
-    __shared__ int32_t s_next;
+    __shared__ int32_t s_next, s_step;
     const int32_t inst = blockIdx.x;
     const InstMeta im = b.meta[inst];
     const int32_t words = (im.n + 31) / 32;
     greedy_block(b.dist + im.mat_off, im.n, im.P, 0, b.cnn + inst, sh,
-                 reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1)), s_next);
+                 reinterpret_cast<uint64_t*>(sh + ((words + 1) & ~1)), s_next, s_step, nullptr, 0);
     __syncthreads();
     if (threadIdx.x == 0) {
         const double cnn = (double)b.cnn[inst];
@@ -2322,7 +2361,7 @@ hipError_t launch_fix_diagonal(int32_t* dist, int32_t n, int32_t P, hipStream_t 
 }
 
 hipError_t launch_dist_max(const int32_t* dist, int32_t n, int32_t P, int32_t* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_dist_max, dim3(cdiv(n, 256), n), dim3(256), 0, s, dist, n, P, out);
+    hipLaunchKernelGGL(k_dist_max, dim3(n < 1024 ? n : 1024), dim3(256), 0, s, dist, n, P, out);
     return hipGetLastError();
 }
 
@@ -2336,10 +2375,11 @@ hipError_t launch_fill_rows(float* a, int32_t n, int32_t P, float v, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, hipStream_t s) {
+hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, const int32_t* nn,
+                         int32_t k, hipStream_t s) {
     const int32_t words = (n + 31) / 32;
     const size_t lds = (size_t)((words + 1) & ~1) * 4 + 16 * sizeof(uint64_t);
-    hipLaunchKernelGGL(k_greedy, dim3(1), dim3(1024), lds, s, dist, n, P, start, out);
+    hipLaunchKernelGGL(k_greedy, dim3(1), dim3(1024), lds, s, dist, n, P, start, out, nn, k);
     return hipGetLastError();
 }
 
