@@ -290,6 +290,33 @@ def test_as_deposit_paths_agree_and_replay(aging):
             c.close()
 
 
+def test_top_lists_rebuild_after_pheromone_set_from_outside():
+    """A row top list keeps only weights above (1 - rho) times the previous
+    sweep's 256th weight.  Pheromone scaled down from outside breaks that
+    bound: every row must detect it (no list, row scans, same tours), and the
+    next sweep selects from scratch."""
+    n, m, k = 5000, 16, 6
+    say(f"case: top lists after an outside pheromone change n={n} m={m} k={k}")
+    rng = np.random.default_rng(51)
+    xy = instances.uniform(n, 100000, seed=51)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=k, beta=2.0, rho=0.2, seed=77, construct_path=1)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, n))
+        listed = []
+        for it in range(3):
+            if it == 1:
+                col.set_pheromone((col.pheromone() * np.float32(1e-3)).astype(np.float32))
+            snap = snapshot(col)
+            before = col.counters()
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+            after = col.counters()
+            listed.append((after[0] - before[0], after[3] - before[3]))
+        assert listed[0][1] > 0 and listed[2][1] > 0, listed
+        assert listed[1][0] > 0 and listed[1][1] == 0, listed  # every row rejected its list
+
+
 def test_as_deposit_paths_agree_large_n():
     """n > 10240: the adjacency scatter runs in row bands.  Both deposit
     paths from the same pheromone give the same tours and pheromone within
