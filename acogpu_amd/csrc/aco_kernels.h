@@ -149,6 +149,9 @@ __host__ __device__ inline int32_t adj_pitch(int32_t m) { return (2 * m + 15) & 
 
 // Largest n of a batch whose rows gather in LDS four at a time.
 constexpr int32_t ADJ_BATCH_MAX_N = 10240;
+// Rows per wave of the batched gather: four while 4 waves x 4 rows of nmax
+// floats fit in 64 KB of LDS, else one.
+__host__ __device__ inline int gather_rows(int32_t nmax) { return nmax <= 1024 ? 4 : 1; }
 
 // Ants per staged scatter block for rows of n cities: the block's columns of
 // every row (n x 2A 16-bit slots) fit in LDS.  0: n too large, slots are

@@ -2307,30 +2307,108 @@ __global__ __launch_bounds__(256) void kb_exchange(const BatchBuffers b) {
     if (threadIdx.x == 0) b.state[me].glob_best = win.len;
 }
 
-// One staged scatter block per (instance, first ant) item.
-__global__ __launch_bounds__(256) void kb_adj_scatter(const BatchBuffers b) {
+// One staged scatter block per (instance, first ant) item; 16 waves, so
+// each wave reads two of the item's 32 tours.
+__global__ __launch_bounds__(1024) void kb_adj_scatter(const BatchBuffers b) {
     extern __shared__ __align__(16) uint16_t adj_stage_b[];
     const int2 it = b.adj_items[blockIdx.x];
     const InstMeta im = b.meta[it.x];
     adj_scatter_staged(batch_deposit_args(b, it.x, im), it.y, b.adj_ants, 0, im.n, adj_stage_b);
 }
 
-// One wave per row, four rows per block; each wave's LDS row holds nmax
-// floats (dynamic LDS 4 nmax floats).
+// R consecutive rows per wave, four waves per block; each wave's LDS holds R
+// rows of nmax floats (R = gather_rows(nmax)).  When a wave's rows belong to
+// one instance (all but the waves that straddle an instance boundary) the
+// instance's metadata, its ant weights and the slots of all its rows are
+// loaded together, the runs of equal targets summed into the LDS rows, and
+// the rows' nonzero sums added to tau with their loads issued together:
+// a few memory round trips per wave instead of several per row.  Other
+// waves take their rows one at a time.
+template <int R>
 __global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b, int32_t nmax) {
+    constexpr int GATHER_ROWS = R;
     extern __shared__ float delta_rows[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float* delta = delta_rows + (size_t)wave * nmax;
-    const int32_t r = blockIdx.x * 4 + wave;
-    if (r >= b.total_rows) return;
-    const int32_t inst = b.row_inst[r];
+    float* dl = delta_rows + (size_t)wave * GATHER_ROWS * nmax;
+    const int32_t r0 = (blockIdx.x * 4 + wave) * GATHER_ROWS;
+    if (r0 >= b.total_rows) return;
+    const int32_t inst = b.row_inst[r0];
+    const bool one = r0 + GATHER_ROWS <= b.total_rows && b.row_inst[r0 + GATHER_ROWS - 1] == inst;
+    if (!one) {
+        for (int q = 0; q < GATHER_ROWS && r0 + q < b.total_rows; ++q) {
+            const int32_t r = r0 + q;
+            const int32_t iq = b.row_inst[r];
+            const InstMeta im = b.meta[iq];
+            const DepositArgs a = batch_deposit_args(b, iq, im);
+            for (int32_t j = lane; j < a.n; j += 64) dl[j] = 0.0f;
+            wave_lds_sync();
+            adj_gather_row(a, r - im.city_off, dl, lane, 64);
+            wave_lds_sync();
+            adj_apply_row(a, r - im.city_off, dl, lane, 64);
+            wave_lds_sync();
+        }
+        return;
+    }
     const InstMeta im = b.meta[inst];
     const DepositArgs a = batch_deposit_args(b, inst, im);
-    for (int32_t j = lane; j < a.n; j += 64) delta[j] = 0.0f;
+    const int32_t x0 = r0 - im.city_off;
+    for (int32_t j = lane; j < GATHER_ROWS * a.n; j += 64) dl[(j / a.n) * nmax + j % a.n] = 0.0f;
     wave_lds_sync();
-    adj_gather_row(a, r - im.city_off, delta, lane, 64);
+    const size_t pitch = (size_t)adj_pitch(a.m);
+    const int32_t nd = min(a.m, a.state->ndep);
+    for (int32_t a0 = 8 * lane; a0 < nd; a0 += 512) {
+        float d[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) d[q] = (a0 + q < nd) ? a.dep_val[a0 + q] : 0.0f;
+        uint4 v[GATHER_ROWS][2];
+#pragma unroll
+        for (int r = 0; r < GATHER_ROWS; ++r) {
+            const uint16_t* row = a.adj + (size_t)(x0 + r) * pitch + 2 * a0;
+            v[r][0] = *reinterpret_cast<const uint4*>(row);
+            v[r][1] = *reinterpret_cast<const uint4*>(row + 8);
+        }
+#pragma unroll
+        for (int r = 0; r < GATHER_ROWS; ++r) {
+            const uint32_t w32[8] = {v[r][0].x, v[r][0].y, v[r][0].z, v[r][0].w,
+                                     v[r][1].x, v[r][1].y, v[r][1].z, v[r][1].w};
+            float* delta = dl + (size_t)r * nmax;
+            int32_t ts = -1, tp = -1;
+            float as = 0.0f, ap = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (d[q] == 0.0f) continue;  // past nd, or an ant without a deposit: stale slots
+                const int32_t su = (int32_t)(w32[q] & 0xffffu), pr = (int32_t)(w32[q] >> 16);
+                if (su == ts) {
+                    as += d[q];
+                } else {
+                    if (ts >= 0) atomicAdd(&delta[ts], as);
+                    ts = su;
+                    as = d[q];
+                }
+                if (pr == tp) {
+                    ap += d[q];
+                } else {
+                    if (tp >= 0) atomicAdd(&delta[tp], ap);
+                    tp = pr;
+                    ap = d[q];
+                }
+            }
+            if (ts >= 0) atomicAdd(&delta[ts], as);
+            if (tp >= 0) atomicAdd(&delta[tp], ap);
+        }
+    }
     wave_lds_sync();
-    adj_apply_row(a, r - im.city_off, delta, lane, 64);
+    for (int32_t j = lane; j < a.n; j += 64) {
+        float dv[GATHER_ROWS], tv[GATHER_ROWS];
+#pragma unroll
+        for (int r = 0; r < GATHER_ROWS; ++r) dv[r] = dl[(size_t)r * nmax + j];
+#pragma unroll
+        for (int r = 0; r < GATHER_ROWS; ++r)
+            tv[r] = dv[r] != 0.0f ? a.tau_next[(size_t)(x0 + r) * a.P + j] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < GATHER_ROWS; ++r)
+            if (dv[r] != 0.0f) a.tau_next[(size_t)(x0 + r) * a.P + j] = tv[r] + dv[r];
+    }
 }
 
 __global__ __launch_bounds__(256) void kb_deposit(const BatchBuffers b) {
@@ -2507,9 +2585,13 @@ hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, i
             break;
         default:
             if (b.variant == 0 && b.adj) {
-                hipLaunchKernelGGL(kb_adj_scatter, dim3(b.nadj_items), dim3(256), (size_t)4 * nmax * b.adj_ants, s, b);
-                hipLaunchKernelGGL(kb_adj_gather, dim3(cdiv(b.total_rows, 4)), dim3(256),
-                                   4 * (size_t)nmax * sizeof(float), s, b, nmax);
+                hipLaunchKernelGGL(kb_adj_scatter, dim3(b.nadj_items), dim3(1024), (size_t)4 * nmax * b.adj_ants, s, b);
+                const int R = gather_rows(nmax);
+                const size_t glds = 4 * (size_t)R * nmax * sizeof(float);
+                if (R == 4)
+                    hipLaunchKernelGGL(kb_adj_gather<4>, dim3(cdiv(b.total_rows, 16)), dim3(256), glds, s, b, nmax);
+                else
+                    hipLaunchKernelGGL(kb_adj_gather<1>, dim3(cdiv(b.total_rows, 4)), dim3(256), glds, s, b, nmax);
             } else {
                 hipLaunchKernelGGL(kb_deposit, dim3(b.variant == 0 ? b.total_ants : b.count * (b.w + 1)), dim3(256),
                                    0, s, b);
