@@ -1788,8 +1788,8 @@ __device__ __forceinline__ void adj_apply_row(const DepositArgs& a, int32_t x, f
 }
 
 // Block (g, band) = blockIdx.x % groups, / groups: ants g A .. g A + A - 1,
-// rows band X .. band X + X - 1.
-__global__ __launch_bounds__(256) void k_adj_scatter_staged(const DepositArgs a, int32_t A, int32_t X) {
+// rows band X .. band X + X - 1.  16 waves: one or two tours per wave.
+__global__ __launch_bounds__(1024) void k_adj_scatter_staged(const DepositArgs a, int32_t A, int32_t X) {
     extern __shared__ __align__(16) uint16_t adj_stage[];
     if (a.state->skip) return;
     const int32_t groups = (a.m + A - 1) / A;
@@ -1797,8 +1797,9 @@ __global__ __launch_bounds__(256) void k_adj_scatter_staged(const DepositArgs a,
     adj_scatter_staged(a, g * A, A, band * X, X, adj_stage);
 }
 
-// One block per row: the LDS row holds n floats (n <= ADJ_MAX_N).
-__global__ __launch_bounds__(256) void k_adj_gather(const DepositArgs a) {
+// One block of 1024 threads per row: the LDS row holds n floats (n <=
+// ADJ_MAX_N); zeroing and applying it take n / 1024 passes.
+__global__ __launch_bounds__(1024) void k_adj_gather(const DepositArgs a) {
     extern __shared__ float delta_row[];
     if (a.state->skip) return;
     for (int32_t j = threadIdx.x; j < a.n; j += blockDim.x) delta_row[j] = 0.0f;
@@ -2510,13 +2511,13 @@ hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s) {
     if (!a.adj || !a.dense || a.n > ADJ_MAX_N) return hipErrorInvalidValue;
     const int32_t A = adj_stage_ants(a.n);
     if (A) {
-        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(256), (size_t)4 * a.n * A, s, a, A, a.n);
+        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(1024), (size_t)4 * a.n * A, s, a, A, a.n);
     } else {  // bands of ADJ_BAND_ROWS rows, ADJ_BAND_ANTS ants per block
         const int32_t bands = cdiv(a.n, ADJ_BAND_ROWS);
-        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, ADJ_BAND_ANTS) * bands), dim3(256),
+        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, ADJ_BAND_ANTS) * bands), dim3(1024),
                            (size_t)4 * ADJ_BAND_ROWS * ADJ_BAND_ANTS, s, a, ADJ_BAND_ANTS, ADJ_BAND_ROWS);
     }
-    hipLaunchKernelGGL(k_adj_gather, dim3(a.n), dim3(256), (size_t)a.n * sizeof(float), s, a);
+    hipLaunchKernelGGL(k_adj_gather, dim3(a.n), dim3(1024), (size_t)a.n * sizeof(float), s, a);
     return hipGetLastError();
 }
 
