@@ -147,19 +147,29 @@ constexpr int32_t ADJ_MAX_N = 32768;
 // and the gather's 16-slot loads never leave their row.
 __host__ __device__ inline int32_t adj_pitch(int32_t m) { return (2 * m + 15) & ~15; }
 
-// Largest n of a batch whose rows gather in LDS four at a time.
-constexpr int32_t ADJ_BATCH_MAX_N = 10240;
+// Largest n of a batch with the adjacency deposit (its scatter needs
+// adj_stage_ants(n) > 0).
+constexpr int32_t ADJ_BATCH_MAX_N = 8192;
 // Rows per wave of the batched gather: four while 4 waves x 4 rows of nmax
 // floats fit in 64 KB of LDS, else one.
 __host__ __device__ inline int gather_rows(int32_t nmax) { return nmax <= 1024 ? 4 : 1; }
 
-// Ants per staged scatter block for rows of n cities: the block's columns of
-// every row (n x 2A 16-bit slots) fit in LDS.  0: n too large, slots are
-// stored one by one.
-__host__ __device__ inline int32_t adj_stage_ants(int32_t n) {
-    return n <= 1280 ? 32 : n <= 2560 ? 16 : n <= 5120 ? 8 : n <= 10240 ? 4 : 0;
+// Staged scatter: an LDS row holds a block's 2A slots of one city plus one
+// padding dword (2A + 2 16-bit entries, an odd number of dwords), so the
+// slots that one store instruction writes into rows of random cities fall
+// into different LDS banks.
+__host__ __device__ inline int32_t adj_stage_stride(int32_t A) { return 2 * A + 2; }
+__host__ __device__ inline size_t adj_stage_lds(int32_t rows, int32_t A) {
+    return (size_t)rows * adj_stage_stride(A) * sizeof(uint16_t);
 }
-// Larger n: blocks of ADJ_BAND_ANTS ants and ADJ_BAND_ROWS rows (128 KB of LDS).
+// Ants per staged scatter block for rows of n cities: the largest A whose
+// staging of every row fits in 160 KB of LDS; 0 beyond (row bands instead).
+__host__ __device__ inline int32_t adj_stage_ants(int32_t n) {
+    for (int32_t A = 32; A >= 4; A >>= 1)
+        if (adj_stage_lds(n, A) <= 163840) return A;
+    return 0;
+}
+// Larger n: blocks of ADJ_BAND_ANTS ants and ADJ_BAND_ROWS rows (136 KB of LDS).
 constexpr int32_t ADJ_BAND_ANTS = 16;
 constexpr int32_t ADJ_BAND_ROWS = 2048;
 

@@ -1688,6 +1688,7 @@ __device__ __forceinline__ void adj_scatter_staged(const DepositArgs& a, int32_t
                                                    uint16_t* st) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int32_t w2 = 2 * A;
+    const int32_t S = adj_stage_stride(A);
     const int32_t rows = min(X, a.n - x0);
     for (int32_t j = wave; j < A; j += nw) {
         const int32_t ant = a0 + j;
@@ -1706,8 +1707,8 @@ __device__ __forceinline__ void adj_scatter_staged(const DepositArgs& a, int32_t
             for (int q = 0; q < 8; ++q) {
                 const int32_t x = xs[q], y = ys[q];
                 if (x < 0) continue;
-                if ((uint32_t)(x - x0) < (uint32_t)rows) st[(x - x0) * w2 + 2 * j] = (uint16_t)y;
-                if ((uint32_t)(y - x0) < (uint32_t)rows) st[(y - x0) * w2 + 2 * j + 1] = (uint16_t)x;
+                if ((uint32_t)(x - x0) < (uint32_t)rows) st[(x - x0) * S + 2 * j] = (uint16_t)y;
+                if ((uint32_t)(y - x0) < (uint32_t)rows) st[(y - x0) * S + 2 * j + 1] = (uint16_t)x;
             }
         }
     }
@@ -1718,8 +1719,10 @@ __device__ __forceinline__ void adj_scatter_staged(const DepositArgs& a, int32_t
     for (int32_t idx = threadIdx.x; idx < rows * chunks; idx += blockDim.x) {
         const int32_t r = idx / chunks, c = idx - r * chunks;
         if (8 * c >= lim) continue;
+        // the padded LDS row is only 4-byte aligned: four dword reads
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(st + r * S + 8 * c);
         *reinterpret_cast<uint4*>(a.adj + (size_t)(x0 + r) * pitch + 2 * a0 + 8 * c) =
-            *reinterpret_cast<const uint4*>(st + r * w2 + 8 * c);
+            make_uint4(src[0], src[1], src[2], src[3]);
     }
 }
 
@@ -2511,11 +2514,11 @@ hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s) {
     if (!a.adj || !a.dense || a.n > ADJ_MAX_N) return hipErrorInvalidValue;
     const int32_t A = adj_stage_ants(a.n);
     if (A) {
-        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(1024), (size_t)4 * a.n * A, s, a, A, a.n);
+        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(1024), adj_stage_lds(a.n, A), s, a, A, a.n);
     } else {  // bands of ADJ_BAND_ROWS rows, ADJ_BAND_ANTS ants per block
         const int32_t bands = cdiv(a.n, ADJ_BAND_ROWS);
         hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, ADJ_BAND_ANTS) * bands), dim3(1024),
-                           (size_t)4 * ADJ_BAND_ROWS * ADJ_BAND_ANTS, s, a, ADJ_BAND_ANTS, ADJ_BAND_ROWS);
+                           adj_stage_lds(ADJ_BAND_ROWS, ADJ_BAND_ANTS), s, a, ADJ_BAND_ANTS, ADJ_BAND_ROWS);
     }
     hipLaunchKernelGGL(k_adj_gather, dim3(a.n), dim3(1024), (size_t)a.n * sizeof(float), s, a);
     return hipGetLastError();
@@ -2586,7 +2589,8 @@ hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, i
             break;
         default:
             if (b.variant == 0 && b.adj) {
-                hipLaunchKernelGGL(kb_adj_scatter, dim3(b.nadj_items), dim3(1024), (size_t)4 * nmax * b.adj_ants, s, b);
+                hipLaunchKernelGGL(kb_adj_scatter, dim3(b.nadj_items), dim3(1024), adj_stage_lds(nmax, b.adj_ants), s,
+                                   b);
                 const int R = gather_rows(nmax);
                 const size_t glds = 4 * (size_t)R * nmax * sizeof(float);
                 if (R == 4)
