@@ -2127,7 +2127,10 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
     const uint32_t kk = i < a.k ? (uint32_t)a.k : 0u;
     const int32_t loff = i < a.k ? i : a.n * a.k;
     int32_t c = start;
-    int2 cj = cand[(int32_t)__umul24((uint32_t)c, kk) + loff];
+    // per-lane row base; an entry address is one 24-bit multiply-add
+    const char* candb = reinterpret_cast<const char*>(cand + loff);
+    const uint32_t kk8 = 8u * kk;
+    int2 cj = *reinterpret_cast<const int2*>(candb + __umul24((uint32_t)c, kk8));
     int32_t tbuf = start;
     int32_t prev_last = 0;
     int64_t len = 0;
@@ -2148,9 +2151,11 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
         // j >> 3 addresses lane j >> 5 of the group (bpermute drops the two
         // low address bits; gb * 4 and j >> 3 share no bits for n <= 32 L);
         // the bit field extract reads bit j mod 32
-        const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((gb << 2) + (j >> 3), (int)vreg);
-        const bool vj = __builtin_amdgcn_ubfe(word, (uint32_t)j, 1u) != 0u;
-        const float w = vj ? 0.0f : cw;
+        // (the signed extract of bit j mod 32 is an all-ones mask for a
+        // visited city, which clears the weight's bits)
+        const int32_t word = __builtin_amdgcn_ds_bpermute((gb << 2) + (j >> 3), (int)vreg);
+        const int32_t vmask = __builtin_amdgcn_sbfe(word, (uint32_t)j, 1u);
+        const float w = __int_as_float(__float_as_int(cw) & ~vmask);
         const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
         const float p = group_scan<G>(w);
         const float tot = group_last<G>(p, lane);
@@ -2189,7 +2194,7 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
                 next = (g == gg) ? ng : next;
             }
         }
-        cj = cand[(int32_t)__umul24((uint32_t)next, kk) + loff];
+        cj = *reinterpret_cast<const int2*>(candb + __umul24((uint32_t)next, kk8));
         vreg |= (i == (next >> 5)) ? (1u << (next & 31)) : 0u;
         tbuf = (i == (s & (L - 1))) ? next : tbuf;
         if (__builtin_expect((s & (L - 1)) == L - 1 || s == a.n - 1, 0)) {
