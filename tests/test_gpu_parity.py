@@ -441,6 +441,50 @@ def test_c2_full_size_properties():
         check_iteration(col, p, dist, snap, ref)
 
 
+def test_c3_full_size_properties():
+    """The n = 20000 Ant System workload at full size (m = 1024, k = 32, row
+    top lists on): every tour a permutation with its exact length, the best
+    tour's length the best-so-far, and, after a few iterations, the next
+    iteration's first 16 ants replayed bit for bit by the oracle from the
+    GPU's own pheromone (ants are independent under the counter-based RNG).
+    The candidate lists fed to the oracle are the GPU's, checked against a
+    brute-force stable sort on sampled rows."""
+    n, m, k = 20000, 1024, 32
+    say(f"case: C3 full size n={n} m={m} k={k} AS, 3 iterations + replay of 16 ants")
+    xy = instances.uniform(n, 100000, seed=3)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=k, beta=2.0, rho=0.2, variant=capi.VARIANT_AS,
+                            tau0_policy=capi.TAU0_AS, seed=5)
+    with make(p, xy=xy) as col:
+        col.run(3)
+        fb, _, tours_built, listed = col.counters()
+        assert tours_built == 3 * m and fb > 0 and listed > 0
+        tours = col.tours()
+        np.testing.assert_array_equal(np.sort(tours, axis=1), np.broadcast_to(np.arange(n), (m, n)))
+        lens = dist[tours, np.roll(tours, -1, axis=1)].astype(np.int64).sum(axis=1)
+        np.testing.assert_array_equal(col.lengths(), lens)
+        ib, gb, _ = col.stats()
+        assert gb <= ib and instances.tour_length(dist, col.best_tour()) == gb
+        nn = col.nn()
+        rows = np.random.default_rng(7).choice(n, 40, replace=False)
+        for i in rows:
+            d = dist[i].astype(np.int64)
+            d[i] = np.iinfo(np.int64).max
+            np.testing.assert_array_equal(nn[i], np.lexsort((np.arange(n), d))[:k])
+        tau = col.pheromone()
+        it = col.aging()["iter"]
+        col.iterate()
+        eta = orc.eta_table(p.beta, orc.max_offdiag(dist))
+        choice, choice_nn, _, _ = orc.sweep(tau, dist, eta, nn, p.alpha, p.rho)
+        del tau
+        np.testing.assert_array_equal(col.choice_nn(), choice_nn)
+        q = 16
+        ref_tours, ref_len, _ = orc.construct(orc.MODE_CANDIDATE, dist, nn, choice, choice_nn, None, None, eta,
+                                              p.alpha, q, p.seed, it, p.stream_id)
+        np.testing.assert_array_equal(col.tours()[:q], ref_tours)
+        np.testing.assert_array_equal(col.lengths()[:q], ref_len)
+
+
 @pytest.mark.parametrize("mode", [capi.MODE_CANDIDATE, capi.MODE_FULLROW])
 def test_run_until_halts_on_device_like_the_reference_loop(mode):
     """aco_run_until: the optimum test on the iteration best (src/TSP.cpp:
