@@ -19,7 +19,66 @@ namespace aco {
 
 // tau^alpha rounded once from fp64 (the oracle rounds the same fp64 power);
 // out of line so that the alpha == 1 fast path stays small.
-__device__ __noinline__ float pow_alpha(float p, float alpha) { return (float)pow((double)p, (double)alpha); }
+// tau^alpha for alpha != 1, defined as 2^(alpha log2 tau) evaluated in fp64
+// by a fixed sequence of IEEE operations (no library calls, no fused
+// operations), so the CPU oracle computes the same bits: log2 from the
+// exponent plus 2 atanh((m-1)/(m+1)) / ln 2 for the mantissa m in
+// [sqrt(1/2), sqrt(2)) (odd series to t^23, |t| <= 0.172), 2^z from the
+// integer part as an exponent and e^(f ln 2), f in [0, 1), by its Taylor
+// series to the 18th power.  Relative error ~1e-15 before the rounding to
+// fp32.  Same operations in oracle/aco_oracle.c (orc_pow_alpha).
+__device__ __noinline__ float pow_alpha(float p, float alpha) {
+    if (!(p > 0.0f)) return p == 0.0f ? 0.0f : __builtin_nanf("");
+    double x = (double)p;  // a float: normal in fp64
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    int32_t e = (int32_t)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull));
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double t = (m - 1.0) / (m + 1.0);
+    const double t2 = t * t;
+    double s = 1.0 / 23.0;
+    s = s * t2 + 1.0 / 21.0;
+    s = s * t2 + 1.0 / 19.0;
+    s = s * t2 + 1.0 / 17.0;
+    s = s * t2 + 1.0 / 15.0;
+    s = s * t2 + 1.0 / 13.0;
+    s = s * t2 + 1.0 / 11.0;
+    s = s * t2 + 1.0 / 9.0;
+    s = s * t2 + 1.0 / 7.0;
+    s = s * t2 + 1.0 / 5.0;
+    s = s * t2 + 1.0 / 3.0;
+    s = s * t2 + 1.0;
+    const double lg = (double)e + ((2.0 * t) * s) * 1.4426950408889634;
+    const double z = (double)alpha * lg;
+    const double k = floor(z);
+    if (k < -1022.0) return 0.0f;
+    if (k > 1023.0) return __builtin_inff();
+    const double g = (z - k) * 0.6931471805599453;
+    double q = 1.0 / 6402373705728000.0;  // 1/18!
+    q = q * g + 1.0 / 355687428096000.0;
+    q = q * g + 1.0 / 20922789888000.0;
+    q = q * g + 1.0 / 1307674368000.0;
+    q = q * g + 1.0 / 87178291200.0;
+    q = q * g + 1.0 / 6227020800.0;
+    q = q * g + 1.0 / 479001600.0;
+    q = q * g + 1.0 / 39916800.0;
+    q = q * g + 1.0 / 3628800.0;
+    q = q * g + 1.0 / 362880.0;
+    q = q * g + 1.0 / 40320.0;
+    q = q * g + 1.0 / 5040.0;
+    q = q * g + 1.0 / 720.0;
+    q = q * g + 1.0 / 120.0;
+    q = q * g + 1.0 / 24.0;
+    q = q * g + 1.0 / 6.0;
+    q = q * g + 0.5;
+    q = q * g + 1.0;
+    q = q * g + 1.0;
+    const double scale = __longlong_as_double((long long)((uint64_t)((int64_t)k + 1023) << 52));
+    return (float)(q * scale);
+}
 
 __device__ __forceinline__ float apply_alpha(float p, float alpha) { return alpha == 1.0f ? p : pow_alpha(p, alpha); }
 

@@ -178,10 +178,51 @@ float orc_eta(const float *table, int32_t table_len, int32_t d) {
     return table[d];
 }
 
+/* tau^alpha for alpha != 1 (the reference's pow, src/Colony.cu:264), here
+ * defined as 2^(alpha log2 tau) in fp64 by a fixed sequence of IEEE
+ * operations so that the GPU's pow_alpha (acogpu_amd/csrc/aco_kernels.hip)
+ * gives the same bits: log2 = exponent + 2 atanh((m-1)/(m+1)) / ln 2 with the
+ * mantissa m in [sqrt(1/2), sqrt(2)) (odd series to t^23), 2^z = 2^floor(z)
+ * e^(frac(z) ln 2) (Taylor series to the 18th power). */
+float orc_pow_alpha(float p, float alpha) {
+    if (!(p > 0.0f)) return p == 0.0f ? 0.0f : NAN;
+    union { double d; uint64_t u; } v = {(double)p}, w;
+    int32_t e = (int32_t)((v.u >> 52) & 0x7ff) - 1023;
+    w.u = (v.u & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m = w.d;
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double t = (m - 1.0) / (m + 1.0);
+    const double t2 = t * t;
+    static const double odd[12] = {1.0 / 23.0, 1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0, 1.0 / 15.0, 1.0 / 13.0,
+                                   1.0 / 11.0, 1.0 / 9.0,  1.0 / 7.0,  1.0 / 5.0,  1.0 / 3.0,  1.0};
+    double s = odd[0];
+    for (int i = 1; i < 12; i++) s = s * t2 + odd[i];
+    const double lg = (double)e + ((2.0 * t) * s) * 1.4426950408889634;
+    const double z = (double)alpha * lg;
+    const double k = floor(z);
+    if (k < -1022.0) return 0.0f;
+    if (k > 1023.0) return INFINITY;
+    const double g = (z - k) * 0.6931471805599453;
+    static const double inv_fact[19] = {
+        1.0 / 6402373705728000.0, 1.0 / 355687428096000.0, 1.0 / 20922789888000.0, 1.0 / 1307674368000.0,
+        1.0 / 87178291200.0,      1.0 / 6227020800.0,      1.0 / 479001600.0,      1.0 / 39916800.0,
+        1.0 / 3628800.0,          1.0 / 362880.0,          1.0 / 40320.0,          1.0 / 5040.0,
+        1.0 / 720.0,              1.0 / 120.0,             1.0 / 24.0,             1.0 / 6.0,
+        0.5,                      1.0,                     1.0};
+    double q = inv_fact[0];
+    for (int i = 1; i < 19; i++) q = q * g + inv_fact[i];
+    union { double d; uint64_t u; } sc;
+    sc.u = (uint64_t)((int64_t)k + 1023) << 52;
+    return (float)(q * sc.d);
+}
+
 static inline float apply_alpha(float p, float alpha) {
     /* tau^alpha; the reference drops alpha in construction
      * (inc/Colony.cuh:29-30), which equals this for alpha == 1. */
-    return alpha == 1.0f ? p : (float)pow((double)p, (double)alpha);
+    return alpha == 1.0f ? p : orc_pow_alpha(p, alpha);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -17,6 +17,8 @@
  *   orc_euc2d_*            TSPLIB closed form, brute force, a280 duplicate
  *   orc_nn_lists           brute-force stable argsort
  *   orc_eta_table          closed form (pow) spot checks
+ *   orc_pow_alpha          fp64 numpy power rounded to fp32 on a grid (<= 1 ulp,
+ *                          equal almost everywhere); exact powers of two
  *   orc_wave_scan          exact integer prefix sums; fp64 error bound
  *   orc_select_candidate   chi-square frequencies; exact-arithmetic boundaries
  *   orc_sweep              elementwise closed forms
@@ -58,6 +60,7 @@ void orc_dist_from_float_coords(const float *x, const float *y, int32_t n, int32
 
 void orc_nn_lists(const int32_t *dist, int32_t n, int32_t k, int32_t *nn);
 int32_t orc_eta_table(double beta, int32_t dmax, float *table);
+float orc_pow_alpha(float p, float alpha);
 float orc_eta(const float *table, int32_t table_len, int32_t d);
 
 void orc_wave_scan(float v[64]);

@@ -59,6 +59,8 @@ def _declare(L):
     L.orc_dist_from_int_coords.argtypes = [vp, i32, vp]
     L.orc_dist_from_float_coords.argtypes = [vp, vp, i32, vp]
     L.orc_nn_lists.argtypes = [vp, i32, i32, vp]
+    L.orc_pow_alpha.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.orc_pow_alpha.restype = ctypes.c_float
     L.orc_eta_table.argtypes = [f64, i32, vp]
     L.orc_eta_table.restype = i32
     L.orc_wave_scan.argtypes = [vp]
@@ -151,6 +153,11 @@ def eta_table(beta, dmax):
     t = np.empty(dmax + 1, dtype=np.float32)
     lib().orc_eta_table(float(beta), int(dmax), _p(t))
     return t
+
+
+def pow_alpha(x, alpha):
+    """tau^alpha as the kernels define it (one call per element)."""
+    return np.array([lib().orc_pow_alpha(float(v), float(alpha)) for v in np.ravel(x)], dtype=np.float32)
 
 
 def wave_scan(v):

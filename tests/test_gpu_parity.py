@@ -290,6 +290,34 @@ def test_as_deposit_paths_agree_and_replay(aging):
             c.close()
 
 
+@pytest.mark.parametrize("alpha", [1.0, 1.5])
+def test_large_n_rank_based_aging_young_ants(alpha):
+    """n >= 4096 with aging: young ants (pulled-toward-the-mean weights) scan
+    rows in their fallbacks while the others use the row top lists; with
+    alpha != 1 there are no top lists at all."""
+    n, m, k = 4200, 32, 6
+    say(f"case: RBAS aging forced state n={n} m={m} k={k} alpha={alpha} (global-memory path)")
+    rng = np.random.default_rng(23)
+    xy = instances.uniform(n, 100000, seed=23)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=k, w=4, alpha=alpha, beta=2.0, rho=0.25, variant=capi.VARIANT_RBAS,
+                            aging=1, min_age=10, max_age=12, global_perf=5, min_pop=30, seed=98,
+                            construct_path=1)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, n))
+        age = rng.integers(0, 14, size=m).astype(np.int32)
+        valid = (rng.random(m) < 0.8).astype(np.uint8)
+        col.set_aging(age=age, valid=valid, perf=np.zeros(m, np.int32), prev_len=np.zeros(m, np.int64),
+                      scalars=np.array([int(valid.sum()), 1, 4, 17], np.int32))
+        col.set_best(10 ** 12, np.arange(n, dtype=np.int32))
+        for it in range(2):
+            snap = snapshot(col)
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+        listed = col.counters()[3]
+        assert (listed > 0) if alpha == 1.0 else (listed == 0)
+
+
 def test_top_lists_rebuild_after_pheromone_set_from_outside():
     """A row top list keeps only weights above (1 - rho) times the previous
     sweep's 256th weight.  Pheromone scaled down from outside breaks that

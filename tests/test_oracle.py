@@ -129,6 +129,25 @@ def test_nn_lists_brute_force(n, k, seed):
 
 
 # ----------------------------------------------------------------- heuristic
+def test_pow_alpha_against_fp64_power():
+    """tau^alpha (alpha != 1) is defined by a fixed fp64 operation sequence
+    shared with the kernels; it must agree with the fp64 power rounded to fp32
+    (numpy) to one ulp everywhere and exactly almost everywhere, give exact
+    results on powers of two, and x^0 = 1."""
+    rng = np.random.default_rng(0)
+    x = np.exp(rng.uniform(np.log(1e-30), np.log(1e3), 20000)).astype(np.float32)
+    for a in (0.5, 1.5, 2.0, 3.7):
+        ours = orc.pow_alpha(x, a)
+        ref = (x.astype(np.float64) ** np.float64(np.float32(a))).astype(np.float32)
+        ulps = np.abs(ours.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+        assert ulps.max() <= 1, a
+        assert (ulps == 0).mean() >= 0.9999, a
+    np.testing.assert_array_equal(orc.pow_alpha(x[:500], 0.0), np.ones(500, np.float32))
+    p2 = np.float32(2.0) ** np.arange(-40, 40).astype(np.float32)
+    np.testing.assert_array_equal(orc.pow_alpha(p2, 3.0), (p2.astype(np.float64) ** 3).astype(np.float32))
+    assert orc.pow_alpha(np.float32(0.0), 1.5)[0] == 0.0
+
+
 def test_eta_table_closed_form():
     for beta in (2.0, 3.0, 5.0):
         t = orc.eta_table(beta, 5000)
