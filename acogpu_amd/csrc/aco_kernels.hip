@@ -2392,6 +2392,26 @@ __global__ __launch_bounds__(1024) void kb_adj_scatter(const BatchBuffers b) {
 // the rows' nonzero sums added to tau with their loads issued together:
 // a few memory round trips per wave instead of several per row.  Other
 // waves take their rows one at a time.
+// Adds each lane's pending (target t, sum v) to the LDS row: the targets of
+// the first two leading lanes are summed over the whole wave first (one DPP
+// scan each) and added once, the rest with an LDS atomic per lane.  Once the
+// colony agrees on a row's neighbours most lanes hold the same target, whose
+// atomics would otherwise serialise on one LDS address.  Every lane calls it.
+__device__ __forceinline__ void combine_add(float* delta, int32_t t, float v, int lane) {
+#pragma unroll
+    for (int round = 0; round < 2; ++round) {
+        const unsigned long long act = __ballot(t >= 0);
+        if (!act) return;
+        const int lead = __ffsll((long long)act) - 1;
+        const int32_t tl = readlane_i(t, lead);
+        const bool mine = t == tl;
+        const float x = readlane_f(wave_inclusive_scan(mine ? v : 0.0f), 63);
+        if (lane == lead) atomicAdd(&delta[tl], x);
+        if (mine) t = -1;
+    }
+    if (t >= 0) atomicAdd(&delta[t], v);
+}
+
 template <int R>
 __global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b, int32_t nmax) {
     constexpr int GATHER_ROWS = R;
@@ -2424,16 +2444,22 @@ __global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b, int32
     wave_lds_sync();
     const size_t pitch = (size_t)adj_pitch(a.m);
     const int32_t nd = min(a.m, a.state->ndep);
-    for (int32_t a0 = 8 * lane; a0 < nd; a0 += 512) {
+    for (int32_t base = 0; base < nd; base += 512) {
+        const int32_t a0 = base + 8 * lane;
+        const bool act = a0 < nd;
         float d[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) d[q] = (a0 + q < nd) ? a.dep_val[a0 + q] : 0.0f;
+        for (int q = 0; q < 8; ++q) d[q] = (act && a0 + q < nd) ? a.dep_val[a0 + q] : 0.0f;
         uint4 v[GATHER_ROWS][2];
 #pragma unroll
         for (int r = 0; r < GATHER_ROWS; ++r) {
-            const uint16_t* row = a.adj + (size_t)(x0 + r) * pitch + 2 * a0;
-            v[r][0] = *reinterpret_cast<const uint4*>(row);
-            v[r][1] = *reinterpret_cast<const uint4*>(row + 8);
+            v[r][0] = make_uint4(0, 0, 0, 0);
+            v[r][1] = make_uint4(0, 0, 0, 0);
+            if (act) {
+                const uint16_t* row = a.adj + (size_t)(x0 + r) * pitch + 2 * a0;
+                v[r][0] = *reinterpret_cast<const uint4*>(row);
+                v[r][1] = *reinterpret_cast<const uint4*>(row + 8);
+            }
         }
 #pragma unroll
         for (int r = 0; r < GATHER_ROWS; ++r) {
@@ -2461,8 +2487,8 @@ __global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b, int32
                     ap = d[q];
                 }
             }
-            if (ts >= 0) atomicAdd(&delta[ts], as);
-            if (tp >= 0) atomicAdd(&delta[tp], ap);
+            combine_add(delta, ts, as, lane);
+            combine_add(delta, tp, ap, lane);
         }
     }
     wave_lds_sync();
