@@ -1785,56 +1785,84 @@ __device__ __forceinline__ void adj_scatter_staged(const DepositArgs& a, int32_t
     }
 }
 
+// Adds each lane's pending (target t, sum v) to the LDS row: the targets of
+// the first two leading lanes are summed over the whole wave first (one DPP
+// scan each) and added once, the rest with an LDS atomic per lane.  Once the
+// colony agrees on a row's neighbours most lanes hold the same target, whose
+// atomics would otherwise serialise on one LDS address.  Every lane calls it.
+__device__ __forceinline__ void combine_add(float* delta, int32_t t, float v, int lane) {
+#pragma unroll
+    for (int round = 0; round < 2; ++round) {
+        const unsigned long long act = __ballot(t >= 0);
+        if (!act) return;
+        const int lead = __ffsll((long long)act) - 1;
+        const int32_t tl = readlane_i(t, lead);
+        const bool mine = t == tl;
+        const float x = readlane_f(wave_inclusive_scan(mine ? v : 0.0f), 63);
+        if (lane == lead) atomicAdd(&delta[tl], x);
+        if (mine) t = -1;
+    }
+    if (t >= 0) atomicAdd(&delta[t], v);
+}
+
 // Pass 2, one row x per `nthr` threads (thread t of the row): the row's
 // slots are summed into the LDS row `delta` (zero on entry, zero again on
 // return) by their ants' dep_val (slots of ants with dep_val 0 are stale and
 // skipped), then added once to tau_next where nonzero.  Each thread reads
-// the slots of 8 consecutive ants as two 16-byte loads and runs its
-// successors and its predecessors into one pending sum each, which it adds
-// to LDS only when the target changes: once the colony agrees on a row's
-// neighbours, most of the row's LDS atomics disappear.  (Float sums then
-// associate differently from a per-slot order; both orders are within the
-// fp32 rounding the deposit's tests allow.)
+// the slots of 8 consecutive ants as two 16-byte loads and sums their 16
+// targets (successors and predecessors alike) into two (target, sum)
+// accumulators, a third target going straight to LDS; the accumulators are
+// then combined over the wave (combine_add).  Once the colony agrees on a
+// row's two neighbours, almost every deposit lands in the accumulators and
+// the row takes a few LDS atomics per wave.  (Float sums then associate
+// differently from a per-slot order; both orders are within the fp32
+// rounding the deposit's tests allow.)  Every lane of a wave calls it.
 __device__ __forceinline__ void adj_gather_row(const DepositArgs& a, int32_t x, float* delta, int t, int nthr) {
     const uint16_t* row = a.adj + (size_t)x * adj_pitch(a.m);
     const int32_t nd = min(a.m, a.state->ndep);
-    for (int32_t a0 = 8 * t; a0 < nd; a0 += 8 * nthr) {
-        uint16_t sl[16];
-        const uint4 v0 = *reinterpret_cast<const uint4*>(row + 2 * a0);
-        const uint4 v1 = *reinterpret_cast<const uint4*>(row + 2 * a0 + 8);
+    const int lane = t & 63;
+    const int32_t wave0 = t - lane;  // nthr is a multiple of 64: whole waves share each pass
+    for (int32_t base = 0; base < nd; base += 8 * nthr) {
+        if (base + 8 * wave0 >= nd) break;  // no ant of this wave in this pass (wave-uniform)
+        const int32_t a0 = base + 8 * t;
+        const bool act = a0 < nd;
+        uint4 v0 = make_uint4(0, 0, 0, 0), v1 = make_uint4(0, 0, 0, 0);
+        if (act) {
+            v0 = *reinterpret_cast<const uint4*>(row + 2 * a0);
+            v1 = *reinterpret_cast<const uint4*>(row + 2 * a0 + 8);
+        }
+        float d[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) d[q] = (act && a0 + q < nd) ? a.dep_val[a0 + q] : 0.0f;
         const uint32_t w32[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        int32_t t1 = -1, t2 = -1;
+        float s1 = 0.0f, s2 = 0.0f;
+        auto put = [&](int32_t tg, float v) {
+            if (tg == t1) {
+                s1 += v;
+            } else if (tg == t2) {
+                s2 += v;
+            } else if (t1 < 0) {
+                t1 = tg;
+                s1 = v;
+            } else if (t2 < 0) {
+                t2 = tg;
+                s2 = v;
+            } else {
+                atomicAdd(&delta[tg], v);
+            }
+        };
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            sl[2 * q] = (uint16_t)(w32[q] & 0xffffu);
-            sl[2 * q + 1] = (uint16_t)(w32[q] >> 16);
+            if (d[q] == 0.0f) continue;  // past nd, or an ant without a deposit: stale slots
+            put((int32_t)(w32[q] & 0xffffu), d[q]);
+            put((int32_t)(w32[q] >> 16), d[q]);
         }
-        int32_t ts = -1, tp = -1;
-        float as = 0.0f, ap = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            if (a0 + q >= nd) break;
-            const float d = a.dep_val[a0 + q];
-            if (d == 0.0f) continue;
-            const int32_t su = sl[2 * q], pr = sl[2 * q + 1];
-            if (su == ts) {
-                as += d;
-            } else {
-                if (ts >= 0) atomicAdd(&delta[ts], as);
-                ts = su;
-                as = d;
-            }
-            if (pr == tp) {
-                ap += d;
-            } else {
-                if (tp >= 0) atomicAdd(&delta[tp], ap);
-                tp = pr;
-                ap = d;
-            }
-        }
-        if (ts >= 0) atomicAdd(&delta[ts], as);
-        if (tp >= 0) atomicAdd(&delta[tp], ap);
+        combine_add(delta, t1, s1, lane);
+        combine_add(delta, t2, s2, lane);
     }
 }
+
 
 // Second half of pass 2: the row's sums go to tau_next and the LDS row is
 // cleared for the next row.
@@ -2392,26 +2420,6 @@ __global__ __launch_bounds__(1024) void kb_adj_scatter(const BatchBuffers b) {
 // the rows' nonzero sums added to tau with their loads issued together:
 // a few memory round trips per wave instead of several per row.  Other
 // waves take their rows one at a time.
-// Adds each lane's pending (target t, sum v) to the LDS row: the targets of
-// the first two leading lanes are summed over the whole wave first (one DPP
-// scan each) and added once, the rest with an LDS atomic per lane.  Once the
-// colony agrees on a row's neighbours most lanes hold the same target, whose
-// atomics would otherwise serialise on one LDS address.  Every lane calls it.
-__device__ __forceinline__ void combine_add(float* delta, int32_t t, float v, int lane) {
-#pragma unroll
-    for (int round = 0; round < 2; ++round) {
-        const unsigned long long act = __ballot(t >= 0);
-        if (!act) return;
-        const int lead = __ffsll((long long)act) - 1;
-        const int32_t tl = readlane_i(t, lead);
-        const bool mine = t == tl;
-        const float x = readlane_f(wave_inclusive_scan(mine ? v : 0.0f), 63);
-        if (lane == lead) atomicAdd(&delta[tl], x);
-        if (mine) t = -1;
-    }
-    if (t >= 0) atomicAdd(&delta[t], v);
-}
-
 template <int R>
 __global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b, int32_t nmax) {
     constexpr int GATHER_ROWS = R;
@@ -2466,29 +2474,35 @@ __global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b, int32
             const uint32_t w32[8] = {v[r][0].x, v[r][0].y, v[r][0].z, v[r][0].w,
                                      v[r][1].x, v[r][1].y, v[r][1].z, v[r][1].w};
             float* delta = dl + (size_t)r * nmax;
-            int32_t ts = -1, tp = -1;
-            float as = 0.0f, ap = 0.0f;
+            // two (target, sum) accumulators over the lane's 16 deposits
+            // (successor and predecessor of 8 ants): the row's two tour
+            // neighbours once the colony agrees, whichever direction each
+            // ant travelled; a third target goes straight to LDS
+            int32_t t1 = -1, t2 = -1;
+            float s1 = 0.0f, s2 = 0.0f;
+            auto put = [&](int32_t t, float x) {
+                if (t == t1) {
+                    s1 += x;
+                } else if (t == t2) {
+                    s2 += x;
+                } else if (t1 < 0) {
+                    t1 = t;
+                    s1 = x;
+                } else if (t2 < 0) {
+                    t2 = t;
+                    s2 = x;
+                } else {
+                    atomicAdd(&delta[t], x);
+                }
+            };
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 if (d[q] == 0.0f) continue;  // past nd, or an ant without a deposit: stale slots
-                const int32_t su = (int32_t)(w32[q] & 0xffffu), pr = (int32_t)(w32[q] >> 16);
-                if (su == ts) {
-                    as += d[q];
-                } else {
-                    if (ts >= 0) atomicAdd(&delta[ts], as);
-                    ts = su;
-                    as = d[q];
-                }
-                if (pr == tp) {
-                    ap += d[q];
-                } else {
-                    if (tp >= 0) atomicAdd(&delta[tp], ap);
-                    tp = pr;
-                    ap = d[q];
-                }
+                put((int32_t)(w32[q] & 0xffffu), d[q]);
+                put((int32_t)(w32[q] >> 16), d[q]);
             }
-            combine_add(delta, ts, as, lane);
-            combine_add(delta, tp, ap, lane);
+            combine_add(delta, t1, s1, lane);
+            combine_add(delta, t2, s2, lane);
         }
     }
     wave_lds_sync();
