@@ -669,6 +669,36 @@ __device__ __forceinline__ void store_tour_length(const ConstructArgs& a, const 
     if (lane == 0) a.lengths[ant] = len;
 }
 
+// One roulette step: the first lane whose inclusive prefix of the weights w
+// (0 for visited or absent candidates) exceeds u * total and whose weight is
+// positive gives the next city j (selectCities src/Colony.cu:821-851); with
+// no such lane the last positive lane (u * total rounded up to the total),
+// and with a zero total the fallback's city.  Ballots come straight from
+// the compare masks; the common case falls through and the rare ones are
+// laid apart.
+template <bool K32, typename Fallback>
+__device__ __forceinline__ int32_t roulette_pick(float w, int32_t j, float u, Fallback&& fallback) {
+    const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
+    const float p = K32 ? wave_inclusive_scan32(w) : wave_inclusive_scan(w);
+    // u * total read from the last lane of p * u: one lane read instead of a
+    // read and a move before the multiply, the same product.  The empty asm
+    // keeps the compiler from folding the multiply back behind the read.
+    float pu = p * u;
+    asm volatile("" : "+v"(pu));
+    const float rr = readlane_f(pu, K32 ? 31 : 63);
+    const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
+    // s_ff1 gives the lowest set bit, -1 for no hit (the lane read then
+    // selects lane 63 and is replaced below)
+    int32_t first;
+    asm("s_ff1_i32_b64 %0, %1" : "=s"(first) : "s"(hit));
+    int32_t next = readlane_i(j, first);
+    if (__builtin_expect(hit == 0ull, 0)) {
+        const float total = readlane_f(p, K32 ? 31 : 63);
+        next = total > 0.0f ? readlane_i(j, 63 - __clzll((long long)pos)) : fallback();
+    }
+    return next;
+}
+
 // ---------------------------------------------------------------------------
 // Tour construction, candidate-list mode, global-memory variant (any n).
 // One ant per wave64 for all n-1 steps (the reference launches 3(n-1)
@@ -708,28 +738,24 @@ __device__ __forceinline__ void construct_cand_ant(const ConstructArgs& a, int32
     float cw;
     load_row(start, j, cw);
     unsigned long long fb = 0, scanned = 0, listed = 0;
-    for (int32_t s = 1; s < a.n; ++s) {
-        rng_step(rng, s, (uint32_t)ant, iter, a, lane);
-        const float u = word_to_uniform(rng_word(rng, s));
-        const float w = is_visited(vis, j) ? 0.0f : cw;
-        const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
-        const float p = K32 ? wave_inclusive_scan32(w) : wave_inclusive_scan(w);
-        const float total = readlane_f(p, K32 ? 31 : 63);
-        const float rr = u * total;
-        const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
-        int32_t next;
-        if (__builtin_expect(hit != 0ull, 1)) {
-            next = readlane_i(j, __ffsll((long long)hit) - 1);
-        } else if (total > 0.0f) {
-            next = readlane_i(j, 63 - __clzll((long long)pos));
-        } else {
-            ++fb;
-            next = fallback_argmax(a, vis, c, young, ratio, lane, scanned, listed);
+    // steps in chunks of 64 (see construct_staged_ant)
+    for (int32_t s0 = 0; s0 < a.n; s0 += 64) {
+        if (s0 > 0) rng_step(rng, s0, (uint32_t)ant, iter, a, lane);
+        const float uc = word_to_uniform(rng.cur);
+        const int32_t s_end = min(s0 + 64, a.n);
+        for (int32_t s = s0 > 0 ? s0 : 1; s < s_end; ++s) {
+            const float u = readlane_f(uc, s & 63);
+            const float w = is_visited(vis, j) ? 0.0f : cw;
+            const int32_t next = roulette_pick<K32>(w, j, u, [&]() {
+                ++fb;
+                return fallback_argmax(a, vis, c, young, ratio, lane, scanned, listed);
+            });
+            load_row(next, j, cw);
+            mark_visited(vis, next, lane);
+            tbuf = (lane == s - s0) ? next : tbuf;
+            c = next;
         }
-        load_row(next, j, cw);
-        mark_visited(vis, next, lane);
-        stage_city(tour, tbuf, s, next, a.n, lane);
-        c = next;
+        if (lane < s_end - s0) tour[s0 + lane] = tbuf;
     }
     finish_counters(a, fb, scanned, lane, listed);
     store_tour_length(a, tour, ant, lane);
@@ -925,55 +951,46 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
     const uint32_t kk2 = 2u * kk, kk4 = 4u * kk;
     int32_t j = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)c, kk2));
     float cw = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)c, kk4));
-    for (int32_t s = 1; s < a.n; ++s) {
-        rng_step(rng, s, (uint32_t)ant, iter, a, lane);
-        const float u = word_to_uniform(rng_word(rng, s));
-        __builtin_assume((uint32_t)j < 65536u);  // a 16-bit LDS entry
-        float w;
-        if (VREG_VIS) {
-            // ds_bpermute takes a byte address and drops its two low bits:
-            // j >> 3 addresses lane j >> 5, the owner of j's bitmap word; the
-            // signed bit field extract of bit j mod 32 gives an all-ones mask
-            // for a visited city, which clears the weight's bits
-            const int32_t word = __builtin_amdgcn_ds_bpermute(j >> 3, (int)vreg);
-            const int32_t vmask = __builtin_amdgcn_sbfe(word, (uint32_t)j, 1u);
-            w = __int_as_float(__float_as_int(cw) & ~vmask);
-        } else {
-            w = is_visited(vis_lds, j) ? 0.0f : cw;
+    // steps in chunks of 64: a chunk's random words sit one per lane in
+    // rng.cur and its cities one per lane in tbuf, so the step itself tests
+    // neither the refill nor the tour flush
+    for (int32_t s0 = 0; s0 < a.n; s0 += 64) {
+        if (s0 > 0) rng_step(rng, s0, (uint32_t)ant, iter, a, lane);
+        const float uc = word_to_uniform(rng.cur);  // the chunk's uniforms, one per lane
+        const int32_t s_end = min(s0 + 64, a.n);
+        for (int32_t s = s0 > 0 ? s0 : 1; s < s_end; ++s) {
+            const float u = readlane_f(uc, s & 63);
+            __builtin_assume((uint32_t)j < 65536u);  // a 16-bit LDS entry
+            float w;
+            if (VREG_VIS) {
+                // ds_bpermute takes a byte address and drops its two low bits:
+                // j >> 3 addresses lane j >> 5, the owner of j's bitmap word; the
+                // signed bit field extract of bit j mod 32 gives an all-ones mask
+                // for a visited city, which clears the weight's bits
+                const int32_t word = __builtin_amdgcn_ds_bpermute(j >> 3, (int)vreg);
+                const int32_t vmask = __builtin_amdgcn_sbfe(word, (uint32_t)j, 1u);
+                w = __int_as_float(__float_as_int(cw) & ~vmask);
+            } else {
+                w = is_visited(vis_lds, j) ? 0.0f : cw;
+            }
+            const int32_t next = roulette_pick<K32>(w, j, u, [&]() {
+                ++fb;
+                if (!VREG_VIS) return fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned, listed);
+                return fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned);
+            });
+            const int32_t jn = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)next, kk2));
+            const float cwn = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)next, kk4));
+            if (VREG_VIS) {
+                if (lane == (next >> 5)) vreg |= 1u << (next & 31);
+            } else {
+                mark_visited(vis_lds, next, lane);
+            }
+            tbuf = (lane == s - s0) ? next : tbuf;
+            c = next;
+            j = jn;
+            cw = cwn;
         }
-        // ballots straight from the compare masks (no round trip through a
-        // VGPR); the common case falls through, the rare ones are laid apart
-        const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
-        const float p = K32 ? wave_inclusive_scan32(w) : wave_inclusive_scan(w);
-        const float total = readlane_f(p, K32 ? 31 : 63);
-        const float rr = u * total;
-        const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
-        int32_t next;
-        if (__builtin_expect(hit != 0ull, 1)) {
-            next = readlane_i(j, __ffsll((long long)hit) - 1);
-        } else if (total > 0.0f) {
-            next = readlane_i(j, 63 - __clzll((long long)pos));
-        } else {
-            ++fb;
-            if (!VREG_VIS)
-                next = fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned, listed);
-            else
-                next = fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned);
-        }
-        const int32_t jn = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)next, kk2));
-        const float cwn = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)next, kk4));
-        if (VREG_VIS) {
-            vreg |= (lane == (next >> 5)) ? (1u << (next & 31)) : 0u;
-        } else {
-            mark_visited(vis_lds, next, lane);
-        }
-        tbuf = (lane == (s & 63)) ? next : tbuf;
-        if (__builtin_expect((s & 63) == 63 || s == a.n - 1, 0)) {
-            if (lane <= (s & 63)) t16[(s & ~63) + lane] = (uint16_t)tbuf;
-        }
-        c = next;
-        j = jn;
-        cw = cwn;
+        if (lane < s_end - s0) t16[s0 + lane] = (uint16_t)tbuf;
     }
     finish_counters(a, fb, scanned, lane, listed);
     // tour out (coalesced) and exact length, edge i -> i+1 per lane
