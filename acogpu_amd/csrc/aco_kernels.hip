@@ -968,10 +968,14 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
                 // signed bit field extract of bit j mod 32 gives an all-ones mask
                 // for a visited city, which clears the weight's bits
                 const int32_t word = __builtin_amdgcn_ds_bpermute(j >> 3, (int)vreg);
+                // the current city joins the set only now, while the lane
+                // read is in flight: it is none of its own candidates
+                if (lane == (c >> 5)) vreg |= 1u << (c & 31);
                 const int32_t vmask = __builtin_amdgcn_sbfe(word, (uint32_t)j, 1u);
                 w = __int_as_float(__float_as_int(cw) & ~vmask);
             } else {
                 w = is_visited(vis_lds, j) ? 0.0f : cw;
+                mark_visited(vis_lds, c, lane);  // likewise after the read
             }
             const int32_t next = roulette_pick<K32>(w, j, u, [&]() {
                 ++fb;
@@ -980,11 +984,6 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             });
             const int32_t jn = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)next, kk2));
             const float cwn = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)next, kk4));
-            if (VREG_VIS) {
-                if (lane == (next >> 5)) vreg |= 1u << (next & 31);
-            } else {
-                mark_visited(vis_lds, next, lane);
-            }
             tbuf = (lane == s - s0) ? next : tbuf;
             c = next;
             j = jn;
