@@ -975,6 +975,7 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
                 w = __int_as_float(__float_as_int(cw) & ~vmask);
             } else {
                 w = is_visited(vis_lds, j) ? 0.0f : cw;
+                mark_visited(vis_lds, c, lane);  // likewise after the read
             }
             const int32_t next = roulette_pick<K32>(w, j, u, [&]() {
                 ++fb;
@@ -983,7 +984,6 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             });
             const int32_t jn = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)next, kk2));
             const float cwn = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)next, kk4));
-            if (!VREG_VIS) mark_visited(vis_lds, next, lane);  // off the step's chain here
             tbuf = (lane == s - s0) ? next : tbuf;
             c = next;
             j = jn;
