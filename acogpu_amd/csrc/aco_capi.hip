@@ -35,6 +35,7 @@ struct aco_ctx {
     int32_t* d_nn = nullptr;
     uint16_t* d_nn16 = nullptr;
     float* d_eta = nullptr;
+    float* d_eta_full = nullptr;  // aging: heuristic of every edge, pitched (young ants' fallback rows)
     int32_t eta_len = 0;
     int32_t eta_pow = 0;  // aco::eta_integer_power of (beta, largest distance)
     float* d_mean = nullptr;
@@ -114,7 +115,7 @@ static void free_all(aco_ctx* c) {
                     c->d_tau_nn, c->d_eta_nn, c->d_nn16,
                     c->d_mean, c->d_tours, c->d_len, c->d_prev_len, c->d_age, c->d_perf, c->d_valid,
                     c->d_best_tour, c->d_dep_ant, c->d_dep_val, c->d_state, c->d_scratch64, c->d_adj,
-                    c->d_top, c->d_top_thr};
+                    c->d_top, c->d_top_thr, c->d_eta_full};
     for (void* q : ptrs)
         if (q) hipFree(q);
     for (auto& e : c->ev)
@@ -336,6 +337,13 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     }
     TRYB(dalloc(c, &c->d_eta, (size_t)c->eta_len));
     HIPB(hipMemcpy(c->d_eta, eta.data(), sizeof(float) * eta.size(), hipMemcpyHostToDevice));
+    // Aging colonies: young ants weigh a fallback row's every city with the
+    // heuristic, a gather per city from the table; a pitched copy of the
+    // table's values turns that into row loads (one memory round trip)
+    if (c->p.aging && (size_t)c->n * c->P <= aco::ETA_FULL_MAX) {
+        TRYB(dalloc(c, &c->d_eta_full, (size_t)c->n * c->P));
+        HIPB(aco::launch_eta_full(c->d_dist, c->d_eta, c->eta_len, c->n, c->P, c->d_eta_full, c->stream));
+    }
     HIPB(hipDeviceSynchronize());
     *out = c;
     return ACO_OK;
@@ -475,6 +483,7 @@ static hipError_t enqueue_construct(aco_ctx* c, int cur) {
     ca.tau = tau;
     ca.mean = c->d_mean;
     ca.eta = c->d_eta;
+    ca.eta_full = c->d_eta_full;
     ca.eta_len = c->eta_len;
     ca.tours = c->d_tours;
     ca.lengths = c->d_len;

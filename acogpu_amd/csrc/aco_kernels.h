@@ -86,6 +86,7 @@ struct ConstructArgs {
     int32_t words;           // visited-bitmap words per ant
     int32_t young_blocks;    // trailing blocks reserved for young aging ants (staged path)
     const uint16_t* top;     // n x TOP_K row top lists (null: none)
+    const float* eta_full;   // pitched heuristic of every edge (aging: young ants' fallback rows; null: gathered)
 };
 
 struct StatsArgs {
@@ -240,6 +241,11 @@ int32_t batch_construct_blocks_per_cu(int32_t nmax, int32_t ants_per_block, int3
                                       size_t lds_bytes);
 
 hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s);
+// out (pitched n x P) = the heuristic table's value of every edge's distance
+hipError_t launch_eta_full(const int32_t* dist, const float* eta, int32_t eta_len, int32_t n, int32_t P, float* out,
+                           hipStream_t s);
+// Largest n x P for which aging colonies keep the heuristic of every edge.
+constexpr size_t ETA_FULL_MAX = (size_t)1 << 26;
 hipError_t launch_fix_diagonal(int32_t* dist, int32_t n, int32_t P, hipStream_t s);
 hipError_t launch_dist_max(const int32_t* dist, int32_t n, int32_t P, int32_t* out, hipStream_t s);
 hipError_t launch_nn_lists(const int32_t* dist, int32_t n, int32_t P, int32_t k, int32_t* nn, hipStream_t s);

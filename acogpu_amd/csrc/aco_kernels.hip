@@ -104,6 +104,15 @@ __global__ __launch_bounds__(256) void k_dist_from_xy(const int32_t* __restrict_
     dist[(size_t)i * P + j] = d;
 }
 
+__global__ __launch_bounds__(256) void k_eta_full(const int32_t* __restrict__ dist, const float* __restrict__ eta,
+                                                  int32_t eta_len, int32_t n, int32_t P, float* __restrict__ out) {
+    const int32_t i = blockIdx.y;
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= P) return;
+    const size_t x = (size_t)i * P + j;
+    out[x] = j < n ? eta_of(eta, eta_len, dist[x]) : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void k_fix_diagonal(int32_t* __restrict__ dist, int32_t n, int32_t P) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) dist[(size_t)i * P + i] = ACO_MAX_DISTANCE;
@@ -459,7 +468,8 @@ __device__ __forceinline__ float young_formula(float tau, float mc, float ratio,
 
 __device__ __forceinline__ float young_weight(const ConstructArgs& a, int32_t c, int32_t j, float ratio) {
     const size_t xy = (size_t)c * a.P + j;
-    return young_formula(a.tau[xy], a.mean[c], ratio, eta_of(a.eta, a.eta_len, a.dist[xy]), a.alpha);
+    const float e = a.eta_full != nullptr ? a.eta_full[xy] : eta_of(a.eta, a.eta_len, a.dist[xy]);
+    return young_formula(a.tau[xy], a.mean[c], ratio, e, a.alpha);
 }
 
 __device__ __forceinline__ bool is_visited(const uint32_t* vis, int32_t j) {
@@ -821,29 +831,46 @@ __device__ __forceinline__ int32_t fallback_argmax_vreg(const ConstructArgs& a, 
                 }
             }
         } else {
-            // all loads first (16-byte tau and dist, then the heuristic
-            // gathers), then the pulled-toward-the-mean weights
+            // all loads first (16-byte tau and heuristic rows, or tau and
+            // dist followed by the heuristic gathers), then the
+            // pulled-toward-the-mean weights
             const float mc = a.mean[c];
-            float t[32];
-            int32_t d[32];
+            float t[32], e[32];
             const float4* t4 = reinterpret_cast<const float4*>(a.tau + row);
-            const int4* d4 = reinterpret_cast<const int4*>(a.dist + row);
+            if (a.eta_full != nullptr) {  // one round trip
+                const float4* e4 = reinterpret_cast<const float4*>(a.eta_full + row);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const float4 tv = t4[q];
-                const int4 dv = d4[q];
-                t[4 * q] = tv.x;
-                t[4 * q + 1] = tv.y;
-                t[4 * q + 2] = tv.z;
-                t[4 * q + 3] = tv.w;
-                d[4 * q] = dv.x;
-                d[4 * q + 1] = dv.y;
-                d[4 * q + 2] = dv.z;
-                d[4 * q + 3] = dv.w;
+                for (int q = 0; q < 8; ++q) {
+                    const float4 tv = t4[q];
+                    const float4 ev = e4[q];
+                    t[4 * q] = tv.x;
+                    t[4 * q + 1] = tv.y;
+                    t[4 * q + 2] = tv.z;
+                    t[4 * q + 3] = tv.w;
+                    e[4 * q] = ev.x;
+                    e[4 * q + 1] = ev.y;
+                    e[4 * q + 2] = ev.z;
+                    e[4 * q + 3] = ev.w;
+                }
+            } else {
+                int32_t d[32];
+                const int4* d4 = reinterpret_cast<const int4*>(a.dist + row);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float4 tv = t4[q];
+                    const int4 dv = d4[q];
+                    t[4 * q] = tv.x;
+                    t[4 * q + 1] = tv.y;
+                    t[4 * q + 2] = tv.z;
+                    t[4 * q + 3] = tv.w;
+                    d[4 * q] = dv.x;
+                    d[4 * q + 1] = dv.y;
+                    d[4 * q + 2] = dv.z;
+                    d[4 * q + 3] = dv.w;
+                }
+#pragma unroll
+                for (int q = 0; q < 32; ++q) e[q] = eta_of(a.eta, a.eta_len, d[q]);
             }
-            float e[32];
-#pragma unroll
-            for (int q = 0; q < 32; ++q) e[q] = eta_of(a.eta, a.eta_len, d[q]);
 #pragma unroll
             for (int q = 0; q < 32; ++q) {
                 if ((free_bits >> q) & 1u) {
@@ -2554,6 +2581,12 @@ static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1
 
 hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s) {
     hipLaunchKernelGGL(k_dist_from_xy, dim3(cdiv(P, 256), n), dim3(256), 0, s, xy, n, P, dist);
+    return hipGetLastError();
+}
+
+hipError_t launch_eta_full(const int32_t* dist, const float* eta, int32_t eta_len, int32_t n, int32_t P, float* out,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_eta_full, dim3(cdiv(P, 256), n), dim3(256), 0, s, dist, eta, eta_len, n, P, out);
     return hipGetLastError();
 }
 

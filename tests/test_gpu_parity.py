@@ -81,6 +81,25 @@ def test_as_iteration_replay(n, m, k, alpha, beta, rho, seed, path):
 
 
 @pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("n", [127, 128, 129, 192])
+def test_step_chunk_boundaries(n, path):
+    """The construction runs its steps in chunks of 64 (a chunk's random words
+    and tour entries one per lane): tours that end just before, at and just
+    after a chunk boundary."""
+    say(f"case: AS chunk boundary n={n} path={path}")
+    rng = np.random.default_rng(n)
+    xy = instances.uniform(n, 1000, seed=300 + n)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=40, k=12, beta=2.0, rho=0.3, seed=900 + n, construct_path=path)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, n))
+        for it in range(2):
+            snap = snapshot(col)
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+
+
+@pytest.mark.parametrize("path", PATHS)
 def test_as_iteration_replay_fallback_heavy(path):
     say(f"case: AS candidate n=400 m=48 k=4 (fallback heavy) path={path}")
     xy = instances.uniform(400, 2000, seed=9)
