@@ -440,6 +440,7 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur) {
     sa.tau = tau;
     sa.dist = c->d_dist;
     sa.eta = c->d_eta;
+    sa.eta_full = c->d_eta_full;
     sa.eta_len = c->eta_len;
     sa.nn = cand ? c->d_nn : nullptr;
     sa.k = c->k;

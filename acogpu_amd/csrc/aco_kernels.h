@@ -60,6 +60,7 @@ struct SweepArgs {
     DevState* state;         // single colony: halting flags (null in a batch)
     uint16_t* top;           // n x TOP_K row top lists (null: none)
     float* top_thr;          // n: the TOP_K-th weight of the previous sweep
+    const float* eta_full;   // pitched heuristic of every edge (read instead of dist; may be null)
 };
 
 struct ConstructArgs {

@@ -357,32 +357,43 @@ __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lan
         // >= fl(fl(w (1 - rho)) (1 - 2^-20)) for the unit roundoff u = 2^-24
         top.thr = (a.top_thr[i] * a.one_minus_rho) * 0.99999904632568359375f;
     }
+    const float* __restrict__ erow = a.eta_full != nullptr ? a.eta_full + row : nullptr;
     for (int32_t g0 = 0; g0 < a.n; g0 += 1024) {
         float4 t[4];
         int4 d[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int32_t j0 = g0 + 256 * s + 4 * lane;
-            if (j0 < a.n) {
-                t[s] = *reinterpret_cast<const float4*>(trow + j0);
-                d[s] = *reinterpret_cast<const int4*>(drow + j0);
-            } else {
-                t[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-                d[s] = make_int4(0, 0, 0, 0);
-            }
-        }
         float e[4][4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const int32_t dv[4] = {d[s].x, d[s].y, d[s].z, d[s].w};
+            const int32_t j0 = g0 + 256 * s + 4 * lane;
+            t[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+            d[s] = make_int4(0, 0, 0, 0);
+            e[s][0] = e[s][1] = e[s][2] = e[s][3] = 0.0f;
+            if (j0 < a.n) {
+                t[s] = *reinterpret_cast<const float4*>(trow + j0);
+                if (erow != nullptr) {
+                    const float4 ev = *reinterpret_cast<const float4*>(erow + j0);
+                    e[s][0] = ev.x;
+                    e[s][1] = ev.y;
+                    e[s][2] = ev.z;
+                    e[s][3] = ev.w;
+                } else {
+                    d[s] = *reinterpret_cast<const int4*>(drow + j0);
+                }
+            }
+        }
+        if (erow == nullptr) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                // 1 / d^beta computed (the table's value) rather than
-                // gathered: the gathers were most of the sweep's L2 requests
-                if (a.eta_pow)
-                    e[s][q] = (uint32_t)dv[q] < (uint32_t)a.eta_len ? eta_power_value(a.eta_pow, dv[q]) : 0.0f;
-                else
-                    e[s][q] = eta_of(a.eta, a.eta_len, dv[q]);
+            for (int s = 0; s < 4; ++s) {
+                const int32_t dv[4] = {d[s].x, d[s].y, d[s].z, d[s].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    // 1 / d^beta computed (the table's value) rather than
+                    // gathered: the gathers were most of the sweep's L2 requests
+                    if (a.eta_pow)
+                        e[s][q] = (uint32_t)dv[q] < (uint32_t)a.eta_len ? eta_power_value(a.eta_pow, dv[q]) : 0.0f;
+                    else
+                        e[s][q] = eta_of(a.eta, a.eta_len, dv[q]);
+                }
             }
         }
 #pragma unroll
