@@ -2278,71 +2278,76 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
     int32_t pend = 0;
     unsigned long long fb = 0, scanned = 0;
     int32_t* tour = a.tours + (size_t)ant * a.n;
-    for (int32_t s = 1; s < a.n; ++s) {
-        if (__builtin_expect((s & (L - 1)) == 0, 0)) {
-            if ((s & 255) == 0)
-                refill(s);
+    // steps in L-step segments: the uniform refill or advance comes before
+    // a segment and the tour write after it, not tested at every step
+    for (int32_t s0 = 0; s0 < a.n; s0 += L) {
+        if (s0 > 0) {
+            if ((s0 & 255) == 0)
+                refill(s0);
             else
                 advance();
         }
-        const int32_t j = cj.x;
-        const float cw = __int_as_float(cj.y);
-        const float u = __int_as_float(
-            __builtin_amdgcn_ds_bpermute((gb << 2) + ((s & (L - 1)) << 2), __float_as_int(q[0])));
-        // j >> 3 addresses lane j >> 5 of the group (bpermute drops the two
-        // low address bits; gb * 4 and j >> 3 share no bits for n <= 32 L);
-        // the bit field extract reads bit j mod 32
-        // (the signed extract of bit j mod 32 is an all-ones mask for a
-        // visited city, which clears the weight's bits)
-        const int32_t word = __builtin_amdgcn_ds_bpermute((gb << 2) + (j >> 3), (int)vreg);
-        const int32_t vmask = __builtin_amdgcn_sbfe(word, (uint32_t)j, 1u);
-        const float w = __int_as_float(__float_as_int(cw) & ~vmask);
-        const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
-        const float p = group_scan<G>(w);
-        const float tot = group_last<G>(p, lane);
-        const float rr = u * tot;
-        const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
-        // first hit of the group; no hit gives L (another group's lane, the
-        // result is then replaced below)
-        int32_t f;
-        if constexpr (L == 16)
-            f = __builtin_ctz(((uint32_t)(hit >> gb) & 0xffffu) | 0x10000u);
-        else
-            f = __builtin_ctzll((hit >> gb) | (1ull << 32));
-        int32_t next = bperm_i(gb + f, j);
-        const unsigned long long nohit = __builtin_amdgcn_ballot_w64(f == L);
-        if (__builtin_expect(nohit != 0ull, 0)) {
-#pragma unroll 1
-            for (int gg = 0; gg < G; ++gg) {
-                if (!((nohit >> (gg * L)) & 1ull)) continue;
-                const float tg = readlane_f(tot, gg * L);
-                const uint32_t pg = (uint32_t)((pos >> (gg * L)) & (L == 32 ? 0xffffffffull : 0xffffull));
-                int32_t ng;
-                if (tg > 0.0f && pg != 0u) {  // rounding: the group's last positive lane
-                    ng = readlane_i(j, gg * L + 31 - __clz((int)pg));
-                } else {
-                    const bool vg = ant0 + gg < a.m;
-                    fb += vg ? 1 : 0;
-                    unsigned long long sc = 0;
-                    if constexpr (G == 4) {  // eight cities per lane over the whole wave
-                        ng = fallback_argmax_spread<8>(a, vreg, gg * L, readlane_i(c, gg * L), lane, sc);
-                    } else {  // 32 cities on each of the group's lanes' words
-                        const uint32_t vword = (uint32_t)bperm_i((gg * L + lane) & 63, (int32_t)vreg);
-                        ng = fallback_argmax_vreg(a, vword, readlane_i(c, gg * L), false, 0.0f, lane, sc);
+        const int32_t s_end = min(s0 + L, a.n);
+        for (int32_t s = s0 > 0 ? s0 : 1; s < s_end; ++s) {
+            const int32_t j = cj.x;
+            const float cw = __int_as_float(cj.y);
+            const float u = __int_as_float(
+                __builtin_amdgcn_ds_bpermute((gb << 2) + ((s & (L - 1)) << 2), __float_as_int(q[0])));
+            // j >> 3 addresses lane j >> 5 of the group (bpermute drops the two
+            // low address bits; gb * 4 and j >> 3 share no bits for n <= 32 L);
+            // the bit field extract reads bit j mod 32
+            // (the signed extract of bit j mod 32 is an all-ones mask for a
+            // visited city, which clears the weight's bits)
+            const int32_t word = __builtin_amdgcn_ds_bpermute((gb << 2) + (j >> 3), (int)vreg);
+            const int32_t vmask = __builtin_amdgcn_sbfe(word, (uint32_t)j, 1u);
+            const float w = __int_as_float(__float_as_int(cw) & ~vmask);
+            const unsigned long long pos = __builtin_amdgcn_ballot_w64(w > 0.0f);
+            const float p = group_scan<G>(w);
+            const float tot = group_last<G>(p, lane);
+            const float rr = u * tot;
+            const unsigned long long hit = __builtin_amdgcn_ballot_w64(p > rr) & pos;
+            // first hit of the group; no hit gives L (another group's lane, the
+            // result is then replaced below)
+            int32_t f;
+            if constexpr (L == 16)
+                f = __builtin_ctz(((uint32_t)(hit >> gb) & 0xffffu) | 0x10000u);
+            else
+                f = __builtin_ctzll((hit >> gb) | (1ull << 32));
+            int32_t next = bperm_i(gb + f, j);
+            const unsigned long long nohit = __builtin_amdgcn_ballot_w64(f == L);
+            if (__builtin_expect(nohit != 0ull, 0)) {
+    #pragma unroll 1
+                for (int gg = 0; gg < G; ++gg) {
+                    if (!((nohit >> (gg * L)) & 1ull)) continue;
+                    const float tg = readlane_f(tot, gg * L);
+                    const uint32_t pg = (uint32_t)((pos >> (gg * L)) & (L == 32 ? 0xffffffffull : 0xffffull));
+                    int32_t ng;
+                    if (tg > 0.0f && pg != 0u) {  // rounding: the group's last positive lane
+                        ng = readlane_i(j, gg * L + 31 - __clz((int)pg));
+                    } else {
+                        const bool vg = ant0 + gg < a.m;
+                        fb += vg ? 1 : 0;
+                        unsigned long long sc = 0;
+                        if constexpr (G == 4) {  // eight cities per lane over the whole wave
+                            ng = fallback_argmax_spread<8>(a, vreg, gg * L, readlane_i(c, gg * L), lane, sc);
+                        } else {  // 32 cities on each of the group's lanes' words
+                            const uint32_t vword = (uint32_t)bperm_i((gg * L + lane) & 63, (int32_t)vreg);
+                            ng = fallback_argmax_vreg(a, vword, readlane_i(c, gg * L), false, 0.0f, lane, sc);
+                        }
+                        scanned += vg ? sc : 0ull;
                     }
-                    scanned += vg ? sc : 0ull;
+                    next = (g == gg) ? ng : next;
                 }
-                next = (g == gg) ? ng : next;
             }
+            cj = *reinterpret_cast<const int2*>(candb + __umul24((uint32_t)next, kk8));
+            vreg |= (i == (next >> 5)) ? (1u << (next & 31)) : 0u;
+            tbuf = (i == s - s0) ? next : tbuf;
+            c = next;
         }
-        cj = *reinterpret_cast<const int2*>(candb + __umul24((uint32_t)next, kk8));
-        vreg |= (i == (next >> 5)) ? (1u << (next & 31)) : 0u;
-        tbuf = (i == (s & (L - 1))) ? next : tbuf;
-        if (__builtin_expect((s & (L - 1)) == L - 1 || s == a.n - 1, 0)) {
-            // segment s0 .. s: write it, add the previous segment's edge
-            // lengths and load this one's (edge into entry s0 + i)
-            const int32_t s0 = s & ~(L - 1);
-            const int32_t cnt = s - s0 + 1;
+        {
+            // segment s0 .. s_end - 1: write it, add the previous segment's
+            // edge lengths and load this one's (edge into entry s0 + i)
+            const int32_t cnt = s_end - s0;
             // with every lane active: bpermute reads 0 from an inactive lane,
             // and a conditional operand would run it only where taken
             const int32_t left = bperm_i(lane - 1, tbuf);
@@ -2355,7 +2360,6 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
             }
             prev_last = bperm_i(gb + L - 1, tbuf);
         }
-        c = next;
     }
     len += pend;
     if (valid && i == 0) len += a.dist[(size_t)c * a.P + start];
