@@ -309,12 +309,14 @@ def test_as_deposit_paths_agree_and_replay(aging):
             c.close()
 
 
-@pytest.mark.parametrize("alpha", [1.0, 1.5])
-def test_large_n_rank_based_aging_young_ants(alpha):
+@pytest.mark.parametrize("alpha,n", [(1.0, 4200), (1.5, 4200), (1.0, 8200)])
+def test_large_n_rank_based_aging_young_ants(alpha, n):
     """n >= 4096 with aging: young ants (pulled-toward-the-mean weights) scan
     rows in their fallbacks while the others use the row top lists; with
-    alpha != 1 there are no top lists at all."""
-    n, m, k = 4200, 32, 6
+    alpha != 1 there are no top lists at all.  At n = 8200 (n x pitch above
+    2^26) the colony keeps no per-edge heuristic and young fallbacks gather
+    the heuristic table."""
+    m, k = 32, 6
     say(f"case: RBAS aging forced state n={n} m={m} k={k} alpha={alpha} (global-memory path)")
     rng = np.random.default_rng(23)
     xy = instances.uniform(n, 100000, seed=23)
