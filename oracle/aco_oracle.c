@@ -2,7 +2,7 @@
  * aco_oracle.c -- CPU restatement of the reference Ant-System hot path.
  *
  * TEST INFRASTRUCTURE ONLY (the checker).  The product never links it.
- * Built with -O2 -ffp-contract=off and without fast-math so that every fp32
+ * Built with -O3 -ffp-contract=off and without fast-math so that every fp32
  * operation is one IEEE round-to-nearest operation in program order, the same
  * arithmetic the gfx950 kernels perform.  See aco_oracle.h for what pins each
  * function.

@@ -21,8 +21,15 @@
  *                          equal almost everywhere); exact powers of two
  *   orc_wave_scan          exact integer prefix sums; fp64 error bound
  *   orc_select_candidate   chi-square frequencies; exact-arithmetic boundaries
- *   orc_sweep              elementwise closed forms
- *   orc_construct          permutation + length invariants, greedy limit
+ *   orc_sweep              elementwise numpy closed forms of choice, choice_nn
+ *                          and the evaporated tau (floor included), alpha 1
+ *                          and 1.5, heuristic override, d = 0, diagonal
+ *   orc_construct          permutation + length invariants; chi-square of the
+ *                          first move against numpy weights in all three modes
+ *                          (young ants too); brute-force farthest-unvisited
+ *                          walks on zero and overflowing sums; argmax fallback
+ *   (tests/oracle_pins.py; tests/test_oracle_mutants.py plants one mistake per
+ *    copy of this file and requires these pins to fail on it)
  *   orc_stats              brute-force argmin / ranking
  *   orc_deposit_f64        numpy np.add.at restatement over tour edges
  *   orc_aging_*            closed-form retirement/recruitment traces

@@ -7,6 +7,7 @@ function it restates.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 
@@ -33,9 +34,29 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        _lib = ctypes.CDLL(LIB_PATH)
-        _declare(_lib)
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def load(path):
+    """A separately built copy of the oracle library (declared, not installed)."""
+    L = ctypes.CDLL(path)
+    _declare(L)
+    return L
+
+
+@contextlib.contextmanager
+def using(path):
+    """Routes every call of this module to the library at `path` for the
+    duration of the block (the tests' mutation checks build deliberately
+    broken copies of the oracle and load them this way)."""
+    global _lib
+    saved = lib()
+    _lib = load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def _p(a):

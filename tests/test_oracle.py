@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
+import oracle_pins as pins
 from acogpu_amd import instances
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -260,7 +261,10 @@ def test_fallback_is_argmax_of_unvisited():
             vis.add(nxt)
 
 
-def test_aging_young_weights_pull_toward_mean():
+def test_aging_young_state_only_below_min_age_after_convergence():
+    """Which ants are young: before convergence nobody, after it only ants
+    below min_age (the weights young ants use are pinned by the first-move
+    frequencies in test_oracle_pins_young_*)."""
     xy, d, eta, nn, tau = _setup(n=64, k=10, seed=9)
     rng = np.random.default_rng(1)
     tau = rng.uniform(0.001, 0.02, size=(64, 64)).astype(np.float32)
@@ -404,3 +408,14 @@ def test_c1_refseq_fixture_reproduces():
     _, dist, tau0 = mk.c1_instance()
     assert np.float32(tau0) == np.float32(ref["tau0"])
     assert mk.refseq_best(dist, tau0, ref["seeds"][0]) == ref["best"][0]
+
+
+# ----------------------------------------------------------------- closed-form / brute-force pins
+@pytest.mark.parametrize("name", sorted(pins.CHECKS))
+def test_oracle_pins(name):
+    """Sweep elementwise closed forms, young-ant weights (chi-square of the
+    first move against numpy weights), full-row and reference-sequential
+    roulette (chi-square over two scan chunks), and the farthest-unvisited
+    fallback on zero and overflowing sums (brute force); see
+    tests/oracle_pins.py."""
+    pins.CHECKS[name]()
