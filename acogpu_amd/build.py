@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libacogpu_amd.so")
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", os.path.join(ROCM, "bin", "hipcc"))
 
 # -ffp-contract=off: fp32 expressions evaluate one IEEE operation at a time
 # (bit-exact replay against the CPU oracle).  -packed-fp32-ops: no packed fp32
@@ -53,9 +54,10 @@ def build_app() -> str:
     """acouav: the C++ Colony classes and command line over the C ABI."""
     exe = os.path.join(OUT, "acouav")
     srcs = [os.path.join(CSRC, s) for s in APP_SOURCES]
-    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
-          "-I", os.path.join(CSRC, "colony"), "-I", os.path.join(CSRC, "app"), *srcs,
-          "-L", OUT, "-lacogpu_amd", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"])
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(CSRC, "colony"), "-I", os.path.join(CSRC, "app"), "-I", os.path.join(ROCM, "include"),
+          *srcs, "-L", OUT, "-lacogpu_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", exe + ".tmp"])
     os.replace(exe + ".tmp", exe)
     for v in APP_VARIANTS:
         link = os.path.join(OUT, v)

@@ -51,6 +51,10 @@ SIGNATURES = {
     "aco_run": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_run_until": (ctypes.c_int, [_vp, _i32, _i64, _i32, _vp, _vp]),
     "aco_phase": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_phase_end": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "aco_probe_step": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_batch_profile": (ctypes.c_int, [_vp, _i32, _vp]),
+    "aco_run_until_time": (ctypes.c_int, [_vp, _i32, _i64, ctypes.c_float, _i32, _vp, _vp, _vp]),
     "aco_get_stats": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "aco_get_best_tour": (ctypes.c_int, [_vp, _vp]),
     "aco_get_tours": (ctypes.c_int, [_vp, _vp]),
@@ -165,14 +169,18 @@ def default_params(**kw) -> Params:
 class Colony:
     """One aco_ctx: the Colony/AntSystem/RankBasedAntSystem state on the GPU."""
 
-    def __init__(self, params: Params, dist=None, xy=None):
+    def __init__(self, params: Params, dist=None, xy=None, dist_device_ptr=None):
+        """dist: host n x n int32 matrix; xy: host coordinates; dist_device_ptr:
+        address of an n x n int32 matrix in device memory (the reference's
+        GPU convention)."""
         self._L = lib()
         self.params = params
         self.n, self.m = params.n, params.m
         self._ctx = ctypes.c_void_p()
         d = None if dist is None else np.ascontiguousarray(dist, dtype=np.int32)
         c = None if xy is None else np.ascontiguousarray(xy, dtype=np.int32)
-        rc = self._L.aco_create(ctypes.byref(params), _p(d), _p(c), ctypes.byref(self._ctx))
+        dp = ctypes.c_void_p(int(dist_device_ptr)) if dist_device_ptr is not None else _p(d)
+        rc = self._L.aco_create(ctypes.byref(params), dp, _p(c), ctypes.byref(self._ctx))
         if rc != 0:
             raise AcoError(f"aco_create failed ({rc}): {self._L.aco_last_error(None).decode()}")
 
@@ -206,10 +214,22 @@ class Colony:
         self._chk(self._L.aco_iterate(self._ctx, _p(t)))
         return t
 
-    def phase(self, phase):
+    def phase(self, phase, wait=True):
+        """One phase; wait=False enqueues it without waiting (complete the
+        iteration with phase_end)."""
+        if not wait:
+            self._chk(self._L.aco_phase(self._ctx, int(phase), None))
+            return None
         ms = ctypes.c_float()
         self._chk(self._L.aco_phase(self._ctx, int(phase), ctypes.byref(ms)))
         return ms.value
+
+    def phase_end(self):
+        """(timers[6], iter_best, glob_best, valid ants) of aco_phase_end."""
+        t = np.zeros(6, dtype=np.float32)
+        ib, gb, nv = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_uint32()
+        self._chk(self._L.aco_phase_end(self._ctx, _p(t), ctypes.byref(ib), ctypes.byref(gb), ctypes.byref(nv)))
+        return t, ib.value, gb.value, nv.value
 
     def run(self, iterations):
         ms = ctypes.c_float()
@@ -223,6 +243,13 @@ class Colony:
                                         ctypes.byref(halted)))
         return ran.value, bool(halted.value)
 
+    def run_until_time(self, max_iterations, stop_at, time_limit_ms, chunk=32):
+        """(iterations run, halted code 0/1 optimum/2 time, elapsed ms) of aco_run_until_time."""
+        ran, halted, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+        self._chk(self._L.aco_run_until_time(self._ctx, int(max_iterations), int(stop_at), float(time_limit_ms),
+                                             int(chunk), ctypes.byref(ran), ctypes.byref(halted), ctypes.byref(ms)))
+        return ran.value, halted.value, ms.value
+
     def profile(self, iterations):
         out = np.zeros(4, dtype=np.float32)
         self._chk(self._L.aco_profile(self._ctx, int(iterations), _p(out)))
@@ -232,6 +259,12 @@ class Colony:
         ms = ctypes.c_float()
         self._chk(self._L.aco_calibrate_copy(self._ctx, int(nbytes), int(reps), ctypes.byref(ms)))
         return ms.value
+
+    def probe_step(self, iters=200000):
+        """(core clock cycles per step, ns per step) of the one-wave step-chain probe."""
+        out = np.zeros(2, dtype=np.float32)
+        self._chk(self._L.aco_probe_step(self._ctx, int(iters), _p(out)))
+        return float(out[0]), float(out[1])
 
     # -- state -------------------------------------------------------------
     def stats(self):
@@ -385,6 +418,12 @@ class Batch:
         ms = ctypes.c_float()
         self._chk(self._L.aco_batch_run(self._ctx, int(iterations), ctypes.byref(ms)))
         return ms.value
+
+    def profile(self, iterations):
+        """Average HIP-event ms of {sweep, construction, statistics, deposit}."""
+        out = np.zeros(4, dtype=np.float32)
+        self._chk(self._L.aco_batch_profile(self._ctx, int(iterations), _p(out)))
+        return out
 
     def autotune(self, reps=3):
         w = ctypes.c_int32()

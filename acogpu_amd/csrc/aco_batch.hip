@@ -28,6 +28,7 @@ struct aco_batch {
     std::vector<int32_t> n, m;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {};
+    hipEvent_t kev[5] = {};  // aco_batch_profile: around the four kernels of an iteration
     BatchBuffers buf{};
     float* tau[2] = {nullptr, nullptr};
     int2* items = nullptr;
@@ -72,6 +73,8 @@ static void bfree(aco_batch* b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (void* q : b->allocs) (void)hipFree(q);
     for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : b->kev)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
@@ -230,6 +233,23 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
         return bfail(nullptr, ACO_E_ARG, "ants_per_wave %d needs k <= %d and n <= %d (k=%d, largest n=%d)",
                      p->ants_per_wave, 64 / p->ants_per_wave, 2048 / p->ants_per_wave, p->k, nm);
     }
+    {
+        // the construction kernel stages the largest instance's candidate
+        // table in LDS: refuse what cannot launch on this device
+        int dev = 0, max_lds = 65536;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            max_lds = 65536;
+        (void)hipGetLastError();
+        const size_t need = aco::batch_construct_lds_bytes(b->nmax, p->k, 1, 1) + 256;
+        if (need > (size_t)max_lds) {
+            const int32_t nm = b->nmax;
+            delete b;
+            return bfail(nullptr, ACO_E_RANGE,
+                         "the largest instance (n=%d, k=%d) needs %zu bytes of LDS for batched construction "
+                         "(limit %d)", nm, p->k, need, max_lds);
+        }
+    }
     b->total_rows = rows;
     b->total_ants = ants;
 
@@ -379,10 +399,9 @@ extern "C" int aco_batch_initialize(aco_batch* b) {
     BHIP(b, hipMemsetAsync(b->buf.queue, 0, sizeof(uint32_t) * aco::QUEUE_BINS, b->stream));
     BHIP(b, hipMemsetAsync(b->buf.best_tours, 0, sizeof(int32_t) * b->total_rows, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
+    // the iteration graph captures buffer addresses and the launch shape
+    // only (aco_batch_autotune rebuilds it): it stays valid across
+    // initialisations
     b->initialized = true;
     return ACO_OK;
 }
@@ -552,6 +571,37 @@ extern "C" int aco_batch_get_config(aco_batch* b, int32_t* config) {
     config[2] = b->buf.nitems;
     config[3] = (int32_t)aco::batch_construct_lds_bytes(b->nmax, b->k, b->apb, b->apw);
     config[4] = b->apw;
+    return ACO_OK;
+}
+
+// Per-kernel HIP-event times of `iterations` eagerly launched iterations
+// (the kernels the graph replays): kernel_ms = {sweep, construction,
+// statistics, deposit} averages.
+extern "C" int aco_batch_profile(aco_batch* b, int32_t iterations, float* kernel_ms) {
+    if (!b || !kernel_ms || iterations < 1) return ACO_E_ARG;
+    if (!b->initialized) return bfail(b, ACO_E_STATE, "aco_batch_profile before aco_batch_initialize");
+    for (auto& e : b->kev)
+        if (!e) BHIP(b, hipEventCreate(&e));
+    double acc[4] = {0, 0, 0, 0};
+    for (int32_t it = 0; it < iterations; ++it) {
+        BatchBuffers B = b->buf;
+        B.tau_cur = b->tau[b->cur];
+        B.tau_next = b->tau[1 - b->cur];
+        const int32_t grids[4] = {0, b->grid, (b->mmax + 31) / 32, 0};
+        BHIP(b, hipEventRecord(b->kev[0], b->stream));
+        for (int q = 0; q < 4; ++q) {
+            BHIP(b, aco::launch_batch_iteration_kernel(B, q, b->nmax, grids[q], b->stream));
+            BHIP(b, hipEventRecord(b->kev[q + 1], b->stream));
+        }
+        b->cur ^= 1;
+        BHIP(b, hipEventSynchronize(b->kev[4]));
+        for (int q = 0; q < 4; ++q) {
+            float t = 0.0f;
+            BHIP(b, hipEventElapsedTime(&t, b->kev[q], b->kev[q + 1]));
+            acc[q] += t;
+        }
+    }
+    for (int q = 0; q < 4; ++q) kernel_ms[q] = (float)(acc[q] / iterations);
     return ACO_OK;
 }
 

@@ -59,6 +59,9 @@ struct aco_ctx {
     hipGraphExec_t graph = nullptr;       // two iterations starting at cur == 0
     hipGraphExec_t graph_long = nullptr;  // kGraphLong iterations starting at cur == 0
     hipEvent_t ev[6] = {};
+    hipEvent_t pev[5][2] = {};  // asynchronous phases: start/end event per phase
+    uint32_t phases_async = 0;  // bit p: phase p enqueued without waiting since the iteration began
+    DevState* h_state = nullptr;  // pinned copy of d_state taken after an asynchronous update phase
     char err[512] = {0};
 };
 
@@ -120,6 +123,10 @@ static void free_all(aco_ctx* c) {
         if (q) hipFree(q);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
+    for (auto& pe : c->pev)
+        for (auto& e : pe)
+            if (e) hipEventDestroy(e);
+    if (c->h_state) hipHostFree(c->h_state);
     if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -255,6 +262,9 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
 
     HIPB(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPB(hipEventCreate(&e));
+    for (auto& pe : c->pev)
+        for (auto& e : pe) HIPB(hipEventCreate(&e));
+    HIPB(hipHostMalloc(reinterpret_cast<void**>(&c->h_state), sizeof(DevState), hipHostMallocDefault));
     const size_t nP = (size_t)c->n * c->P;
     TRYB(dalloc(c, &c->d_dist, nP));
     TRYB(dalloc(c, &c->d_tau[0], nP));
@@ -414,14 +424,8 @@ extern "C" int aco_initialize(aco_ctx* c) {
                                       c->d_eta_nn, c->stream));
     int rc = reset_state(c);
     if (rc) return rc;
-    if (c->graph) {
-        hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
-    if (c->graph_long) {
-        hipGraphExecDestroy(c->graph_long);
-        c->graph_long = nullptr;
-    }
+    // the iteration graphs capture buffer addresses and create-time
+    // parameters only: they stay valid across initialisations
     c->initialized = true;
     return ACO_OK;
 }
@@ -668,6 +672,7 @@ extern "C" int aco_run(aco_ctx* c, int32_t iterations, float* elapsed_ms) {
 }
 
 static int copy_in(aco_ctx* c, void* dst, const void* src, size_t bytes);
+static int copy_out(aco_ctx* c, void* dst, const void* src, size_t bytes);
 static int read_state(aco_ctx* c, DevState* st);
 
 // TSP::execute's loop (src/TSP.cpp:23-79, termination src/TSP.cpp:168-171)
@@ -677,6 +682,14 @@ static int read_state(aco_ctx* c, DevState* st);
 // state once per `chunk` iterations.
 extern "C" int aco_run_until(aco_ctx* c, int32_t max_iterations, int64_t stop_at, int32_t chunk,
                              int32_t* iterations_run, int32_t* halted) {
+    return aco_run_until_time(c, max_iterations, stop_at, -1.0f, chunk, iterations_run, halted, nullptr);
+}
+
+// The same loop with the time test of TSP::terminationCondition
+// (elapsed >= maxTime, src/TSP.cpp:168-171): elapsed is the HIP-event time
+// of the iterations run, checked before every chunk.
+extern "C" int aco_run_until_time(aco_ctx* c, int32_t max_iterations, int64_t stop_at, float time_limit_ms,
+                                  int32_t chunk, int32_t* iterations_run, int32_t* halted, float* elapsed_ms) {
     if (!c) return ACO_E_ARG;
     if (!c->initialized) return fail(c, ACO_E_STATE, "aco_run_until before aco_initialize");
     if (max_iterations < 0 || chunk < 1) return fail(c, ACO_E_ARG, "negative iteration count or chunk < 1");
@@ -691,15 +704,23 @@ extern "C" int aco_run_until(aco_ctx* c, int32_t max_iterations, int64_t stop_at
     st.skip = 0;
     if ((rc = copy_in(c, c->d_state, &st, sizeof(st)))) return rc;
     int32_t left = max_iterations;
+    double elapsed = 0.0;
+    bool timed_out = false;
     while (left > 0) {
+        if (time_limit_ms >= 0.0f && elapsed >= (double)time_limit_ms) {
+            timed_out = true;
+            break;
+        }
         const int32_t step = left < chunk ? left : chunk;
-        if ((rc = aco_run(c, step, nullptr))) return rc;
+        float ms = 0.0f;
+        if ((rc = aco_run(c, step, &ms))) return rc;
+        elapsed += ms;
         left -= step;
         if ((rc = read_state(c, &st))) return rc;
         if (st.halted) break;
     }
     const int32_t ran = (int32_t)(st.iter - iter0);
-    const int32_t was_halted = st.halted;
+    const int32_t was_halted = st.halted ? 1 : (timed_out ? 2 : 0);
     // skipped iterations wrote nothing: the current pheromone buffer follows
     // the iterations that ran
     c->cur = (cur0 + ran) & 1;
@@ -709,6 +730,7 @@ extern "C" int aco_run_until(aco_ctx* c, int32_t max_iterations, int64_t stop_at
     if ((rc = copy_in(c, c->d_state, &st, sizeof(st)))) return rc;
     if (iterations_run) *iterations_run = ran;
     if (halted) *halted = was_halted;
+    if (elapsed_ms) *elapsed_ms = (float)elapsed;
     return ACO_OK;
 }
 
@@ -751,6 +773,29 @@ extern "C" int aco_calibrate_copy(aco_ctx* c, int64_t bytes, int32_t reps, float
     hipFree(b);
     HIP_TRY(c, e);
     *ms = t / reps;
+    return ACO_OK;
+}
+
+// Latency floor of one construction step at this colony's k: a single wave
+// repeating the step's dependent chain (k_probe_step).  out = {core clock
+// cycles per step, ns per step}.
+extern "C" int aco_probe_step(aco_ctx* c, int32_t iters, float* out) {
+    if (!c || !out || iters < 1) return ACO_E_ARG;
+    if (c->k < 1) return fail(c, ACO_E_STATE, "the step probe needs candidate lists");
+    int dev = 0, wall_khz = 100000;
+    HIP_TRY(c, hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || wall_khz <= 0)
+        wall_khz = 100000;
+    (void)hipGetLastError();
+    unsigned long long* d_out = reinterpret_cast<unsigned long long*>(c->d_scratch64);  // 4 x 8 bytes
+    int32_t* d_sink = reinterpret_cast<int32_t*>(c->d_scratch64 + 2);
+    HIP_TRY(c, aco::launch_probe_step(iters < 1000 ? iters : 1000, c->k, d_out, d_sink, c->stream));  // warm-up
+    HIP_TRY(c, aco::launch_probe_step(iters, c->k, d_out, d_sink, c->stream));
+    unsigned long long h[2] = {0, 0};
+    int rc = copy_out(c, h, d_out, sizeof(h));
+    if (rc) return rc;
+    out[0] = (float)((double)h[0] / iters);
+    out[1] = (float)((double)h[1] * 1e6 / wall_khz / iters);
     return ACO_OK;
 }
 
@@ -936,13 +981,52 @@ extern "C" int aco_phase(aco_ctx* c, int32_t phase, float* ms) {
         if (!(c->next_phase == ACO_PHASE_EVAPORATE && phase == ACO_PHASE_UPDATE_PHEROMONE))
             return fail(c, ACO_E_STATE, "phase %d out of order (expected %d)", phase, c->next_phase);
     }
-    HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
-    HIP_TRY(c, enqueue_phase(c, c->cur, phase));
-    HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
-    HIP_TRY(c, hipEventSynchronize(c->ev[1]));
-    if (ms) HIP_TRY(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+    if (ms) {
+        HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+        HIP_TRY(c, enqueue_phase(c, c->cur, phase));
+        HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+        HIP_TRY(c, hipEventSynchronize(c->ev[1]));
+        HIP_TRY(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+        c->phases_async = 0;
+    } else {
+        // no wait: the phase's events are kept for aco_phase_end
+        if (phase == ACO_PHASE_PROBABILITIES) c->phases_async = 0;
+        HIP_TRY(c, hipEventRecord(c->pev[phase][0], c->stream));
+        HIP_TRY(c, enqueue_phase(c, c->cur, phase));
+        HIP_TRY(c, hipEventRecord(c->pev[phase][1], c->stream));
+        c->phases_async |= 1u << phase;
+        if (phase == ACO_PHASE_UPDATE_PHEROMONE)
+            HIP_TRY(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    }
     c->next_phase = next_of[phase];
     if (phase == ACO_PHASE_UPDATE_PHEROMONE) c->cur ^= 1;
+    return ACO_OK;
+}
+
+// Completes an iteration whose phases were enqueued without waiting: one
+// wait for the stream, then the phase times from their events and the
+// statistics from the state copied after the update phase.
+extern "C" int aco_phase_end(aco_ctx* c, float* timers, int64_t* iter_best, int64_t* glob_best,
+                             uint32_t* valid_ants) {
+    if (!c) return ACO_E_ARG;
+    if (!(c->phases_async & (1u << ACO_PHASE_UPDATE_PHEROMONE)))
+        return fail(c, ACO_E_STATE, "aco_phase_end without an iteration of asynchronous phases (ms == NULL)");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float t[5] = {0, 0, 0, 0, 0};
+    for (int ph = 0; ph < 5; ++ph)
+        if (c->phases_async & (1u << ph)) HIP_TRY(c, hipEventElapsedTime(&t[ph], c->pev[ph][0], c->pev[ph][1]));
+    c->phases_async = 0;
+    if (timers) {
+        timers[ACO_TIME_INIT_TOUR] = 0.0f;  // placement runs inside the tour kernel
+        timers[ACO_TIME_TOUR] = t[ACO_PHASE_TOUR];
+        timers[ACO_TIME_COMPUTE_ANT_DISTANCE] = t[ACO_PHASE_ANT_DISTANCE];
+        timers[ACO_TIME_EVAPORATE_PHEROMONE] = t[ACO_PHASE_EVAPORATE];
+        timers[ACO_TIME_UPDATE_PHEROMONE] = t[ACO_PHASE_UPDATE_PHEROMONE];
+        timers[ACO_TIME_COMPUTE_PROBABILITIES] = t[ACO_PHASE_PROBABILITIES];
+    }
+    if (iter_best) *iter_best = c->h_state->iter_best;
+    if (glob_best) *glob_best = c->h_state->glob_best;
+    if (valid_ants) *valid_ants = c->p.aging ? (uint32_t)c->h_state->num_valid : (uint32_t)c->m;
     return ACO_OK;
 }
 

@@ -278,5 +278,6 @@ hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s);
 // every pheromone row gathers its deposits in LDS (needs a.adj, a.m, n <= ADJ_MAX_N).
 hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s);
 hipError_t launch_copy(const float4* src, float4* dst, int64_t count, hipStream_t s);
+hipError_t launch_probe_step(int32_t iters, int32_t k, unsigned long long* out, int32_t* sink, hipStream_t s);
 
 }  // namespace aco

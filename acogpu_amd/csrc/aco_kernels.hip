@@ -2778,6 +2778,63 @@ hipError_t launch_batch_exchange(const BatchBuffers& b, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Latency floor of the staged construction step: one wave repeats the
+// step's dependent chain and nothing else -- the LDS reads of the candidate
+// entry and weight at the current city (24-bit multiply-add addresses, lanes
+// >= k on a zero entry), the ds_bpermute visited test, the roulette scan,
+// ballot and pick of roulette_pick, the lane read of the step's uniform --
+// over a 64-city table.  out = {core clock cycles, wall-clock ticks} of
+// `iters` steps; sink keeps the chain live.
+template <bool K32>
+__global__ __launch_bounds__(64) void k_probe_step(int32_t iters, int32_t k, unsigned long long* out,
+                                                   int32_t* sink) {
+    __shared__ uint16_t nl[64 * 64 + 1];
+    __shared__ float wl[64 * 64 + 1];
+    const int lane = threadIdx.x;
+    for (int e = lane; e < 64 * 64 + 1; e += 64) {
+        const int c = e / 64, r = e % 64;
+        nl[e] = (uint16_t)((c + 1 + 7 * r) & 63);
+        wl[e] = e == 64 * 64 ? 0.0f : 1.0f + (float)((c * 31 + r * 17) & 15);
+    }
+    __syncthreads();
+    const uint32_t vreg = lane < 2 ? 0x5a5a5a5au : 0u;  // about half of the 64 cities visited
+    const uint32_t kk = lane < k ? (uint32_t)k : 0u;
+    const int32_t loff = lane < k ? lane : 64 * 64;
+    const char* nlb = reinterpret_cast<const char*>(nl + loff);
+    const char* wlb = reinterpret_cast<const char*>(wl + loff);
+    const uint32_t kk2 = 2u * kk, kk4 = 4u * kk;
+    const float uc = (float)((lane * 40503u) & 0xffffu) * (1.0f / 65536.0f);
+    int32_t c = 0;
+    int32_t j = *reinterpret_cast<const uint16_t*>(nlb);
+    float cw = *reinterpret_cast<const float*>(wlb);
+    const unsigned long long t0 = clock64(), w0 = wall_clock64();
+    for (int32_t it = 0; it < iters; ++it) {
+        const float u = readlane_f(uc, it & 63);
+        __builtin_assume((uint32_t)j < 65536u);
+        const int32_t word = __builtin_amdgcn_ds_bpermute(j >> 3, (int)vreg);
+        const int32_t vmask = __builtin_amdgcn_sbfe(word, (uint32_t)j, 1u);
+        const float w = __int_as_float(__float_as_int(cw) & ~vmask);
+        const int32_t next = roulette_pick<K32>(w, j, u, [&]() { return (c + 1) & 63; });
+        j = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)next, kk2));
+        cw = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)next, kk4));
+        c = next;
+    }
+    const unsigned long long t1 = clock64(), w1 = wall_clock64();
+    if (lane == 0) {
+        out[0] = t1 - t0;
+        out[1] = w1 - w0;
+        sink[0] = c;
+    }
+}
+
+hipError_t launch_probe_step(int32_t iters, int32_t k, unsigned long long* out, int32_t* sink, hipStream_t s) {
+    if (k <= 32)
+        hipLaunchKernelGGL(k_probe_step<true>, dim3(1), dim3(64), 0, s, iters, k, out, sink);
+    else
+        hipLaunchKernelGGL(k_probe_step<false>, dim3(1), dim3(64), 0, s, iters, k, out, sink);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const float4* src, float4* dst, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, src, dst, count);
     return hipGetLastError();

@@ -8,23 +8,23 @@
 #include "acouav.h"
 
 static const char* kHelp =
-    "  -a, --alpha          # alpha (influence of pheromone trails)\n"
-    "  -b, --beta           # beta (influence of heuristic information)\n"
-    "  -r, --rho            # rho: pheromone trail evaporation\n"
-    "  -s, --tours          # number of steps in each trial\n"
-    "  -t, --time           # maximum time for each trial\n"
-    "  -i, --tsplibfile     f inputfile (TSPLIB format necessary)\n"
-    "  -p, --outfile        f output file where results will be stored\n"
-    "  -o, --optimum        # stop if tour better or equal optimum is found\n"
-    "  -m, --ants           # number of ants\n"
-    "  -c, --repetitions    # number of repetitions at which solution exits\n"
-    "  -j, --minage           min age for tetirement of an ant\n"
-    "  -k, --maxage           max age for tetirement of an ant\n"
-    "  -l, --localperformance           local performance of an ant in deciding retirement\n"
-    "  -g, --globalperformance           global performance of all ants in deciding retirement\n"
-    "  -x, --minpopulation           min population on ants in deciding recruitment\n"
-    "  -y, --maxpopulation           max population of ants in deciding retirement\n"
-    "  -v, --verbose        display output to standard console\n"
+    "  -a, --alpha #            weight of the pheromone term (tau^alpha)\n"
+    "  -b, --beta #             weight of the distance heuristic (eta^beta)\n"
+    "  -r, --rho #              evaporation rate per iteration, 0..1\n"
+    "  -s, --tours #            iteration limit\n"
+    "  -t, --time #             time limit in ms (summed phase times)\n"
+    "  -i, --tsplibfile FILE    instance in TSPLIB format (EUC_2D)\n"
+    "  -p, --outfile FILE       per-iteration statistics are written here\n"
+    "  -o, --optimum #          stop once an iteration's best tour is this short\n"
+    "  -m, --ants #             colony size\n"
+    "  -c, --repetitions #      repetition limit\n"
+    "  -j, --minage #           aging: below this age after convergence an ant is young\n"
+    "  -k, --maxage #           aging: ants of this age retire (one per iteration)\n"
+    "  -l, --localperformance # aging: accepted, not used by the rules\n"
+    "  -g, --globalperformance # aging: recruit after this many iterations without improvement\n"
+    "  -x, --minpopulation #    aging: recruit while fewer ants are valid\n"
+    "  -y, --maxpopulation #    aging: accepted, not used by the rules\n"
+    "  -v, --verbose            echo the statistics to the console\n"
     "  -h, --help           display this help text and exit\n"
     " MI355X build selections:\n"
     "      --aging / --noaging       ant aging variant (default from the program name)\n"
@@ -33,7 +33,9 @@ static const char* kHelp =
     "      --w #                    ranked ants of the rank-based system (default 5)\n"
     "      --dorigo-tau0            tau0 = 0.5 w (w-1) / (rho C_nn) for the rank-based system\n"
     "      --candidates #           candidate list length k (1..64); 0 = every city (default)\n"
-    "      --seed #                 random stream key (default 12345678)\n";
+    "      --seed #                 random stream key (default 12345678)\n"
+    "      --host-distances         hand the distance matrix to the colony in host memory\n"
+    "                               (default: device memory, as the reference's GPU build)\n";
 
 Commandline::Commandline()
     : numAnts(10), maxTime(0x7fffffff), maxIter(10), maxReps(0x7fffffff), optimum(-1), alpha(1.0f), beta(5.0f),
@@ -91,6 +93,7 @@ bool Commandline::parse(int argc, char* argv[]) {
         else if (o == "--dorigo-tau0") dorigoTau0 = true;
         else if (o == "--candidates") candidates = std::atoi(next(i));
         else if (o == "--seed") seed = std::strtoull(next(i), nullptr, 10);
+        else if (o == "--host-distances") hostDistances = true;
         // unknown arguments are ignored, as in the reference
     }
     return true;

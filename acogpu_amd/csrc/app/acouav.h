@@ -63,6 +63,7 @@ class Commandline {
     int w = 5;
     int candidates = 0;  // 0: every city in NN order, as the reference
     uint64_t seed = 12345678;
+    bool hostDistances = false;  // default: device copy, as src/Setup.cpp:60-66
 };
 
 class TSPReader {

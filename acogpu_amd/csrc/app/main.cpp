@@ -1,5 +1,9 @@
 // acouav main (src/Setup.cpp:73-129): parse, read TSPLIB, distances, colony,
 // header, run.
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <cstdlib>
 #include <iomanip>
 #include <iostream>
 #include <memory>
@@ -24,11 +28,25 @@ int main(int argc, char* argv[]) {
     Stopwatch sw;
     sw.start();
     std::vector<int> dist = euc2dMatrix(reader.x, reader.y);
+    // The reference's GPU build hands the colony a device copy of the matrix
+    // and frees it after the run (src/Setup.cpp:60-66, 120-124).
+    int* distances = dist.data();
+    int* d_dist = nullptr;
+    if (!cl.hostDistances) {
+        const size_t bytes = dist.size() * sizeof(int);
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_dist), bytes);
+        if (e == hipSuccess) e = hipMemcpy(d_dist, dist.data(), bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            std::fprintf(stderr, "device copy of the distance matrix: %s\n", hipGetErrorString(e));
+            return EXIT_FAILURE;
+        }
+        distances = d_dist;
+    }
     const float calcdisttime = sw.elapsed();
 
     std::unique_ptr<Colony> colony;
     if (cl.rankBased) {
-        auto* rb = new RankBasedAntSystem(dist.data(), reader.nodes, cl.numAnts);
+        auto* rb = new RankBasedAntSystem(distances, reader.nodes, cl.numAnts);
         rb->setAlpha(cl.alpha);
         rb->setBeta(cl.beta);
         rb->setRho(cl.rho);
@@ -36,11 +54,11 @@ int main(int argc, char* argv[]) {
         rb->setDorigoInitialPheromone(cl.dorigoTau0);
         colony.reset(rb);
     } else if (cl.aging) {
-        colony.reset(new AntSystem(dist.data(), reader.nodes, cl.numAnts, cl.alpha, cl.beta, cl.rho,
+        colony.reset(new AntSystem(distances, reader.nodes, cl.numAnts, cl.alpha, cl.beta, cl.rho,
                                    cl.minRetirementAge, cl.maxRetirementAge, cl.localPeformance,
                                    cl.globalPerformance, cl.minPopulationLimit, cl.maxPopulationLimit));
     } else {
-        colony.reset(new AntSystem(dist.data(), reader.nodes, cl.numAnts, cl.alpha, cl.beta, cl.rho));
+        colony.reset(new AntSystem(distances, reader.nodes, cl.numAnts, cl.alpha, cl.beta, cl.rho));
     }
     if (cl.aging) {
         colony->setAging(true);
@@ -63,5 +81,7 @@ int main(int argc, char* argv[]) {
     }
     TSP tsp(cl.maxTime, cl.maxIter, cl.maxReps, cl.optimum);
     tsp.execute(reader.name.c_str(), *colony, cl.outfile.c_str(), calcdisttime, cl.aging);
+    colony.reset();
+    if (d_dist) (void)hipFree(d_dist);
     return 0;
 }

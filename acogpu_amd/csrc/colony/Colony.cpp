@@ -1,5 +1,7 @@
 // Colony over the C ABI.  The iteration order of constructSolution follows
-// src/Colony.cu:537-594; each step is one device phase (aco_phase).
+// src/Colony.cu:537-594; each step is one device phase (aco_phase), enqueued
+// without waiting, and aco_phase_end waits once per iteration for the phase
+// times and the statistics the caller reads next.
 #include "Colony.h"
 
 #include <cstdio>
@@ -39,11 +41,21 @@ void Colony::initialize() {
         aco_destroy(ctx);
         ctx = nullptr;
     }
-    if (params.mode == ACO_MODE_FULLROW && params.n > 8192) {
-        params.mode = ACO_MODE_CANDIDATE;  // the full row does not fit the device path
-        if (params.k <= 0 || params.k > 64) params.k = 32;
-    }
+    statsCached = false;
     int rc = aco_create(&params, distances, nullptr, &ctx);
+    if (rc == ACO_E_RANGE && params.mode == ACO_MODE_FULLROW) {
+        // The reference's candidate set (every city in NN order) needs the
+        // visited set and the row staged in LDS; beyond that size the colony
+        // runs on candidate lists instead, and says so.
+        const int k = (params.k > 0 && params.k <= 64 && params.k < params.n) ? params.k : 32;
+        std::fprintf(stderr,
+                     "acogpu: %s; running with candidate lists of the %d nearest cities (argmax fallback) "
+                     "instead of every city in NN order\n",
+                     aco_last_error(nullptr), k);
+        params.mode = ACO_MODE_CANDIDATE;
+        params.k = k;
+        rc = aco_create(&params, distances, nullptr, &ctx);
+    }
     if (rc != ACO_OK) {
         std::fprintf(stderr, "acogpu error %d: %s\n", rc, aco_last_error(nullptr));
         std::exit(EXIT_FAILURE);
@@ -51,29 +63,32 @@ void Colony::initialize() {
     check(aco_initialize(ctx));
 }
 
-float Colony::runPhase(int phase) {
+void Colony::runPhase(int phase) {
     if (!ctx) {
         std::fprintf(stderr, "Colony: iteration before initialize()\n");
         std::exit(EXIT_FAILURE);
     }
-    float ms = 0.0f;
-    check(aco_phase(ctx, phase, &ms));
-    lastPhaseMs = ms;
-    return ms;
+    statsCached = false;
+    check(aco_phase(ctx, phase, nullptr));  // enqueued; no wait
 }
 
 void Colony::constructSolution(float timers[]) {
-    float t[TOTAL_TIMERS] = {0, 0, 0, 0, 0, 0};
     // initTour (placement) and updateAntAges run inside the tour kernel and
-    // the probability sweep; their slot stays 0.
-    t[TIME_COMPUTE_PROBABILITIES] = runPhase(ACO_PHASE_PROBABILITIES);
-    t[TIME_TOUR] = runPhase(ACO_PHASE_TOUR);
-    t[TIME_COMPUTE_ANT_DISTANCE] = runPhase(ACO_PHASE_ANT_DISTANCE);
+    // the probability sweep; their slot reads 0.
+    runPhase(ACO_PHASE_PROBABILITIES);
+    runPhase(ACO_PHASE_TOUR);
+    runPhase(ACO_PHASE_ANT_DISTANCE);
     // virtual steps of the subclasses (AntSystem / RankBasedAntSystem)
     evaporatePheromones();
-    t[TIME_EVAPORATE_PHEROMONE] = lastPhaseMs;
     updatePheromones();
-    t[TIME_UPDATE_PHEROMONE] = lastPhaseMs;
+    float t[TOTAL_TIMERS] = {0, 0, 0, 0, 0, 0};
+    int64_t ib = 0, gb = 0;
+    uint32_t nv = 0;
+    check(aco_phase_end(ctx, t, &ib, &gb, &nv));
+    cachedIterBest = (long)ib;
+    cachedGlobBest = (long)gb;
+    cachedValidAnts = nv;
+    statsCached = true;
     if (timers)
         for (int i = 0; i < TOTAL_TIMERS; ++i) timers[i] = t[i];
 }
@@ -92,15 +107,18 @@ int Colony::getNumAnts() { return numAnts; }
 int Colony::getNumCities() { return numCities; }
 int Colony::getReps() { return reps; }
 
-int Colony::run(int maxIterations) {
+int Colony::run(int maxIterations, float maxTimeMs) {
     if (!ctx) initialize();
+    statsCached = false;
     int32_t ran = 0, halted = 0;
-    check(aco_run_until(ctx, maxIterations, optimum > 0 ? (int64_t)optimum : -1, 64, &ran, &halted));
+    check(aco_run_until_time(ctx, maxIterations, optimum > 0 ? (int64_t)optimum : -1, maxTimeMs, 64, &ran, &halted,
+                             nullptr));
     return ran;
 }
 
 long Colony::getIterBestDist() {
     if (!ctx) return kMaxDistance - 1;
+    if (statsCached) return cachedIterBest;
     int64_t ib = 0, gb = 0;
     check(aco_get_stats(ctx, &ib, &gb, nullptr));
     return (long)ib;
@@ -108,6 +126,7 @@ long Colony::getIterBestDist() {
 
 long Colony::getGlobBestDist() {
     if (!ctx) return kMaxDistance;
+    if (statsCached) return cachedGlobBest;
     int64_t ib = 0, gb = 0;
     check(aco_get_stats(ctx, &ib, &gb, nullptr));
     return (long)gb;
@@ -115,6 +134,7 @@ long Colony::getGlobBestDist() {
 
 unsigned int Colony::getNumValidAnts() {
     if (!ctx) return (unsigned)numAnts;
+    if (statsCached) return cachedValidAnts;
     uint32_t nv = 0;
     check(aco_get_stats(ctx, nullptr, nullptr, &nv));
     return nv;

@@ -59,8 +59,10 @@ class Colony {
     void setOptimum(const int optimum) { this->optimum = optimum; }
     // Extension: up to maxIterations iterations with no host round trip per
     // iteration, stopping after the first iteration whose best reaches the
-    // optimum (setOptimum); returns the number of iterations run.
-    int run(int maxIterations);
+    // optimum (setOptimum) or once maxTimeMs of device time has been spent
+    // (checked every 64 iterations; < 0: no limit); returns the number of
+    // iterations run.
+    int run(int maxIterations, float maxTimeMs = -1.0f);
 
     // aging (inc/Colony.h:128-139)
     int getMinRetirementAge();
@@ -92,8 +94,9 @@ class Colony {
     virtual void evaporatePheromones() = 0;
     virtual void updatePheromones() = 0;
 
-    // Runs one phase of the device iteration; returns its time in ms.
-    float runPhase(int phase);
+    // Enqueues one phase of the device iteration (no wait); constructSolution
+    // collects the phase times once per iteration.
+    void runPhase(int phase);
     void check(int rc) const;
 
     aco_params params;
@@ -104,7 +107,10 @@ class Colony {
     int optimum = -1;
     float initialPheromone = 1e-6f;
     aco_ctx* ctx = nullptr;
-    float lastPhaseMs = 0.0f;  // time of the last runPhase, ms
+    // statistics of the last constructSolution (valid until the next phase)
+    bool statsCached = false;
+    long cachedIterBest = 0, cachedGlobBest = 0;
+    unsigned int cachedValidAnts = 0;
 };
 
 #endif

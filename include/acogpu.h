@@ -112,13 +112,22 @@ int aco_iterate(aco_ctx *c, float *timers);
  * buffer), TOUR (initTour + constructPath), ANT_DISTANCE
  * (computeAntDistances), EVAPORATE (no kernel: folded into PROBABILITIES;
  * may be skipped), UPDATE_PHEROMONE (updatePheromones + aging
- * retire/recruit).  ms (may be NULL) gets the phase's HIP-event time. */
+ * retire/recruit).  A non-NULL ms waits for the phase and gets its HIP-event
+ * time; ms == NULL returns at once (aco_phase_end completes the iteration). */
 #define ACO_PHASE_PROBABILITIES 0
 #define ACO_PHASE_TOUR 1
 #define ACO_PHASE_ANT_DISTANCE 2
 #define ACO_PHASE_EVAPORATE 3
 #define ACO_PHASE_UPDATE_PHEROMONE 4
 int aco_phase(aco_ctx *c, int32_t phase, float *ms);
+
+/* With ms == NULL aco_phase enqueues its phase and returns without waiting;
+ * after the UPDATE_PHEROMONE phase of such an iteration, aco_phase_end waits
+ * once and returns what Colony::constructSolution's caller reads next: the
+ * six timers (StatsTimer slots, ms) and getIterBestDist / getGlobBestDist /
+ * getNumValidAnts after the iteration (src/Colony.cu:537-594, 1096-1104;
+ * each output may be NULL).  One host synchronisation per iteration. */
+int aco_phase_end(aco_ctx *c, float *timers, int64_t *iter_best, int64_t *glob_best, uint32_t *valid_ants);
 
 /* TSP::execute's loop body for `iterations` iterations with no host
  * synchronisation inside (hipGraph replay).  elapsed_ms (may be NULL) gets the
@@ -134,6 +143,14 @@ int aco_run(aco_ctx *c, int32_t iterations, float *elapsed_ms);
  * what happened; the colony continues normally afterwards. */
 int aco_run_until(aco_ctx *c, int32_t max_iterations, int64_t stop_at, int32_t chunk, int32_t *iterations_run,
                   int32_t *halted);
+
+/* aco_run_until with the time test of TSP::terminationCondition as well
+ * (elapsed >= maxTime, src/TSP.cpp:168-171): before every chunk the run stops
+ * once the HIP-event time of the iterations run reaches time_limit_ms (< 0:
+ * no limit; 0 runs none).  halted = 1 (optimum), 2 (time limit) or 0;
+ * elapsed_ms (may be NULL) gets that time. */
+int aco_run_until_time(aco_ctx *c, int32_t max_iterations, int64_t stop_at, float time_limit_ms, int32_t chunk,
+                       int32_t *iterations_run, int32_t *halted, float *elapsed_ms);
 
 /* getIterBestDist / getGlobBestDist / getNumValidAnts (src/Colony.cu:
  * 1096-1104, 447-467). */
@@ -185,6 +202,13 @@ int aco_get_config(aco_ctx *c, int32_t *config);
  * deposit}. */
 int aco_profile(aco_ctx *c, int32_t iterations, float *kernel_ms);
 
+/* Latency floor of one construction step at this colony's k: a single wave
+ * repeats the LDS-staged step's dependent chain (candidate entry and weight
+ * from LDS, visited test by ds_bpermute, roulette scan, ballot and pick)
+ * `iters` times; out[0] = core clock cycles per step, out[1] = ns per step.
+ * No counterpart in the reference (measurement only). */
+int aco_probe_step(aco_ctx *c, int32_t iters, float *out);
+
 /* Memory-bandwidth calibration: copies `bytes` (multiple of 16) device to
  * device with a plain streaming kernel `reps` times; returns ms per copy. */
 int aco_calibrate_copy(aco_ctx *c, int64_t bytes, int32_t reps, float *ms);
@@ -212,6 +236,10 @@ int aco_batch_set_pheromone(aco_batch *b, int32_t instance, const float *tau);
 /* config[5] = {ants per block, persistent blocks, work items, LDS bytes per
  * block, ants per wave} */
 int aco_batch_get_config(aco_batch *b, int32_t *config);
+/* Per-kernel HIP-event times of `iterations` eagerly launched iterations:
+ * kernel_ms[4] = averages of {sweep, construction, statistics, deposit}, as
+ * aco_profile. */
+int aco_batch_profile(aco_batch *b, int32_t iterations, float *kernel_ms);
 /* counters[2 * count] = per instance {fallback steps, unvisited cities
  * scanned by fallback steps} since create, as aco_get_counters */
 int aco_batch_get_counters(aco_batch *b, uint64_t *counters);

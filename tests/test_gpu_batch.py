@@ -53,6 +53,42 @@ def test_batch_first_iterations_equal_single_colonies(apb, apw, k):
                 np.testing.assert_allclose(b.pheromone(i), c.pheromone(), rtol=1e-6)
 
 
+# two ants per wave over its whole range: n in (512, 1024] (bitmap words 16-31
+# of a group, visited lookups for cities >= 512, the 32-word fallback), k 20
+# and 32, a fallback-heavy k = 3, sizes not multiples of 32
+@pytest.mark.parametrize("k,sizes", [(20, [530, 611, 777, 803, 999, 1000]), (32, [517, 577, 630]),
+                                     (3, [700, 901, 1000])])
+def test_two_ants_per_wave_large_instances_equal_one_ant_per_wave(k, sizes):
+    say(f"case: ants_per_wave 2 vs 1, k={k}, n={sizes}, 3 iterations")
+    sizes = np.array(sizes, dtype=np.int32)
+    xys = [instances.uniform(int(n), 1000, seed=900 + 7 * i + k) for i, n in enumerate(sizes)]
+    got = {}
+    for apw in (1, 2):
+        with capi.Batch(batch_params(k=k, ants_per_wave=apw, variant=capi.VARIANT_RBAS, w=1,
+                                     tau0_policy=capi.TAU0_CONST, tau0=0.001), sizes, xys) as b:
+            b.initialize()
+            assert b.config()["ants_per_wave"] == apw
+            b.run(3)
+            got[apw] = ([b.tours(i) for i in range(len(sizes))], [b.lengths(i) for i in range(len(sizes))],
+                        b.counters(), b.best()[0])
+    for i in range(len(sizes)):
+        np.testing.assert_array_equal(got[1][0][i], got[2][0][i], err_msg=f"instance {i} tours")
+        np.testing.assert_array_equal(got[1][1][i], got[2][1][i])
+    np.testing.assert_array_equal(got[1][2], got[2][2])
+    np.testing.assert_array_equal(got[1][3], got[2][3])
+    if k == 3:
+        assert (got[2][2][:, 0] > 0).all()  # fallbacks on every instance
+    # the largest instance as a single colony (stream = instance index)
+    i = len(sizes) - 1
+    n = int(sizes[i])
+    q = batch_params(n=n, m=n, k=k, stream_id=i, variant=capi.VARIANT_RBAS, w=1, tau0_policy=capi.TAU0_CONST,
+                     tau0=0.001)
+    with capi.Colony(q, xy=xys[i]) as c:
+        c.initialize()
+        c.run(3)
+        np.testing.assert_array_equal(c.tours(), got[2][0][i])
+
+
 def test_batch_above_2048_cities_equals_single_colonies():
     """n > 2048: the visited set no longer fits one register per lane and
     the batch takes its LDS-bitmap construction kernel."""

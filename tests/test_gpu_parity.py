@@ -534,6 +534,47 @@ def test_c3_full_size_properties():
         np.testing.assert_array_equal(col.lengths()[:q], ref_len)
 
 
+def test_c3_deposit_paths_full_size():
+    """The n = 20000 workload with the deposit the configuration names (fp32
+    atomics, deposit_path 1) and with the adjacency gather: from the same
+    pheromone both build the same tours, and after the deposit both pheromone
+    matrices match an FP64 evaporation + deposit on sampled rows within 1e-5
+    and each other within 2e-6."""
+    n, m, k = 20000, 1024, 32
+    say(f"case: C3 full size n={n} m={m} k={k}, fp32-atomic vs adjacency deposit, one iteration")
+    xy = instances.uniform(n, 100000, seed=3)
+    dist = orc.dist_from_int_coords(xy)
+    rows = np.sort(np.random.default_rng(11).choice(n, 48, replace=False))
+    taus, tours = {}, {}
+    for path in (1, 2):
+        p = capi.default_params(n=n, m=m, k=k, beta=2.0, rho=0.2, variant=capi.VARIANT_AS,
+                                tau0_policy=capi.TAU0_AS, seed=5, deposit_path=path)
+        with make(p, xy=xy) as col:
+            tau0 = float(col.pheromone()[0, 1])
+            col.iterate()
+            tours[path] = col.tours()
+            lens = col.lengths()
+            taus[path] = col.pheromone()[rows].astype(np.float64)
+    np.testing.assert_array_equal(tours[1], tours[2])
+    lens_ref = dist[tours[1], np.roll(tours[1], -1, axis=1)].astype(np.int64).sum(axis=1)
+    np.testing.assert_array_equal(lens, lens_ref)
+    # FP64: evaporation of the uniform tau0, then 1/L on both directions of every tour edge
+    pos = {int(r): q for q, r in enumerate(rows)}
+    ref = np.full((len(rows), n), float(np.float32(np.float32(0.8) * np.float32(tau0))), dtype=np.float64)
+    a, b = tours[1], np.roll(tours[1], -1, axis=1)
+    w = np.broadcast_to((1.0 / lens_ref.astype(np.float64))[:, None], a.shape)
+    for x, y in ((a, b), (b, a)):
+        sel = np.isin(x, rows)
+        np.add.at(ref, (np.array([pos[int(v)] for v in x[sel]]), y[sel]), w[sel])
+    for path in (1, 2):
+        rel = np.abs(taus[path] - ref) / ref
+        say(f"deposit_path {path}: max rel err vs FP64 {rel.max():.2e}")
+        assert rel.max() <= 1e-5
+    mutual = (np.abs(taus[1] - taus[2]) / ref).max()
+    say(f"atomic vs adjacency: max rel difference {mutual:.2e}")
+    assert mutual <= 2e-6
+
+
 @pytest.mark.parametrize("mode", [capi.MODE_CANDIDATE, capi.MODE_FULLROW])
 def test_run_until_halts_on_device_like_the_reference_loop(mode):
     """aco_run_until: the optimum test on the iteration best (src/TSP.cpp:

@@ -221,3 +221,53 @@ def test_trained_prior_gains_over_plain_ant_system():
         gain = 1 - best["prior"].mean() / best["plain"].mean()
         say(f"mean best plain {best['plain'].mean():.0f} prior {best['prior'].mean():.0f} gain {gain * 100:.2f}%")
         assert gain >= 0.02
+
+
+def test_c5_configuration_prior_on_n1000_replays():
+    """The learned-prior evaluation configuration: a uniform n = 1000
+    instance (integer coordinates in [0,1000]^2), m = 1000 ants, k = 20,
+    Ant System alpha 1, beta 5 in fallbacks, rho 0.5, with a prior trained on
+    a batch of n = 200 Ant System runs and applied by aco_prior_apply
+    (exp(3 score) on the candidate edges).  Features bit-exact and scores
+    within the forward tolerance on all 20,000 candidate edges, then one
+    iteration replayed through the oracle with the installed heuristic."""
+    say("case: C5 shape, prior trained on 64 x n=200, applied to n=1000 m=1000 k=20; features, scores, replay")
+    sizes = np.full(64, 200, np.int32)
+    xys = [instances.uniform(200, 1000, seed=700000 + i) for i in range(64)]
+    tp = capi.default_params(n=0, m=0, k=20, beta=5.0, rho=0.5, variant=capi.VARIANT_AS,
+                             tau0_policy=capi.TAU0_AS, seed=3)
+    xy = instances.uniform(1000, 1000, seed=900000)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=1000, m=1000, k=20, beta=5.0, rho=0.5, variant=capi.VARIANT_AS,
+                            tau0_policy=capi.TAU0_AS, seed=11)
+    cfg = capi.default_prior_config()
+    assert cfg.transform == 1 and cfg.gamma == 3.0
+    with capi.Batch(tp, sizes, xys) as tb, capi.Prior(cfg) as pr, capi.Colony(p, xy=xy) as col:
+        tb.initialize()
+        tb.run(60)
+        pr.collect_batch(tb)
+        loss = pr.train(4)
+        assert loss[-1] < loss[0]
+        params = pr.params()
+        col.initialize()
+        pr.apply(col)
+        nn = orc.nn_lists(dist, 20)
+        np.testing.assert_array_equal(col.nn(), nn)
+        f, _ = pr.features(col)
+        want_f = orc.prior_features(dist, nn)
+        np.testing.assert_array_equal(f, want_f)
+        x = want_f.reshape(-1, 8)
+        assert len(x) == 20000
+        logits = orc.mlp_forward(params, x, True)
+        got = pr.forward(x).astype(np.float64)
+        err = np.abs(got - logits) / (1 + np.abs(logits))
+        assert np.mean(err <= 1e-4) >= 0.995 and err.max() <= 2e-2
+        # the installed heuristic is max(exp(min(3 score, 80)), 1e-12) of the
+        # device scores (fp32 product and cap, expf within 1e-6)
+        h = col.heuristic_nn()
+        arg = np.minimum(np.float32(3.0) * got.astype(np.float32), np.float32(80.0)).astype(np.float64)
+        np.testing.assert_allclose(h.reshape(-1), np.maximum(np.exp(arg), 1e-12), rtol=1e-6)
+        col.run(2)
+        snap = snapshot(col)
+        col.iterate()
+        check_iteration(col, p, dist, snap, replay(p, dist, snap, heur_nn=h))
