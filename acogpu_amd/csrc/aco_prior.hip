@@ -73,11 +73,21 @@ __device__ __forceinline__ int32_t nn_dist(const PriorInstance& in, int32_t i, i
     return in.dist[(size_t)i * in.P + in.nn[(size_t)i * in.k + q]];
 }
 
-__global__ __launch_bounds__(256) void k_nn0_sum(const PriorInstance in, unsigned long long* sum) {
-    const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    unsigned long long v = i < in.n ? (unsigned long long)nn_dist(in, i, 0) : 0ull;
+// Sum over the cities of the nearest-neighbour distance, by one block (an
+// integer sum: exact in any order).  The result is written, not accumulated,
+// so no buffer has to be cleared first.
+__global__ __launch_bounds__(1024) void k_nn0_sum(const PriorInstance in, unsigned long long* sum) {
+    __shared__ unsigned long long part[16];
+    unsigned long long v = 0ull;
+    for (int32_t i = threadIdx.x; i < in.n; i += 1024) v += (unsigned long long)nn_dist(in, i, 0);
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(sum, v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0ull;
+        for (int w = 0; w < 16; ++w) t += part[w];
+        *sum = t;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_features(const PriorInstance in, const unsigned long long* sum,
@@ -107,11 +117,22 @@ __global__ __launch_bounds__(256) void k_features(const PriorInstance in, const 
 }
 
 // Position of every city on a tour, then the label of each candidate edge:
-// 1 when j follows or precedes i on the tour.
+// 1 when j follows or precedes i on the tour.  A colony that has not built a
+// tour yet holds no permutation (its best tour is zeroed), so positions
+// start at -1, entries outside [0, n) are skipped, and a city without a
+// position labels its edges 0.
+__global__ __launch_bounds__(256) void k_pos_clear(int32_t n, int32_t* __restrict__ pos) {
+    const int32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n) pos[t] = -1;
+}
+
 __global__ __launch_bounds__(256) void k_tour_pos(const int32_t* __restrict__ tour, int32_t n,
                                                   int32_t* __restrict__ pos) {
     const int32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t < n) pos[tour[t]] = t;
+    if (t < n) {
+        const int32_t c = tour[t];
+        if ((uint32_t)c < (uint32_t)n) pos[c] = t;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_labels(const PriorInstance in, const int32_t* __restrict__ tour,
@@ -120,6 +141,10 @@ __global__ __launch_bounds__(256) void k_labels(const PriorInstance in, const in
     if (e >= in.n * in.k) return;
     const int32_t i = e / in.k, j = in.nn[e];
     const int32_t t = pos[i];
+    if ((uint32_t)t >= (uint32_t)in.n) {
+        y[e] = 0.0f;
+        return;
+    }
     const int32_t nx = tour[t + 1 < in.n ? t + 1 : 0], pv = tour[t > 0 ? t - 1 : in.n - 1];
     y[e] = (nx == j || pv == j) ? 1.0f : 0.0f;
 }
@@ -447,14 +472,13 @@ __global__ __launch_bounds__(256) void k_logit_to_heuristic(const float* __restr
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 hipError_t launch_features(const PriorInstance& in, unsigned long long* sum, float* feat, hipStream_t s) {
-    hipError_t e = hipMemsetAsync(sum, 0, sizeof(unsigned long long), s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_nn0_sum, dim3(cdiv(in.n, 256)), dim3(256), 0, s, in, sum);
+    hipLaunchKernelGGL(k_nn0_sum, dim3(1), dim3(1024), 0, s, in, sum);
     hipLaunchKernelGGL(k_features, dim3(cdiv((int64_t)in.n * in.k, 256)), dim3(256), 0, s, in, sum, feat);
     return hipGetLastError();
 }
 
 hipError_t launch_labels(const PriorInstance& in, const int32_t* tour, int32_t* pos, float* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_pos_clear, dim3(cdiv(in.n, 256)), dim3(256), 0, s, in.n, pos);
     hipLaunchKernelGGL(k_tour_pos, dim3(cdiv(in.n, 256)), dim3(256), 0, s, tour, in.n, pos);
     hipLaunchKernelGGL(k_labels, dim3(cdiv((int64_t)in.n * in.k, 256)), dim3(256), 0, s, in, tour, pos, y);
     return hipGetLastError();

@@ -49,6 +49,32 @@ def test_features_and_labels_bit_exact():
         assert 0 < y.sum() <= 2 * 300
 
 
+def test_labels_before_any_tour_are_zero():
+    """A colony that has built no tour yet has no best tour (it is zeroed):
+    its candidate edges are labelled 0, and after one iteration they follow
+    the best tour again."""
+    say("case: prior labels before the first tour, n=500 k=20 (positions buffer grown from n=120)")
+    with capi.Prior() as pr:
+        xy0 = instances.uniform(120, 1000, seed=2)
+        p0 = capi.default_params(n=120, m=16, k=20, beta=2.0, rho=0.3, seed=2)
+        with capi.Colony(p0, xy=xy0) as c0:
+            c0.initialize()
+            c0.run(2)
+            pr.features(c0)  # positions buffer sized for 120 cities
+        xy = instances.uniform(500, 1000, seed=3)
+        dist = orc.dist_from_int_coords(xy)
+        p = capi.default_params(n=500, m=32, k=20, beta=2.0, rho=0.3, seed=3)
+        with capi.Colony(p, xy=xy) as col:
+            col.initialize()
+            f, y = pr.features(col)
+            nn = orc.nn_lists(dist, 20)
+            np.testing.assert_array_equal(f, orc.prior_features(dist, nn))
+            assert not y.any()
+            col.iterate()
+            _, y1 = pr.features(col)
+            np.testing.assert_array_equal(y1, orc.prior_labels(nn, col.best_tour()))
+
+
 def test_forward_one_path_closed_form_on_mfma():
     """Single nonzero path through W1[3][5] and W2[7][3]: pins the MFMA operand
     layout (a transposed operand reads another input or unit)."""
