@@ -107,6 +107,7 @@ SIGNATURES = {
     "aco_prior_train": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_prior_apply": (ctypes.c_int, [_vp, _vp]),
     "aco_prior_apply_batch": (ctypes.c_int, [_vp, _vp]),
+    "aco_prior_apply_noalloc": (ctypes.c_int, [_vp, _vp]),
     "aco_prior_features": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "aco_prior_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
     "aco_prior_gradient": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
@@ -565,6 +566,10 @@ class Prior:
 
     def apply_batch(self, batch: "Batch"):
         self._chk(self._L.aco_prior_apply_batch(self._ctx, batch._ctx))
+
+    def apply_noalloc(self, colony: "Colony"):
+        """aco_prior_apply with no device allocation (library scratch array)."""
+        self._chk(self._L.aco_prior_apply_noalloc(self._ctx, colony._ctx))
 
     def features(self, colony: "Colony"):
         n, k = colony.n, colony.params.k

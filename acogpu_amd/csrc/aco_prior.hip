@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "aco_device.h"
 #include "aco_prior.h"
@@ -857,6 +858,64 @@ extern "C" int aco_prior_apply(aco_prior* pr, aco_ctx* c) {
     InstanceView v;
     int rc = aco::colony_view(c, &v, pr->err, sizeof(pr->err));
     if (rc || (rc = apply_view(pr, v))) return rc;
+    PHIP(pr, hipStreamSynchronize(pr->stream));
+    return ACO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free apply.  Its features and scores live in `scratch`, a
+// file-scope device array of fixed size (NOALLOC_MAX_EDGES candidate edges:
+// 256 bytes for the nearest-neighbour distance sum, then 8 features and one
+// score per edge, 9.0 MiB in all), which every GPU the library runs on holds
+// until the process ends.  Calls that overlap on one GPU share it and race.
+// k_nn0_sum writes the sum (nothing needs clearing) and the features and
+// scores are written before they are read, so the previous call's contents
+// never matter.
+constexpr int64_t NOALLOC_MAX_EDGES = int64_t(1) << 18;
+constexpr size_t NOALLOC_HEAD = 64;  // floats: the 64-bit sum, padded to 256 bytes
+static __device__ __align__(256) float scratch[NOALLOC_HEAD + 9 * NOALLOC_MAX_EDGES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the array's name is only a host-side stand-in).
+static int noalloc_scratch(aco_prior* pr, float** out) {
+    int dev = 0;
+    PHIP(pr, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return pfail(pr, ACO_E_RANGE, "device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr)
+            return pfail(pr, ACO_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return ACO_OK;
+}
+
+extern "C" int aco_prior_apply_noalloc(aco_prior* pr, aco_ctx* c) {
+    if (!pr) return ACO_E_ARG;
+    InstanceView v;
+    int rc = aco::colony_view(c, &v, pr->err, sizeof(pr->err));
+    if (rc) return rc;
+    const int64_t edges = (int64_t)v.in.n * v.in.k;
+    if (edges > NOALLOC_MAX_EDGES)
+        return pfail(pr, ACO_E_RANGE,
+                     "n*k = %lld candidate edges exceeds the allocation-free limit of %lld (use aco_prior_apply)",
+                     (long long)edges, (long long)NOALLOC_MAX_EDGES);
+    float* base = nullptr;
+    if ((rc = noalloc_scratch(pr, &base))) return rc;
+    unsigned long long* sum = reinterpret_cast<unsigned long long*>(base);
+    float* feat = base + NOALLOC_HEAD;
+    float* logit = feat + 8 * NOALLOC_MAX_EDGES;
+    PHIP(pr, hipStreamSynchronize(v.stream));
+    PHIP(pr, aco::launch_features(v.in, sum, feat, pr->stream));
+    PHIP(pr, aco::launch_mlp_forward(pr->params(), feat, edges, logit, pr->stream));
+    PHIP(pr, aco::launch_logit_to_heuristic(logit, edges, pr->cfg.gamma, pr->cfg.transform, v.eta_nn, pr->stream));
     PHIP(pr, hipStreamSynchronize(pr->stream));
     return ACO_OK;
 }

@@ -322,6 +322,14 @@ int aco_prior_train(aco_prior *pr, int32_t epochs, float *loss);
  * colony / every batch instance (after aco_initialize, which resets them). */
 int aco_prior_apply(aco_prior *pr, aco_ctx *c);
 int aco_prior_apply_batch(aco_prior *pr, aco_batch *b);
+/* aco_prior_apply without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code in
+ * tight serving loops, harnesses that forbid hipMalloc).  The features and
+ * scores live in a fixed library scratch array of 9.0 MiB per GPU, held
+ * until the process ends; colonies with n*k > 262144 candidate edges are
+ * refused with ACO_E_RANGE.  Calls that overlap on one GPU share that array
+ * and race.  The heuristic written equals aco_prior_apply's bit for bit. */
+int aco_prior_apply_noalloc(aco_prior *pr, aco_ctx *c);
 /* Host-array taps: features (n*k*8) and labels (n*k) of a colony's edges;
  * scores of `count` feature rows; loss gradient (no update); one Adam step
  * from a given gradient. */
