@@ -52,6 +52,7 @@ SIGNATURES = {
     "aco_run_until": (ctypes.c_int, [_vp, _i32, _i64, _i32, _vp, _vp]),
     "aco_phase": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_phase_end": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "aco_iterate_stats": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "aco_probe_step": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_batch_profile": (ctypes.c_int, [_vp, _i32, _vp]),
     "aco_run_until_time": (ctypes.c_int, [_vp, _i32, _i64, ctypes.c_float, _i32, _vp, _vp, _vp]),
@@ -214,6 +215,13 @@ class Colony:
         t = np.zeros(6, dtype=np.float32)
         self._chk(self._L.aco_iterate(self._ctx, _p(t)))
         return t
+
+    def iterate_stats(self):
+        """(timers[6], iter_best, glob_best, valid ants) of aco_iterate_stats."""
+        t = np.zeros(6, dtype=np.float32)
+        ib, gb, nv = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_uint32()
+        self._chk(self._L.aco_iterate_stats(self._ctx, _p(t), ctypes.byref(ib), ctypes.byref(gb), ctypes.byref(nv)))
+        return t, ib.value, gb.value, nv.value
 
     def phase(self, phase, wait=True):
         """One phase; wait=False enqueues it without waiting (complete the

@@ -59,9 +59,10 @@ struct aco_ctx {
     hipGraphExec_t graph = nullptr;       // two iterations starting at cur == 0
     hipGraphExec_t graph_long = nullptr;  // kGraphLong iterations starting at cur == 0
     hipEvent_t ev[6] = {};
-    hipEvent_t pev[5][2] = {};  // asynchronous phases: start/end event per phase
+    hipEvent_t pev[6] = {};     // asynchronous phases: iteration start, then the end of phase p at 1 + p
     uint32_t phases_async = 0;  // bit p: phase p enqueued without waiting since the iteration began
-    DevState* h_state = nullptr;  // pinned copy of d_state taken after an asynchronous update phase
+    hipEvent_t tev[5] = {};     // timed iteration: start of its first kernel, then the stop of each phase
+    int64_t* h_stats = nullptr; // coherent pinned host memory: {iteration best, best so far, valid ants}
     char err[512] = {0};
 };
 
@@ -123,10 +124,11 @@ static void free_all(aco_ctx* c) {
         if (q) hipFree(q);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
-    for (auto& pe : c->pev)
-        for (auto& e : pe)
-            if (e) hipEventDestroy(e);
-    if (c->h_state) hipHostFree(c->h_state);
+    for (auto& e : c->pev)
+        if (e) hipEventDestroy(e);
+    for (auto& e : c->tev)
+        if (e) hipEventDestroy(e);
+    if (c->h_stats) hipHostFree(c->h_stats);
     if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -262,9 +264,10 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
 
     HIPB(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPB(hipEventCreate(&e));
-    for (auto& pe : c->pev)
-        for (auto& e : pe) HIPB(hipEventCreate(&e));
-    HIPB(hipHostMalloc(reinterpret_cast<void**>(&c->h_state), sizeof(DevState), hipHostMallocDefault));
+    for (auto& e : c->pev) HIPB(hipEventCreate(&e));
+    for (auto& e : c->tev) HIPB(hipEventCreate(&e));
+    HIPB(hipHostMalloc(reinterpret_cast<void**>(&c->h_stats), 4 * sizeof(int64_t), hipHostMallocCoherent));
+    std::memset(c->h_stats, 0, 4 * sizeof(int64_t));
     const size_t nP = (size_t)c->n * c->P;
     TRYB(dalloc(c, &c->d_dist, nP));
     TRYB(dalloc(c, &c->d_tau[0], nP));
@@ -432,7 +435,7 @@ extern "C" int aco_initialize(aco_ctx* c) {
 
 // The kernels of one iteration, one function per phase, on the ctx stream;
 // they read pheromone buffer `cur` and produce buffer 1-cur.
-static hipError_t enqueue_sweep(aco_ctx* c, int cur) {
+static hipError_t enqueue_sweep(aco_ctx* c, int cur, const aco::LaunchTiming& t) {
     hipError_t e;
     float* tau = c->d_tau[cur];
     float* tau_next = c->d_tau[1 - cur];
@@ -465,11 +468,11 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur) {
     sa.state = c->d_state;
     sa.top = c->d_top;
     sa.top_thr = c->d_top_thr;
-    if ((e = aco::launch_sweep(sa, c->stream)) != hipSuccess) return e;
+    if ((e = aco::launch_sweep(sa, c->stream, t)) != hipSuccess) return e;
     return hipSuccess;
 }
 
-static hipError_t enqueue_construct(aco_ctx* c, int cur) {
+static hipError_t enqueue_construct(aco_ctx* c, int cur, const aco::LaunchTiming& t) {
     hipError_t e;
     float* tau = c->d_tau[cur];
     float* tau_next = c->d_tau[1 - cur];
@@ -508,11 +511,11 @@ static hipError_t enqueue_construct(aco_ctx* c, int cur) {
     ca.words = c->words;
     ca.young_blocks = c->young_blocks;
     ca.top = c->d_top;
-    if ((e = aco::launch_construct(ca, c->path, c->apb, c->stream)) != hipSuccess) return e;
+    if ((e = aco::launch_construct(ca, c->path, c->apb, c->stream, t)) != hipSuccess) return e;
     return hipSuccess;
 }
 
-static hipError_t enqueue_stats(aco_ctx* c, int cur) {
+static hipError_t enqueue_stats(aco_ctx* c, int cur, bool mirror, const aco::LaunchTiming& t) {
     hipError_t e;
     float* tau = c->d_tau[cur];
     float* tau_next = c->d_tau[1 - cur];
@@ -539,11 +542,12 @@ static hipError_t enqueue_stats(aco_ctx* c, int cur) {
     st.max_age = c->p.max_age;
     st.global_perf = c->p.global_perf;
     st.min_pop = c->p.min_pop;
-    if ((e = aco::launch_stats(st, c->stream)) != hipSuccess) return e;
+    st.host_stats = mirror ? c->h_stats : nullptr;
+    if ((e = aco::launch_stats(st, c->stream, t)) != hipSuccess) return e;
     return hipSuccess;
 }
 
-static hipError_t enqueue_deposit(aco_ctx* c, int cur) {
+static hipError_t enqueue_deposit(aco_ctx* c, int cur, const aco::LaunchTiming& t) {
     hipError_t e;
     float* tau = c->d_tau[cur];
     float* tau_next = c->d_tau[1 - cur];
@@ -563,18 +567,21 @@ static hipError_t enqueue_deposit(aco_ctx* c, int cur) {
     da.dense = c->p.variant == ACO_VARIANT_AS;
     da.adj = c->d_adj;
     da.m = c->m;
-    if (da.dense && c->d_adj) return aco::launch_deposit_adj(da, c->stream);
+    if (da.dense && c->d_adj) return aco::launch_deposit_adj(da, c->stream, t);
     const int32_t grid = da.dense ? c->m : c->p.w + 1;
-    if ((e = aco::launch_deposit(da, grid, c->stream)) != hipSuccess) return e;
+    if ((e = aco::launch_deposit(da, grid, c->stream, t)) != hipSuccess) return e;
     return hipSuccess;
 }
 
-static hipError_t enqueue_phase(aco_ctx* c, int cur, int phase) {
+// mirror: the statistics kernel also writes its results to h_stats; t: the
+// phase's first kernel records t.start and its last t.stop (may be null)
+static hipError_t enqueue_phase(aco_ctx* c, int cur, int phase, bool mirror = false,
+                                const aco::LaunchTiming& t = aco::LaunchTiming()) {
     switch (phase) {
-        case ACO_PHASE_PROBABILITIES: return enqueue_sweep(c, cur);
-        case ACO_PHASE_TOUR: return enqueue_construct(c, cur);
-        case ACO_PHASE_ANT_DISTANCE: return enqueue_stats(c, cur);
-        case ACO_PHASE_UPDATE_PHEROMONE: return enqueue_deposit(c, cur);
+        case ACO_PHASE_PROBABILITIES: return enqueue_sweep(c, cur, t);
+        case ACO_PHASE_TOUR: return enqueue_construct(c, cur, t);
+        case ACO_PHASE_ANT_DISTANCE: return enqueue_stats(c, cur, mirror, t);
+        case ACO_PHASE_UPDATE_PHEROMONE: return enqueue_deposit(c, cur, t);
         default: return hipSuccess;  // evaporation is part of the probability sweep
     }
 }
@@ -592,28 +599,60 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     return hipSuccess;
 }
 
-extern "C" int aco_iterate(aco_ctx* c, float* timers) {
+// One timed iteration: the four phases launched in order, the first kernel
+// recording the start and each phase's last kernel its stop from its own
+// dispatch (no marker packets), the statistics kernel writing the
+// iteration's statistics to h_stats; one wait.
+static hipError_t enqueue_timed_iteration(aco_ctx* c, int cur) {
+    static const int order[4] = {ACO_PHASE_PROBABILITIES, ACO_PHASE_TOUR, ACO_PHASE_ANT_DISTANCE,
+                                 ACO_PHASE_UPDATE_PHEROMONE};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) {
+        aco::LaunchTiming t;
+        t.start = i == 0 ? c->tev[0] : nullptr;
+        t.stop = c->tev[i + 1];
+        e = enqueue_phase(c, cur, order[i], true, t);
+    }
+    return e;
+}
+
+static void fill_timers(float* timers, const float t[4]) {
+    // Placement is fused into construction and evaporation into the
+    // probability sweep: their slots report 0, the sweep reports under
+    // TIME_COMPUTE_PROBABILITIES.
+    timers[ACO_TIME_INIT_TOUR] = 0.0f;
+    timers[ACO_TIME_TOUR] = t[1];
+    timers[ACO_TIME_COMPUTE_ANT_DISTANCE] = t[2];
+    timers[ACO_TIME_EVAPORATE_PHEROMONE] = 0.0f;
+    timers[ACO_TIME_UPDATE_PHEROMONE] = t[3];
+    timers[ACO_TIME_COMPUTE_PROBABILITIES] = t[0];
+}
+
+static void read_host_stats(const aco_ctx* c, int64_t* iter_best, int64_t* glob_best, uint32_t* valid_ants) {
+    const volatile int64_t* h = c->h_stats;
+    if (iter_best) *iter_best = h[0];
+    if (glob_best) *glob_best = h[1];
+    if (valid_ants) *valid_ants = (uint32_t)h[2];
+}
+
+extern "C" int aco_iterate_stats(aco_ctx* c, float* timers, int64_t* iter_best, int64_t* glob_best,
+                                 uint32_t* valid_ants) {
     if (!c) return ACO_E_ARG;
     if (!c->initialized) return fail(c, ACO_E_STATE, "aco_iterate before aco_initialize");
     if (c->next_phase != ACO_PHASE_PROBABILITIES) return fail(c, ACO_E_STATE, "aco_iterate inside a phased iteration");
-    HIP_TRY(c, enqueue_iteration(c, c->cur, c->ev));
+    HIP_TRY(c, enqueue_timed_iteration(c, c->cur));
     c->cur ^= 1;
-    HIP_TRY(c, hipEventSynchronize(c->ev[4]));
+    HIP_TRY(c, hipEventSynchronize(c->tev[4]));
     if (timers) {
         float t[4] = {0, 0, 0, 0};
-        for (int i = 0; i < 4; ++i) HIP_TRY(c, hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
-        // Placement is fused into construction and evaporation into the
-        // probability sweep: their slots report 0, the sweep reports under
-        // TIME_COMPUTE_PROBABILITIES.
-        timers[ACO_TIME_INIT_TOUR] = 0.0f;
-        timers[ACO_TIME_TOUR] = t[1];
-        timers[ACO_TIME_COMPUTE_ANT_DISTANCE] = t[2];
-        timers[ACO_TIME_EVAPORATE_PHEROMONE] = 0.0f;
-        timers[ACO_TIME_UPDATE_PHEROMONE] = t[3];
-        timers[ACO_TIME_COMPUTE_PROBABILITIES] = t[0];
+        for (int i = 0; i < 4; ++i) HIP_TRY(c, hipEventElapsedTime(&t[i], c->tev[i], c->tev[i + 1]));
+        fill_timers(timers, t);
     }
+    read_host_stats(c, iter_best, glob_best, valid_ants);
     return ACO_OK;
 }
+
+extern "C" int aco_iterate(aco_ctx* c, float* timers) { return aco_iterate_stats(c, timers, nullptr, nullptr, nullptr); }
 
 // Graphs of an even number of iterations, starting with pheromone buffer 0.
 // Each graph launch leaves a short gap before its first kernel, so long runs
@@ -737,14 +776,23 @@ extern "C" int aco_run_until_time(aco_ctx* c, int32_t max_iterations, int64_t st
 extern "C" int aco_profile(aco_ctx* c, int32_t iterations, float* kernel_ms) {
     if (!c || !kernel_ms || iterations < 1) return ACO_E_ARG;
     if (!c->initialized) return fail(c, ACO_E_STATE, "aco_profile before aco_initialize");
+    // each phase's duration from its own dispatches (start of its first
+    // kernel to the stop of its last)
+    static const int order[4] = {ACO_PHASE_PROBABILITIES, ACO_PHASE_TOUR, ACO_PHASE_ANT_DISTANCE,
+                                 ACO_PHASE_UPDATE_PHEROMONE};
     double acc[4] = {0, 0, 0, 0};
     for (int32_t it = 0; it < iterations; ++it) {
-        HIP_TRY(c, enqueue_iteration(c, c->cur, c->ev));
+        for (int i = 0; i < 4; ++i) {
+            aco::LaunchTiming t;
+            t.start = c->pev[i];
+            t.stop = c->tev[i];
+            HIP_TRY(c, enqueue_phase(c, c->cur, order[i], false, t));
+        }
         c->cur ^= 1;
-        HIP_TRY(c, hipEventSynchronize(c->ev[4]));
+        HIP_TRY(c, hipEventSynchronize(c->tev[3]));
         for (int i = 0; i < 4; ++i) {
             float t = 0;
-            HIP_TRY(c, hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+            HIP_TRY(c, hipEventElapsedTime(&t, c->pev[i], c->tev[i]));
             acc[i] += t;
         }
     }
@@ -982,21 +1030,30 @@ extern "C" int aco_phase(aco_ctx* c, int32_t phase, float* ms) {
             return fail(c, ACO_E_STATE, "phase %d out of order (expected %d)", phase, c->next_phase);
     }
     if (ms) {
-        HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
-        HIP_TRY(c, enqueue_phase(c, c->cur, phase));
-        HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
-        HIP_TRY(c, hipEventSynchronize(c->ev[1]));
-        HIP_TRY(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+        if (phase == ACO_PHASE_EVAPORATE) {
+            *ms = 0.0f;  // folded into the probability sweep: no kernel
+        } else {
+            aco::LaunchTiming t;
+            t.start = c->ev[0];
+            t.stop = c->ev[1];
+            HIP_TRY(c, enqueue_phase(c, c->cur, phase, false, t));
+            HIP_TRY(c, hipEventSynchronize(c->ev[1]));
+            HIP_TRY(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+        }
         c->phases_async = 0;
     } else {
-        // no wait: the phase's events are kept for aco_phase_end
+        // no wait: one event at the iteration's start and one after each
+        // phase, read by aco_phase_end; the statistics go to h_stats
         if (phase == ACO_PHASE_PROBABILITIES) c->phases_async = 0;
-        HIP_TRY(c, hipEventRecord(c->pev[phase][0], c->stream));
-        HIP_TRY(c, enqueue_phase(c, c->cur, phase));
-        HIP_TRY(c, hipEventRecord(c->pev[phase][1], c->stream));
+        if (phase == ACO_PHASE_EVAPORATE) {
+            HIP_TRY(c, hipEventRecord(c->pev[1 + phase], c->stream));  // no kernel: a marker
+        } else {
+            aco::LaunchTiming t;
+            t.start = phase == ACO_PHASE_PROBABILITIES ? c->pev[0] : nullptr;
+            t.stop = c->pev[1 + phase];
+            HIP_TRY(c, enqueue_phase(c, c->cur, phase, phase == ACO_PHASE_ANT_DISTANCE, t));
+        }
         c->phases_async |= 1u << phase;
-        if (phase == ACO_PHASE_UPDATE_PHEROMONE)
-            HIP_TRY(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
     }
     c->next_phase = next_of[phase];
     if (phase == ACO_PHASE_UPDATE_PHEROMONE) c->cur ^= 1;
@@ -1011,10 +1068,15 @@ extern "C" int aco_phase_end(aco_ctx* c, float* timers, int64_t* iter_best, int6
     if (!c) return ACO_E_ARG;
     if (!(c->phases_async & (1u << ACO_PHASE_UPDATE_PHEROMONE)))
         return fail(c, ACO_E_STATE, "aco_phase_end without an iteration of asynchronous phases (ms == NULL)");
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->pev[1 + ACO_PHASE_UPDATE_PHEROMONE]));
     float t[5] = {0, 0, 0, 0, 0};
+    hipEvent_t prev = c->pev[0];
     for (int ph = 0; ph < 5; ++ph)
-        if (c->phases_async & (1u << ph)) HIP_TRY(c, hipEventElapsedTime(&t[ph], c->pev[ph][0], c->pev[ph][1]));
+        if (c->phases_async & (1u << ph)) {
+            HIP_TRY(c, hipEventElapsedTime(&t[ph], prev, c->pev[1 + ph]));
+            prev = c->pev[1 + ph];
+        }
+    const uint32_t done = c->phases_async;
     c->phases_async = 0;
     if (timers) {
         timers[ACO_TIME_INIT_TOUR] = 0.0f;  // placement runs inside the tour kernel
@@ -1024,9 +1086,17 @@ extern "C" int aco_phase_end(aco_ctx* c, float* timers, int64_t* iter_best, int6
         timers[ACO_TIME_UPDATE_PHEROMONE] = t[ACO_PHASE_UPDATE_PHEROMONE];
         timers[ACO_TIME_COMPUTE_PROBABILITIES] = t[ACO_PHASE_PROBABILITIES];
     }
-    if (iter_best) *iter_best = c->h_state->iter_best;
-    if (glob_best) *glob_best = c->h_state->glob_best;
-    if (valid_ants) *valid_ants = c->p.aging ? (uint32_t)c->h_state->num_valid : (uint32_t)c->m;
+    if ((done & (1u << ACO_PHASE_ANT_DISTANCE)) == 0) {  // its statistics were not mirrored
+        int64_t ib = 0, gb = 0;
+        uint32_t nv = 0;
+        int rc = aco_get_stats(c, &ib, &gb, &nv);
+        if (rc) return rc;
+        if (iter_best) *iter_best = ib;
+        if (glob_best) *glob_best = gb;
+        if (valid_ants) *valid_ants = nv;
+        return ACO_OK;
+    }
+    read_host_stats(c, iter_best, glob_best, valid_ants);
     return ACO_OK;
 }
 

@@ -103,6 +103,10 @@ struct StatsArgs {
     DevState* state;
     int32_t n, m, variant, w;
     int32_t aging, max_age, global_perf, min_pop;
+    // nullable: {iteration best, best so far, valid ants} written to host
+    // (coherent pinned) memory after the statistics, so the host reads them
+    // after the iteration with no copy
+    int64_t* host_stats;
 };
 
 struct DepositArgs {
@@ -254,7 +258,15 @@ hipError_t launch_fill_rows(float* a, int32_t n, int32_t P, float v, hipStream_t
 // nn: n x k candidate lists (k < n) to shortcut the steps, or null.
 hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t start, int64_t* out, const int32_t* nn,
                          int32_t k, hipStream_t s);
-hipError_t launch_sweep(const SweepArgs& a, hipStream_t s);
+// Timing of one phase's launches: its first kernel records `start` and its
+// last kernel `stop` from their own dispatches (hipExtLaunchKernel), which
+// adds no marker packets to the stream; both null: plain launches.
+struct LaunchTiming {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+
+hipError_t launch_sweep(const SweepArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());
 // Full-row construction: chunks of 64 candidates whose list entries and
 // weights are loaded together; LDS words per ant (visited bitmap, the weight
 // row in city order, then per candidate its weight, prefix and city, then
@@ -269,14 +281,16 @@ __host__ __device__ inline size_t full_lds_words(int32_t n, int32_t words) {
 // Construction paths: 0 = candidate lists with global reads, 1 = full row,
 // 2 = candidate lists staged in LDS (n*k*6 bytes + per-ant state per block).
 size_t staged_lds_bytes(int32_t n, int32_t k, int32_t words, int32_t ants_per_block);
-hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_per_block, hipStream_t s);
+hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_per_block, hipStream_t s,
+                            const LaunchTiming& t = LaunchTiming());
 hipError_t launch_nn_aux(const int32_t* nn, const int32_t* dist, const float* eta, int32_t eta_len, int32_t n,
                          int32_t P, int32_t k, uint16_t* nn16, float* eta_nn, hipStream_t s);
-hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
-hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s);
+hipError_t launch_stats(const StatsArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());
+hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s,
+                          const LaunchTiming& t = LaunchTiming());
 // Ant System deposit without global atomics: tours scattered into adj, then
 // every pheromone row gathers its deposits in LDS (needs a.adj, a.m, n <= ADJ_MAX_N).
-hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s);
+hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());
 hipError_t launch_copy(const float4* src, float4* dst, int64_t count, hipStream_t s);
 hipError_t launch_probe_step(int32_t iters, int32_t k, unsigned long long* out, int32_t* sink, hipStream_t s);
 

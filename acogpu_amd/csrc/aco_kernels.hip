@@ -12,6 +12,8 @@
 // expression is evaluated as written, one IEEE operation at a time, matching
 // the CPU oracle.
 
+#include <hip/hip_ext.h>
+
 #include "aco_device.h"
 #include "aco_kernels.h"
 
@@ -1755,6 +1757,16 @@ __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
     // changes only below, after the deposit list is made)
     if (threadIdx.x == 0) a.state->tours += a.aging ? (unsigned long long)a.state->num_valid : (unsigned long long)a.m;
     stats_block(a, red, redi, taken, s_improved, s_recruit, red_rank, sh);
+    if (a.host_stats != nullptr) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const DevState* st = a.state;
+            a.host_stats[0] = st->iter_best;
+            a.host_stats[1] = st->glob_best;
+            a.host_stats[2] = a.aging ? (int64_t)st->num_valid : (int64_t)a.m;
+            __threadfence_system();
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2633,9 +2645,20 @@ hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t star
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(const SweepArgs& a, hipStream_t s) {
+// A launch that records the events it is given (start and/or stop) from
+// its own dispatch; without events a plain launch.
+template <typename F, typename... Args>
+static inline void launch_k(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, hipEvent_t start,
+                            hipEvent_t stop, Args... args) {
+    if (start != nullptr || stop != nullptr)
+        hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)lds, s, start, stop, 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+}
+
+hipError_t launch_sweep(const SweepArgs& a, hipStream_t s, const LaunchTiming& t) {
     const size_t lds = a.top != nullptr ? (size_t)4 * TOP_BUF * sizeof(uint64_t) : 0;
-    hipLaunchKernelGGL(k_sweep, dim3(cdiv(a.n, 4)), dim3(256), lds, s, a);
+    launch_k(k_sweep, dim3(cdiv(a.n, 4)), dim3(256), lds, s, t.start, t.stop, a);
     return hipGetLastError();
 }
 
@@ -2645,17 +2668,18 @@ size_t staged_lds_bytes(int32_t n, int32_t k, int32_t words, int32_t ants_per_bl
            (size_t)ants_per_block * (words + (n + 1) / 2) * 4;
 }
 
-hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_per_block, hipStream_t s) {
+hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_per_block, hipStream_t s,
+                            const LaunchTiming& t) {
     const int W = ants_per_block;
     if (path == 0) {
         const size_t lds = (size_t)W * a.words * sizeof(uint32_t);
-        hipLaunchKernelGGL(k_construct_cand, dim3(cdiv(a.m, W)), dim3(64 * W), lds, s, a);
+        launch_k(k_construct_cand, dim3(cdiv(a.m, W)), dim3(64 * W), lds, s, t.start, t.stop, a);
     } else if (path == 2) {
         const size_t lds = staged_lds_bytes(a.n, a.k, a.words, W);
-        hipLaunchKernelGGL(k_construct_staged, dim3(cdiv(a.m, W) + a.young_blocks), dim3(64 * W), lds, s, a);
+        launch_k(k_construct_staged, dim3(cdiv(a.m, W) + a.young_blocks), dim3(64 * W), lds, s, t.start, t.stop, a);
     } else {
         const size_t lds = (size_t)W * full_lds_words(a.n, a.words) * 4;
-        hipLaunchKernelGGL(k_construct_full, dim3(cdiv(a.m, W)), dim3(64 * W), lds, s, a);
+        launch_k(k_construct_full, dim3(cdiv(a.m, W)), dim3(64 * W), lds, s, t.start, t.stop, a);
     }
     return hipGetLastError();
 }
@@ -2667,28 +2691,30 @@ hipError_t launch_nn_aux(const int32_t* nn, const int32_t* dist, const float* et
     return hipGetLastError();
 }
 
-hipError_t launch_stats(const StatsArgs& a, hipStream_t s) {
+hipError_t launch_stats(const StatsArgs& a, hipStream_t s, const LaunchTiming& t) {
     const size_t lds = (size_t)((a.m + 31) / 32) * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_stats, dim3(1), dim3(1024), lds, s, a);
+    launch_k(k_stats, dim3(1), dim3(1024), lds, s, t.start, t.stop, a);
     return hipGetLastError();
 }
 
-hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s, const LaunchTiming& t) {
+    launch_k(k_deposit, dim3(grid), dim3(256), 0, s, t.start, t.stop, a);
     return hipGetLastError();
 }
 
-hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s) {
+hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s, const LaunchTiming& t) {
     if (!a.adj || !a.dense || a.n > ADJ_MAX_N) return hipErrorInvalidValue;
     const int32_t A = adj_stage_ants(a.n);
     if (A) {
-        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(1024), adj_stage_lds(a.n, A), s, a, A, a.n);
+        launch_k(k_adj_scatter_staged, dim3(cdiv(a.m, A)), dim3(1024), adj_stage_lds(a.n, A), s, t.start, nullptr, a,
+                 A, a.n);
     } else {  // bands of ADJ_BAND_ROWS rows, ADJ_BAND_ANTS ants per block
         const int32_t bands = cdiv(a.n, ADJ_BAND_ROWS);
-        hipLaunchKernelGGL(k_adj_scatter_staged, dim3(cdiv(a.m, ADJ_BAND_ANTS) * bands), dim3(1024),
-                           adj_stage_lds(ADJ_BAND_ROWS, ADJ_BAND_ANTS), s, a, ADJ_BAND_ANTS, ADJ_BAND_ROWS);
+        launch_k(k_adj_scatter_staged, dim3(cdiv(a.m, ADJ_BAND_ANTS) * bands), dim3(1024),
+                 adj_stage_lds(ADJ_BAND_ROWS, ADJ_BAND_ANTS), s, t.start, nullptr, a, (int32_t)ADJ_BAND_ANTS,
+                 (int32_t)ADJ_BAND_ROWS);
     }
-    hipLaunchKernelGGL(k_adj_gather, dim3(a.n), dim3(1024), (size_t)a.n * sizeof(float), s, a);
+    launch_k(k_adj_gather, dim3(a.n), dim3(1024), (size_t)a.n * sizeof(float), s, nullptr, t.stop, a);
     return hipGetLastError();
 }
 

@@ -74,8 +74,7 @@ void Writer::write(const Statistics& s) {
           << std::setw(W) << s.globalbest << std::setw(W) << s.t_itertime << std::setw(W) << s.t_globaltime
           << std::setw(W) << s.t_inittour << std::setw(W) << s.t_tour << std::setw(W * 2) << s.t_antdist
           << std::setw(W * 2) << s.t_evapphoremone << std::setw(W * 2) << s.t_updatephoremone << std::setw(W * 2)
-          << s.t_computeprob << "\n"
-          << std::flush;
+          << s.t_computeprob << "\n";  // buffered: the stream is flushed when the file closes
     }
     if (VERBOSE) {
         std::cout << std::left << std::setw(W) << s.iter << std::setw(W) << s.ants << std::setw(W) << s.iterbest

@@ -1,7 +1,9 @@
 // Colony over the C ABI.  The iteration order of constructSolution follows
-// src/Colony.cu:537-594; each step is one device phase (aco_phase), enqueued
-// without waiting, and aco_phase_end waits once per iteration for the phase
-// times and the statistics the caller reads next.
+// src/Colony.cu:537-594: its steps, including the virtual ones of the
+// subclasses, name device phases, which run when the iteration's sequence is
+// complete -- as one graph-launched iteration (aco_iterate_stats) when it is
+// the standard sequence, else phase by phase -- with one wait per iteration
+// for the phase times and the statistics the caller reads next.
 #include "Colony.h"
 
 #include <cstdio>
@@ -69,22 +71,44 @@ void Colony::runPhase(int phase) {
         std::exit(EXIT_FAILURE);
     }
     statsCached = false;
-    check(aco_phase(ctx, phase, nullptr));  // enqueued; no wait
+    if (inIteration) {  // part of constructSolution: runs with the whole sequence
+        if (planned < kMaxPlan) plan[planned] = phase;
+        ++planned;
+        return;
+    }
+    float ms = 0.0f;
+    check(aco_phase(ctx, phase, &ms));  // a step called on its own runs at once
 }
 
 void Colony::constructSolution(float timers[]) {
     // initTour (placement) and updateAntAges run inside the tour kernel and
     // the probability sweep; their slot reads 0.
+    inIteration = true;
+    planned = 0;
     runPhase(ACO_PHASE_PROBABILITIES);
     runPhase(ACO_PHASE_TOUR);
     runPhase(ACO_PHASE_ANT_DISTANCE);
     // virtual steps of the subclasses (AntSystem / RankBasedAntSystem)
     evaporatePheromones();
     updatePheromones();
+    inIteration = false;
+    static const int standard[5] = {ACO_PHASE_PROBABILITIES, ACO_PHASE_TOUR, ACO_PHASE_ANT_DISTANCE,
+                                    ACO_PHASE_EVAPORATE, ACO_PHASE_UPDATE_PHEROMONE};
+    bool isStandard = planned == 5;
+    for (int i = 0; isStandard && i < 5; ++i) isStandard = plan[i] == standard[i];
     float t[TOTAL_TIMERS] = {0, 0, 0, 0, 0, 0};
     int64_t ib = 0, gb = 0;
     uint32_t nv = 0;
-    check(aco_phase_end(ctx, t, &ib, &gb, &nv));
+    if (isStandard) {
+        check(aco_iterate_stats(ctx, t, &ib, &gb, &nv));
+    } else {
+        if (planned > kMaxPlan) {
+            std::fprintf(stderr, "Colony: %d phases in one iteration (at most %d)\n", planned, kMaxPlan);
+            std::exit(EXIT_FAILURE);
+        }
+        for (int i = 0; i < planned; ++i) check(aco_phase(ctx, plan[i], nullptr));
+        check(aco_phase_end(ctx, t, &ib, &gb, &nv));
+    }
     cachedIterBest = (long)ib;
     cachedGlobBest = (long)gb;
     cachedValidAnts = nv;

@@ -94,8 +94,9 @@ class Colony {
     virtual void evaporatePheromones() = 0;
     virtual void updatePheromones() = 0;
 
-    // Enqueues one phase of the device iteration (no wait); constructSolution
-    // collects the phase times once per iteration.
+    // One phase of the device iteration.  Inside constructSolution it joins
+    // the iteration's sequence, which runs when the sequence is complete;
+    // called on its own it runs at once.
     void runPhase(int phase);
     void check(int rc) const;
 
@@ -107,6 +108,11 @@ class Colony {
     int optimum = -1;
     float initialPheromone = 1e-6f;
     aco_ctx* ctx = nullptr;
+    // phases named by the steps of the current constructSolution
+    static const int kMaxPlan = 8;
+    bool inIteration = false;
+    int plan[kMaxPlan] = {};
+    int planned = 0;
     // statistics of the last constructSolution (valid until the next phase)
     bool statsCached = false;
     long cachedIterBest = 0, cachedGlobBest = 0;

@@ -102,8 +102,16 @@ int aco_create(const aco_params *p, const int32_t *dist, const int32_t *xy, aco_
 int aco_initialize(aco_ctx *c);
 
 /* Colony::constructSolution (src/Colony.cu:537-594): one iteration, phase
- * times in milliseconds (timers may be NULL). */
+ * times in milliseconds (timers may be NULL), each from the start of the
+ * phase's first kernel (after the previous phase) to the stop of its last,
+ * recorded by the kernels' own dispatches; one wait. */
 int aco_iterate(aco_ctx *c, float *timers);
+
+/* aco_iterate plus what Colony::constructSolution's caller reads next:
+ * getIterBestDist / getGlobBestDist / getNumValidAnts after the iteration
+ * (src/Colony.cu:1096-1104, 447-467; each output may be NULL), written by the
+ * statistics kernel to host memory, so the wait is the only host round trip. */
+int aco_iterate_stats(aco_ctx *c, float *timers, int64_t *iter_best, int64_t *glob_best, uint32_t *valid_ants);
 
 /* One phase of Colony::constructSolution (src/Colony.cu:537-594), for hosts
  * that drive the virtual steps themselves.  Order per iteration:

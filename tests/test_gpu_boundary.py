@@ -143,6 +143,29 @@ def test_async_phases_equal_iterate_with_one_wait():
             b.phase_end()
 
 
+def test_iterate_stats_graph_equals_phases():
+    """aco_iterate_stats (one graph launch per iteration, event nodes between
+    the kernels, statistics written to host memory by the statistics kernel)
+    gives the iteration of the phase-by-phase path, with its statistics, over
+    both pheromone buffers; the timers add up to the iteration."""
+    say("case: aco_iterate_stats vs waiting phases, RBAS w=1 aging n=200")
+    xy = instances.uniform(200, 1000, seed=37)
+    p = capi.default_params(n=200, m=64, k=16, w=1, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS, aging=1,
+                            min_age=2, max_age=4, global_perf=2, seed=9)
+    with capi.Colony(p, xy=xy) as a, capi.Colony(p, xy=xy) as b:
+        a.initialize()
+        b.initialize()
+        for it in range(9):
+            t, ib, gb, nv = a.iterate_stats()
+            for ph in (0, 1, 2, 3, 4):
+                b.phase(ph)
+            assert (ib, gb, nv) == a.stats() == b.stats(), it
+            assert t[0] == 0 and t[3] == 0 and t[1] > 0 and t[2] > 0 and t[4] > 0 and t[5] > 0
+            np.testing.assert_array_equal(a.tours(), b.tours())
+            np.testing.assert_array_equal(a.pheromone(), b.pheromone())
+        assert a.aging()["num_valid"] < 64  # retirements happened and were reported
+
+
 def test_run_until_time_limit():
     """The time test of TSP::terminationCondition on the device-resident loop:
     a zero limit runs nothing, a positive limit stops at the first chunk
