@@ -26,7 +26,8 @@ enum : int32_t {
     MLP_B3 = 4800,   // 1
     MLP_PARAMS = 4801,
     MLP_STRIDE = 4802,      // partial-gradient row: parameters + loss
-    MLP_MAX_WAVES = 1024,   // backward waves (partial rows) per launch
+    MLP_MAX_WAVES = 1024,   // backward waves per launch (four per block)
+    MLP_MAX_ROWS = 256,     // backward partial rows per launch (one per block)
 };
 static_assert(MLP_PARAMS == ACO_PRIOR_PARAMS, "parameter count");
 
@@ -52,7 +53,7 @@ hipError_t launch_features(const PriorInstance& in, unsigned long long* sum, flo
 hipError_t launch_labels(const PriorInstance& in, const int32_t* tour, int32_t* pos, float* y, hipStream_t s);
 hipError_t launch_mlp_forward(const MlpParams& p, const float* x, int64_t count, float* out, hipStream_t s);
 // grad (MLP_PARAMS) is overwritten, *loss accumulates; partial holds
-// (MLP_MAX_WAVES + MLP_MAX_WAVES / 16) x MLP_STRIDE floats of scratch.
+// (MLP_MAX_ROWS + MLP_MAX_ROWS / 16) x MLP_STRIDE floats of scratch.
 hipError_t launch_mlp_backward(const MlpParams& p, const float* x, const float* y, int64_t count, float pos_weight,
                                float* partial, float* grad, float* loss, hipStream_t s);
 hipError_t launch_adam(const MlpParams& p, float* grad, float* m, float* v, float lr, float beta1, float beta2,

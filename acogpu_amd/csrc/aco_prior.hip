@@ -38,12 +38,23 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+typedef __bf16 bf16x2h __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8h __attribute__((ext_vector_type(8)));
+
+// Four fp32 values rounded to bf16 (nearest even) by gfx950's
+// v_cvt_pk_bf16_f32, two per instruction.
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    bf16x2h lo, hi;
+    lo[0] = (__bf16)a;
+    lo[1] = (__bf16)b;
+    hi[0] = (__bf16)c;
+    hi[1] = (__bf16)d;
+    const uint32_t l = __builtin_bit_cast(uint32_t, lo), h = __builtin_bit_cast(uint32_t, hi);
     bf16x4 v;
-    v[0] = (short)f2bf(a);
-    v[1] = (short)f2bf(b);
-    v[2] = (short)f2bf(c);
-    v[3] = (short)f2bf(d);
+    v[0] = (short)(l & 0xffffu);
+    v[1] = (short)(l >> 16);
+    v[2] = (short)(h & 0xffffu);
+    v[3] = (short)(h >> 16);
     return v;
 }
 
@@ -51,13 +62,53 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// GELU(x) = x Phi(x) with the exact normal CDF.
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// gfx950's 16x16x32 bf16 product with each lane's eight k values given as
+// two 4-value halves.  k is summed over, so A and B only have to list the
+// same k values in the same order in each lane: the halves are taken from
+// two 16-row blocks of a D-layout tile (rows 4(l/16)+q of each), which is
+// the layout the previous layer's product leaves.
+__device__ __forceinline__ f32x4 mfma32(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 c) {
+    const bf16x8h a = __builtin_bit_cast(bf16x8h, __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7));
+    const bf16x8h b = __builtin_bit_cast(bf16x8h, __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7));
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
-__device__ __forceinline__ float gelu_grad(float x) {
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-    return cdf + x * pdf;
+// GELU(z) = z Phi(z), Phi from the erf approximation 7.1.26 of Abramowitz
+// and Stegun (|error of erf| <= 1.5e-7): with u = |z| / sqrt(2),
+// erf(u) ~ 1 - P(t) e^{-u^2}, t = 1 / (1 + p u), P(t) = a1 t + ... + a5 t^5,
+// so Phi = 1 - Q for z >= 0 and Q for z < 0, Q = P(t) e^{-z^2/2} / 2.  The
+// derivative is that of this function: Phi + z Phi', Phi' = e^{-z^2/2}
+// (c t^2 P'(t) + |z| P(t)) / 2 with c = p / sqrt(2).  One hardware
+// exponential (v_exp_f32) and one reciprocal (v_rcp_f32) per value.
+constexpr float GELU_A1 = 0.254829592f, GELU_A2 = -0.284496736f, GELU_A3 = 1.421413741f;
+constexpr float GELU_A4 = -1.453152027f, GELU_A5 = 1.061405429f;
+constexpr float GELU_C = 0.231641888f;       // 0.3275911 / sqrt(2)
+constexpr float GELU_HLOG2E = 0.72134752f;   // log2(e) / 2
+
+template <bool GRAD>
+__device__ __forceinline__ float gelu_eval(float z, float& dg) {
+    const float a = fabsf(z);
+    const float e = __builtin_amdgcn_exp2f(-GELU_HLOG2E * (z * z));  // e^{-z^2/2}
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(GELU_C, a, 1.0f));
+    const float poly = __builtin_fmaf(
+        t, __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, GELU_A5, GELU_A4), GELU_A3), GELU_A2), GELU_A1);
+    const float P = t * poly;
+    const float q = 0.5f * P * e;
+    const float phi = z >= 0.0f ? 1.0f - q : q;
+    if (GRAD) {
+        const float dP = __builtin_fmaf(
+            t, __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, 5.0f * GELU_A5, 4.0f * GELU_A4), 3.0f * GELU_A3),
+                              2.0f * GELU_A2),
+            GELU_A1);
+        const float dphi = 0.5f * e * __builtin_fmaf(GELU_C * t * t, dP, a * P);
+        dg = __builtin_fmaf(z, dphi, phi);
+    }
+    return z * phi;
+}
+
+__device__ __forceinline__ float gelu(float z) {
+    float unused = 0.0f;
+    return gelu_eval<false>(z, unused);
 }
 
 __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
@@ -154,9 +205,49 @@ __global__ __launch_bounds__(256) void k_labels(const PriorInstance in, const in
 // MLP tiles.  One wave handles 16 edges per tile.  Weights (bf16): W1 is
 // 64 x 16 (inputs 8..15 zero), W2 64 x 64, row = output feature.
 struct MlpRegs {
-    bf16x4 w1a[4];     // layer 1 A operands, output block ob
-    bf16x4 w2a[4][4];  // layer 2 A operands [ob][ib]
+    bf16x4 w1a[4];  // layer 1 A operands, output block ob
 };
+
+// Layer-2 weights as MFMA A operands, staged in LDS once per block (16
+// bytes per lane per fragment, read with one ds_read_b128):
+//   fwd [ob][c][lane]  W2[16 ob + l%16][16 (2c) + 4(l/16) + 0..3] then
+//                      W2[16 ob + l%16][16 (2c+1) + 4(l/16) + 0..3]
+//                      (z2 = W2 h1, K pairs of h1's 16-row blocks)
+//   bwd [ib][c][lane]  W2[16 (2c) + 4(l/16) + 0..3][16 ib + l%16] then
+//                      W2[16 (2c+1) + 4(l/16) + 0..3][16 ib + l%16]
+//                      (dh1 = W2^T dz2, K pairs of dz2's blocks)
+struct W2Frags {
+    uint4 fwd[4 * 2 * 64];
+    uint4 bwd[4 * 2 * 64];
+};
+
+__device__ __forceinline__ uint32_t bf_pair(const uint16_t* w, int i0, int i1) {
+    return (uint32_t)w[i0] | ((uint32_t)w[i1] << 16);
+}
+
+__device__ __forceinline__ void stage_w2(const MlpParams& p, W2Frags* F, bool backward) {
+    const uint16_t* w = p.w2_bf16;  // 64 x 64, row = output feature
+    for (int f = threadIdx.x; f < 4 * 2 * 64; f += blockDim.x) {
+        const int blk = f >> 7, c = (f >> 6) & 1, l = f & 63, r = l & 15, g = l >> 4;
+        const int k0 = 16 * (2 * c) + 4 * g, k1 = 16 * (2 * c + 1) + 4 * g;
+        const int o = 16 * blk + r;
+        F->fwd[f] = make_uint4(bf_pair(w, o * 64 + k0, o * 64 + k0 + 1), bf_pair(w, o * 64 + k0 + 2, o * 64 + k0 + 3),
+                               bf_pair(w, o * 64 + k1, o * 64 + k1 + 1), bf_pair(w, o * 64 + k1 + 2, o * 64 + k1 + 3));
+        if (backward) {
+            const int i = 16 * blk + r;
+            F->bwd[f] = make_uint4(bf_pair(w, k0 * 64 + i, (k0 + 1) * 64 + i),
+                                   bf_pair(w, (k0 + 2) * 64 + i, (k0 + 3) * 64 + i),
+                                   bf_pair(w, k1 * 64 + i, (k1 + 1) * 64 + i),
+                                   bf_pair(w, (k1 + 2) * 64 + i, (k1 + 3) * 64 + i));
+        }
+    }
+}
+
+// The two 4-value halves of a staged fragment.
+__device__ __forceinline__ void frag_halves(const uint4& v, bf16x4& a0, bf16x4& a1) {
+    a0 = bf16x4{(short)(v.x & 0xffffu), (short)(v.x >> 16), (short)(v.y & 0xffffu), (short)(v.y >> 16)};
+    a1 = bf16x4{(short)(v.z & 0xffffu), (short)(v.z >> 16), (short)(v.w & 0xffffu), (short)(v.w >> 16)};
+}
 
 // b1, b2, w3 (64 each) and b3, staged in LDS by every MLP kernel.
 enum : int { VEC_B1 = 0, VEC_B2 = 64, VEC_W3 = 128, VEC_B3 = 192, VEC_LEN = 196 };
@@ -187,10 +278,6 @@ __device__ __forceinline__ void load_mlp(const MlpParams& p, MlpRegs& R, int lan
     for (int ob = 0; ob < 4; ++ob) {
         const uint16_t* w1 = p.w1_bf16 + (size_t)(16 * ob + row) * 16 + 4 * grp;
         R.w1a[ob] = bf16x4{(short)w1[0], (short)w1[1], (short)w1[2], (short)w1[3]};
-        for (int ib = 0; ib < 4; ++ib) {
-            const uint16_t* w2 = p.w2_bf16 + (size_t)(16 * ob + row) * 64 + 16 * ib + 4 * grp;
-            R.w2a[ob][ib] = bf16x4{(short)w2[0], (short)w2[1], (short)w2[2], (short)w2[3]};
-        }
     }
 }
 
@@ -203,15 +290,29 @@ __device__ __forceinline__ bf16x4 load_x(const float* __restrict__ x, int64_t e0
     return pack4(v.x, v.y, v.z, v.w);
 }
 
-// Forward of one tile; keeps pre-activations for the backward pass.
+// Forward of one tile; with GRAD it keeps each unit's GELU derivative for
+// the backward pass.
 struct Activations {
-    f32x4 z1[4], z2[4];
+    f32x4 d1[4], d2[4];  // GELU'(z1), GELU'(z2) (GRAD only)
     bf16x4 h1[4], h2[4];
     float out;  // logit of edge l % 16 (valid in every lane group)
 };
 
-__device__ __forceinline__ void forward_tile(const MlpRegs& R, const float* vec, bf16x4 xb, Activations& A,
-                                             int grp) {
+template <bool GRAD>
+__device__ __forceinline__ bf16x4 gelu4(const f32x4& z, f32x4& d) {
+    float g[4];
+    for (int q = 0; q < 4; ++q) {
+        float dq = 0.0f;
+        g[q] = gelu_eval<GRAD>(z[q], dq);
+        d[q] = dq;
+    }
+    return pack4(g[0], g[1], g[2], g[3]);
+}
+
+template <bool GRAD = false>
+__device__ __forceinline__ void forward_tile(const MlpRegs& R, const W2Frags* F, const float* vec, bf16x4 xb,
+                                             Activations& A, int lane) {
+    const int grp = lane >> 4;
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
     for (int ob = 0; ob < 4; ++ob) {
         f32x4 z = mfma16(R.w1a[ob], xb, zero);
@@ -220,20 +321,22 @@ __device__ __forceinline__ void forward_tile(const MlpRegs& R, const float* vec,
         z[1] = z[1] + b.y;
         z[2] = z[2] + b.z;
         z[3] = z[3] + b.w;
-        A.z1[ob] = z;
-        A.h1[ob] = pack4(gelu(z[0]), gelu(z[1]), gelu(z[2]), gelu(z[3]));
+        A.h1[ob] = gelu4<GRAD>(z, A.d1[ob]);
     }
     float partial = 0.0f;
     for (int ob = 0; ob < 4; ++ob) {
         f32x4 z = zero;
-        for (int ib = 0; ib < 4; ++ib) z = mfma16(R.w2a[ob][ib], A.h1[ib], z);
+        for (int c = 0; c < 2; ++c) {
+            bf16x4 a0, a1;
+            frag_halves(F->fwd[(ob * 2 + c) * 64 + lane], a0, a1);
+            z = mfma32(a0, a1, A.h1[2 * c], A.h1[2 * c + 1], z);
+        }
         const float4 b = vec4(vec, VEC_B2, ob, grp);
         z[0] = z[0] + b.x;
         z[1] = z[1] + b.y;
         z[2] = z[2] + b.z;
         z[3] = z[3] + b.w;
-        A.z2[ob] = z;
-        A.h2[ob] = pack4(gelu(z[0]), gelu(z[1]), gelu(z[2]), gelu(z[3]));
+        A.h2[ob] = gelu4<GRAD>(z, A.d2[ob]);
         const float4 w = vec4(vec, VEC_W3, ob, grp);
         partial = partial + w.x * bf2f((uint16_t)A.h2[ob][0]);
         partial = partial + w.y * bf2f((uint16_t)A.h2[ob][1]);
@@ -249,7 +352,10 @@ __device__ __forceinline__ void forward_tile(const MlpRegs& R, const float* vec,
 __global__ __launch_bounds__(256) void k_mlp_forward(const MlpParams p, const float* __restrict__ x, int64_t count,
                                                      float* __restrict__ out) {
     __shared__ __align__(16) float vec[VEC_LEN];
+    __shared__ __align__(16) uint4 w2f[4 * 2 * 64];
+    W2Frags* F = reinterpret_cast<W2Frags*>(w2f);  // forward fragments only
     const int lane = threadIdx.x & 63;
+    stage_w2(p, F, false);
     stage_vectors(p, vec);
     MlpRegs R;
     load_mlp(p, R, lane);
@@ -257,44 +363,45 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const MlpParams p, const fl
     for (int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t * 16 < count; t += waves) {
         const int64_t e0 = t * 16;
         Activations A;
-        forward_tile(R, vec, load_x(x, e0, count, lane), A, lane >> 4);
+        forward_tile(R, F, vec, load_x(x, e0, count, lane), A, lane);
         if (lane < 16 && e0 + lane < count) out[e0 + lane] = A.out;
     }
 }
 
-// Transpose a D-layout 16x16 bf16 tile (feature rows 4(l/16)+q, edge col l%16)
-// into the A/B layout with edges along K: lane l gets rows 4(l/16)+q of
-// column l%16 of the transposed tile, i.e. edge 4(l/16)+q, feature l%16.
-__device__ __forceinline__ bf16x4 transpose16(bf16x4 v, uint16_t* scratch, int lane) {
+// Transposes of D-layout 16x16 bf16 tiles (feature rows 4(l/16)+q, edge
+// column l%16) into the A/B layout with edges along K: lane l gets edge
+// 4(l/16)+q of feature l%16.  All T tiles go through the wave's LDS scratch
+// (T x 256 entries) with one ordering point between the stores and loads.
+template <int T>
+__device__ __forceinline__ void transpose16_batch(const bf16x4 (&v)[T], bf16x4 (&r)[T], uint16_t* scratch,
+                                                  int lane) {
     const int col = lane & 15, grp = lane >> 4;
-    for (int q = 0; q < 4; ++q) scratch[(4 * grp + q) * 16 + col] = (uint16_t)v[q];  // [feature][edge]
+    for (int t = 0; t < T; ++t)
+        for (int q = 0; q < 4; ++q) scratch[t * 256 + (4 * grp + q) * 16 + col] = (uint16_t)v[t][q];
     wave_lds_sync();
-    bf16x4 r;
-    for (int q = 0; q < 4; ++q) r[q] = (short)scratch[col * 16 + 4 * grp + q];  // feature col, edge 4grp+q
+    for (int t = 0; t < T; ++t) {
+        const uint2 w = *reinterpret_cast<const uint2*>(scratch + t * 256 + col * 16 + 4 * grp);
+        r[t] = bf16x4{(short)(w.x & 0xffffu), (short)(w.x >> 16), (short)(w.y & 0xffffu), (short)(w.y >> 16)};
+    }
     wave_lds_sync();
-    return r;
 }
 
 // Backward: binary cross-entropy with logits, positive weight `pos_weight`,
 // mean over `count` edges.  Every wave accumulates its tiles' gradients in
-// fp32 registers and stores one partial row.
+// fp32 registers; the block's waves then add their rows in LDS in wave order
+// and the block stores one partial row (parameters, then the loss).
 __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const float* __restrict__ x,
                                                       const float* __restrict__ y, int64_t count, float pos_weight,
                                                       float* __restrict__ partial) {
-    __shared__ uint16_t tr[4][256];
+    __shared__ __align__(16) uint16_t tr[4][8 * 256];
     __shared__ __align__(16) float vec[VEC_LEN];
+    __shared__ __align__(16) W2Frags F;
+    __shared__ __align__(16) float acc[MLP_STRIDE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane >> 4, col = lane & 15;
+    stage_w2(p, &F, true);
     stage_vectors(p, vec);
     MlpRegs R;
     load_mlp(p, R, lane);
-    // layer-2 transposed weights for dh1 = W2^T dz2: A[i][o] = W2[o][i]
-    bf16x4 w2t[4][4];
-    for (int ib = 0; ib < 4; ++ib)
-        for (int ob = 0; ob < 4; ++ob) {
-            bf16x4 v;
-            for (int q = 0; q < 4; ++q) v[q] = (short)p.w2_bf16[(size_t)(16 * ob + 4 * grp + q) * 64 + 16 * ib + col];
-            w2t[ib][ob] = v;
-        }
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
     f32x4 gW2[4][4], gW1[4];
     for (int a = 0; a < 4; ++a) {
@@ -305,14 +412,15 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
     for (int a = 0; a < 4; ++a)
         for (int q = 0; q < 4; ++q) gb1[a][q] = gb2[a][q] = gw3[a][q] = 0.0f;
     const float inv = 1.0f / (float)count;
+    uint16_t* sc = tr[wave];
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave; t * 16 < count; t += waves) {
         const int64_t e0 = t * 16;
         const bool valid = e0 + col < count;
         const bf16x4 xb = load_x(x, e0, count, lane);
-        Activations A;
-        forward_tile(R, vec, xb, A, grp);
         const float yy = valid ? y[e0 + col] : 0.0f;
+        Activations A;
+        forward_tile<true>(R, &F, vec, xb, A, lane);
         const float s = sigmoidf(A.out);
         // d loss / d logit for -w y log s - (1-y) log(1-s)
         const float g = valid ? (s * (1.0f - yy + pos_weight * yy) - pos_weight * yy) * inv : 0.0f;
@@ -330,74 +438,80 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
             const float w3[4] = {w.x, w.y, w.z, w.w};
             for (int q = 0; q < 4; ++q) {
                 gw3[ob][q] += g * bf2f((uint16_t)A.h2[ob][q]);
-                dz[q] = g * w3[q] * gelu_grad(A.z2[ob][q]);
+                dz[q] = g * w3[q] * A.d2[ob][q];
                 gb2[ob][q] += dz[q];
             }
             dz2b[ob] = pack4(dz[0], dz[1], dz[2], dz[3]);
         }
-        // dW2[o][i] = sum_e dz2[o][e] h1[i][e]: A = dz2 (o x e), B = h1^T (e x i)
-        uint16_t* sc = tr[wave];
-        bf16x4 dz2t[4], h1t[4];
-        for (int a = 0; a < 4; ++a) {
-            dz2t[a] = transpose16(dz2b[a], sc, lane);
-            h1t[a] = transpose16(A.h1[a], sc, lane);
-        }
-        // A operand needs A[o][e] with lane (o = l%16, e = 4grp+q): dz2t holds
-        // feature l%16 of edges 4grp+q -- exactly that; B needs B[e][i] with
-        // lane (i = l%16, e = 4grp+q): h1t holds feature l%16 of edges 4grp+q.
+        // dW2[o][i] = sum_e dz2[o][e] h1[i][e]: A = dz2 (o x e), B = h1^T
+        // (e x i); both transposed so that edges run along K: lane (o or i =
+        // l%16, e = 4grp+q)
+        const bf16x4 tin[8] = {dz2b[0], dz2b[1], dz2b[2], dz2b[3], A.h1[0], A.h1[1], A.h1[2], A.h1[3]};
+        bf16x4 tout[8];
+        transpose16_batch<8>(tin, tout, sc, lane);
         for (int ob = 0; ob < 4; ++ob)
-            for (int ib = 0; ib < 4; ++ib) gW2[ob][ib] = mfma16(dz2t[ob], h1t[ib], gW2[ob][ib]);
-        // dh1 = W2^T dz2 (D layout), dz1 = dh1 * gelu'(z1)
-        bf16x4 dz1t[4];
+            for (int ib = 0; ib < 4; ++ib) gW2[ob][ib] = mfma16(tout[ob], tout[4 + ib], gW2[ob][ib]);
+        // dh1 = W2^T dz2 (D layout; K = the 64 outputs, in pairs of dz2's
+        // blocks), dz1 = dh1 * gelu'(z1)
+        bf16x4 t2in[5], t2out[5];
         for (int ib = 0; ib < 4; ++ib) {
             f32x4 dh = zero;
-            for (int ob = 0; ob < 4; ++ob) dh = mfma16(w2t[ib][ob], dz2b[ob], dh);
+            for (int c = 0; c < 2; ++c) {
+                bf16x4 a0, a1;
+                frag_halves(F.bwd[(ib * 2 + c) * 64 + lane], a0, a1);
+                dh = mfma32(a0, a1, dz2b[2 * c], dz2b[2 * c + 1], dh);
+            }
             float dz[4];
             for (int q = 0; q < 4; ++q) {
-                dz[q] = dh[q] * gelu_grad(A.z1[ib][q]);
+                dz[q] = dh[q] * A.d1[ib][q];
                 gb1[ib][q] += dz[q];
             }
-            dz1t[ib] = transpose16(pack4(dz[0], dz[1], dz[2], dz[3]), sc, lane);
+            t2in[ib] = pack4(dz[0], dz[1], dz[2], dz[3]);
         }
         // dW1[o][i] = sum_e dz1[o][e] x[i][e]: B = x^T (e x i), i in 0..15
-        const bf16x4 xt = transpose16(xb, sc, lane);
-        for (int ob = 0; ob < 4; ++ob) gW1[ob] = mfma16(dz1t[ob], xt, gW1[ob]);
-    }
-    // This wave's partial gradient row (params, then the loss); the rows are
-    // summed in a fixed order by k_reduce_grad1/2, so gradients are deterministic.
-    float* row = partial + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * MLP_STRIDE;
-    // D layout of a gradient tile: row 4grp+q, column col
-    for (int ob = 0; ob < 4; ++ob) {
-        for (int q = 0; q < 4; ++q) {
-            const int o = 16 * ob + 4 * grp + q;
-            if (col < 8) row[MLP_W1 + o * 8 + col] = gW1[ob][q];
-            for (int ib = 0; ib < 4; ++ib) row[MLP_W2 + o * 64 + 16 * ib + col] = gW2[ob][ib][q];
-        }
+        t2in[4] = xb;
+        transpose16_batch<5>(t2in, t2out, sc, lane);
+        for (int ob = 0; ob < 4; ++ob) gW1[ob] = mfma16(t2out[ob], t2out[4], gW1[ob]);
     }
     // bias / w3 gradients: reduce over the 16 edge lanes of each group
     for (int ob = 0; ob < 4; ++ob)
-        for (int q = 0; q < 4; ++q) {
-            float v1 = gb1[ob][q], v2 = gb2[ob][q], v3 = gw3[ob][q];
+        for (int q = 0; q < 4; ++q)
             for (int off = 8; off >= 1; off >>= 1) {
-                v1 += __shfl_xor(v1, off);
-                v2 += __shfl_xor(v2, off);
-                v3 += __shfl_xor(v3, off);
+                gb1[ob][q] += __shfl_xor(gb1[ob][q], off);
+                gb2[ob][q] += __shfl_xor(gb2[ob][q], off);
+                gw3[ob][q] += __shfl_xor(gw3[ob][q], off);
             }
-            if (col == 0) {
-                const int o = 16 * ob + 4 * grp + q;
-                row[MLP_B1 + o] = v1;
-                row[MLP_B2 + o] = v2;
-                row[MLP_W3 + o] = v3;
-            }
-        }
     for (int off = 32; off >= 1; off >>= 1) {
         gb3 += __shfl_xor(gb3, off);
         lsum += __shfl_xor(lsum, off);
     }
-    if (lane == 0) {
-        row[MLP_B3] = gb3;
-        row[MLP_PARAMS] = lsum;
+    // the block's row: waves add theirs in LDS in wave order (every entry of
+    // a row has exactly one writing lane), then one coalesced store
+    for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) {
+        if (wave == wv) {
+            const bool first = wv == 0;
+            auto put = [&](int idx, float v) { acc[idx] = first ? v : acc[idx] + v; };
+            // D layout of a gradient tile: row 4grp+q, column col
+            for (int ob = 0; ob < 4; ++ob)
+                for (int q = 0; q < 4; ++q) {
+                    const int o = 16 * ob + 4 * grp + q;
+                    if (col < 8) put(MLP_W1 + o * 8 + col, gW1[ob][q]);
+                    for (int ib = 0; ib < 4; ++ib) put(MLP_W2 + o * 64 + 16 * ib + col, gW2[ob][ib][q]);
+                    if (col == 0) {
+                        put(MLP_B1 + o, gb1[ob][q]);
+                        put(MLP_B2 + o, gb2[ob][q]);
+                        put(MLP_W3 + o, gw3[ob][q]);
+                    }
+                }
+            if (lane == 0) {
+                put(MLP_B3, gb3);
+                put(MLP_PARAMS, lsum);
+            }
+        }
+        __syncthreads();
     }
+    float* row = partial + (size_t)blockIdx.x * MLP_STRIDE;
+    for (int i = threadIdx.x; i < MLP_STRIDE; i += blockDim.x) row[i] = acc[i];
 }
 
 // Sum of the per-wave partial rows in a fixed order, in two stages.  Stage 1:
@@ -504,11 +618,11 @@ static unsigned backward_blocks(int64_t count) {
 
 hipError_t launch_mlp_backward(const MlpParams& p, const float* x, const float* y, int64_t count, float pos_weight,
                                float* partial, float* grad, float* loss, hipStream_t s) {
-    const unsigned blocks = backward_blocks(count);
+    const unsigned blocks = backward_blocks(count);  // one partial row per block
     hipLaunchKernelGGL(k_mlp_backward, dim3(blocks), dim3(256), 0, s, p, x, y, count, pos_weight, partial);
-    const int32_t rows = (int32_t)(blocks * 4);
+    const int32_t rows = (int32_t)blocks;
     const int32_t chunks = (rows + 63) / 64;
-    float* stage = partial + (size_t)MLP_MAX_WAVES * MLP_STRIDE;
+    float* stage = partial + (size_t)MLP_MAX_ROWS * MLP_STRIDE;
     hipLaunchKernelGGL(k_reduce_grad1, dim3(cdiv(MLP_STRIDE, 64), chunks), dim3(256), 0, s, partial, rows, stage);
     hipLaunchKernelGGL(k_reduce_grad2, dim3(cdiv(MLP_STRIDE, 256)), dim3(256), 0, s, stage, chunks * 4, grad, loss);
     return hipGetLastError();
@@ -693,7 +807,7 @@ extern "C" int aco_prior_create(const aco_prior_config* cfg, aco_prior** out) {
         (rc = palloc(pr, &pr->d_w2, 64 * 64)) || (rc = palloc(pr, &pr->d_m, ACO_PRIOR_PARAMS)) ||
         (rc = palloc(pr, &pr->d_v, ACO_PRIOR_PARAMS)) || (rc = palloc(pr, &pr->d_grad, ACO_PRIOR_PARAMS)) ||
         (rc = palloc(pr, &pr->d_loss, 1)) || (rc = palloc(pr, &pr->d_sum, 1)) ||
-        (rc = palloc(pr, &pr->d_partial, (size_t)(aco::MLP_MAX_WAVES + aco::MLP_MAX_WAVES / 16) * aco::MLP_STRIDE))) {
+        (rc = palloc(pr, &pr->d_partial, (size_t)(aco::MLP_MAX_ROWS + aco::MLP_MAX_ROWS / 16) * aco::MLP_STRIDE))) {
         snprintf(g_prior_err, sizeof(g_prior_err), "%s", pr->err);
         aco_prior_destroy(pr);
         return rc;
