@@ -39,23 +39,14 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 typedef __bf16 bf16x2h __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x4h __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8h __attribute__((ext_vector_type(8)));
 
 // Four fp32 values rounded to bf16 (nearest even) by gfx950's
 // v_cvt_pk_bf16_f32, two per instruction.
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
-    bf16x2h lo, hi;
-    lo[0] = (__bf16)a;
-    lo[1] = (__bf16)b;
-    hi[0] = (__bf16)c;
-    hi[1] = (__bf16)d;
-    const uint32_t l = __builtin_bit_cast(uint32_t, lo), h = __builtin_bit_cast(uint32_t, hi);
-    bf16x4 v;
-    v[0] = (short)(l & 0xffffu);
-    v[1] = (short)(l >> 16);
-    v[2] = (short)(h & 0xffffu);
-    v[3] = (short)(h >> 16);
-    return v;
+    const f32x4 v = {a, b, c, d};
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16x4h));
 }
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -76,33 +67,27 @@ __device__ __forceinline__ f32x4 mfma32(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 
 // GELU(z) = z Phi(z), Phi from the erf approximation 7.1.26 of Abramowitz
 // and Stegun (|error of erf| <= 1.5e-7): with u = |z| / sqrt(2),
 // erf(u) ~ 1 - P(t) e^{-u^2}, t = 1 / (1 + p u), P(t) = a1 t + ... + a5 t^5,
-// so Phi = 1 - Q for z >= 0 and Q for z < 0, Q = P(t) e^{-z^2/2} / 2.  The
-// derivative is that of this function: Phi + z Phi', Phi' = e^{-z^2/2}
-// (c t^2 P'(t) + |z| P(t)) / 2 with c = p / sqrt(2).  One hardware
-// exponential (v_exp_f32) and one reciprocal (v_rcp_f32) per value.
-constexpr float GELU_A1 = 0.254829592f, GELU_A2 = -0.284496736f, GELU_A3 = 1.421413741f;
-constexpr float GELU_A4 = -1.453152027f, GELU_A5 = 1.061405429f;
+// so Phi = 1 - Q for z >= 0 and Q for z < 0, Q = P(t) e^{-z^2/2} / 2 (the
+// coefficients below are the a_i / 2).  The derivative is GELU'(z) =
+// Phi(z) + z phi(z) with the exact normal density phi(z) = e^{-z^2/2} /
+// sqrt(2 pi), which reuses the exponential: only Phi is approximated, so it
+// is within 7.5e-8 of the exact derivative.  One hardware exponential
+// (v_exp_f32) and one reciprocal (v_rcp_f32) per value.
+constexpr float GELU_H1 = 0.127414796f, GELU_H2 = -0.142248368f, GELU_H3 = 0.7107068705f;
+constexpr float GELU_H4 = -0.7265760135f, GELU_H5 = 0.5307027145f;
 constexpr float GELU_C = 0.231641888f;       // 0.3275911 / sqrt(2)
 constexpr float GELU_HLOG2E = 0.72134752f;   // log2(e) / 2
+constexpr float GELU_RSQRT2PI = 0.398942280f;  // 1 / sqrt(2 pi)
 
 template <bool GRAD>
 __device__ __forceinline__ float gelu_eval(float z, float& dg) {
-    const float a = fabsf(z);
-    const float e = __builtin_amdgcn_exp2f(-GELU_HLOG2E * (z * z));  // e^{-z^2/2}
-    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(GELU_C, a, 1.0f));
+    const float e = __builtin_amdgcn_exp2f((z * z) * -GELU_HLOG2E);  // e^{-z^2/2}
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(GELU_C, fabsf(z), 1.0f));
     const float poly = __builtin_fmaf(
-        t, __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, GELU_A5, GELU_A4), GELU_A3), GELU_A2), GELU_A1);
-    const float P = t * poly;
-    const float q = 0.5f * P * e;
+        t, __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, GELU_H5, GELU_H4), GELU_H3), GELU_H2), GELU_H1);
+    const float q = (t * poly) * e;
     const float phi = z >= 0.0f ? 1.0f - q : q;
-    if (GRAD) {
-        const float dP = __builtin_fmaf(
-            t, __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, 5.0f * GELU_A5, 4.0f * GELU_A4), 3.0f * GELU_A3),
-                              2.0f * GELU_A2),
-            GELU_A1);
-        const float dphi = 0.5f * e * __builtin_fmaf(GELU_C * t * t, dP, a * P);
-        dg = __builtin_fmaf(z, dphi, phi);
-    }
+    if (GRAD) dg = __builtin_fmaf(z * e, GELU_RSQRT2PI, phi);
     return z * phi;
 }
 
@@ -112,6 +97,9 @@ __device__ __forceinline__ float gelu(float z) {
 }
 
 __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+constexpr float LOG2E_F = 1.44269504f;  // log2(e)
+constexpr float LN2_F = 0.693147181f;   // ln(2)
 
 // ---------------------------------------------------------------------------
 // Features of candidate edge (i, r), j = nn[i][r]; D_i^q = distance from i to
@@ -208,8 +196,9 @@ struct MlpRegs {
     bf16x4 w1a[4];  // layer 1 A operands, output block ob
 };
 
-// Layer-2 weights as MFMA A operands, staged in LDS once per block (16
-// bytes per lane per fragment, read with one ds_read_b128):
+// Layer-2 weights as MFMA A operands (16 bytes per lane per fragment, read
+// from LDS with one ds_read_b128), written by k_pack_bf16 after every update
+// of the parameters and copied to LDS by every block that uses them:
 //   fwd [ob][c][lane]  W2[16 ob + l%16][16 (2c) + 4(l/16) + 0..3] then
 //                      W2[16 ob + l%16][16 (2c+1) + 4(l/16) + 0..3]
 //                      (z2 = W2 h1, K pairs of h1's 16-row blocks)
@@ -221,26 +210,10 @@ struct W2Frags {
     uint4 bwd[4 * 2 * 64];
 };
 
-__device__ __forceinline__ uint32_t bf_pair(const uint16_t* w, int i0, int i1) {
-    return (uint32_t)w[i0] | ((uint32_t)w[i1] << 16);
-}
-
 __device__ __forceinline__ void stage_w2(const MlpParams& p, W2Frags* F, bool backward) {
-    const uint16_t* w = p.w2_bf16;  // 64 x 64, row = output feature
-    for (int f = threadIdx.x; f < 4 * 2 * 64; f += blockDim.x) {
-        const int blk = f >> 7, c = (f >> 6) & 1, l = f & 63, r = l & 15, g = l >> 4;
-        const int k0 = 16 * (2 * c) + 4 * g, k1 = 16 * (2 * c + 1) + 4 * g;
-        const int o = 16 * blk + r;
-        F->fwd[f] = make_uint4(bf_pair(w, o * 64 + k0, o * 64 + k0 + 1), bf_pair(w, o * 64 + k0 + 2, o * 64 + k0 + 3),
-                               bf_pair(w, o * 64 + k1, o * 64 + k1 + 1), bf_pair(w, o * 64 + k1 + 2, o * 64 + k1 + 3));
-        if (backward) {
-            const int i = 16 * blk + r;
-            F->bwd[f] = make_uint4(bf_pair(w, k0 * 64 + i, (k0 + 1) * 64 + i),
-                                   bf_pair(w, (k0 + 2) * 64 + i, (k0 + 3) * 64 + i),
-                                   bf_pair(w, k1 * 64 + i, (k1 + 1) * 64 + i),
-                                   bf_pair(w, (k1 + 2) * 64 + i, (k1 + 3) * 64 + i));
-        }
-    }
+    uint4* dst = F->fwd;  // bwd follows fwd
+    const int n = backward ? MLP_W2_FRAGS : MLP_W2_FRAGS / 2;
+    for (int f = threadIdx.x; f < n; f += blockDim.x) dst[f] = p.w2_frag[f];
 }
 
 // The two 4-value halves of a staged fragment.
@@ -274,11 +247,7 @@ __device__ __forceinline__ float4 vec4(const float* vec, int base, int ob, int g
 }
 
 __device__ __forceinline__ void load_mlp(const MlpParams& p, MlpRegs& R, int lane) {
-    const int row = lane & 15, grp = lane >> 4;
-    for (int ob = 0; ob < 4; ++ob) {
-        const uint16_t* w1 = p.w1_bf16 + (size_t)(16 * ob + row) * 16 + 4 * grp;
-        R.w1a[ob] = bf16x4{(short)w1[0], (short)w1[1], (short)w1[2], (short)w1[3]};
-    }
+    for (int ob = 0; ob < 4; ++ob) R.w1a[ob] = __builtin_bit_cast(bf16x4, p.w1_frag[ob * 64 + lane]);
 }
 
 // Input tile as the layer-1 B operand: features 4 (l/16) + q of edge l % 16.
@@ -313,35 +282,27 @@ template <bool GRAD = false>
 __device__ __forceinline__ void forward_tile(const MlpRegs& R, const W2Frags* F, const float* vec, bf16x4 xb,
                                              Activations& A, int lane) {
     const int grp = lane >> 4;
-    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    // the biases are the products' initial accumulator values
     for (int ob = 0; ob < 4; ++ob) {
-        f32x4 z = mfma16(R.w1a[ob], xb, zero);
         const float4 b = vec4(vec, VEC_B1, ob, grp);
-        z[0] = z[0] + b.x;
-        z[1] = z[1] + b.y;
-        z[2] = z[2] + b.z;
-        z[3] = z[3] + b.w;
+        const f32x4 z = mfma16(R.w1a[ob], xb, f32x4{b.x, b.y, b.z, b.w});
         A.h1[ob] = gelu4<GRAD>(z, A.d1[ob]);
     }
     float partial = 0.0f;
     for (int ob = 0; ob < 4; ++ob) {
-        f32x4 z = zero;
+        const float4 b = vec4(vec, VEC_B2, ob, grp);
+        f32x4 z = {b.x, b.y, b.z, b.w};
         for (int c = 0; c < 2; ++c) {
             bf16x4 a0, a1;
             frag_halves(F->fwd[(ob * 2 + c) * 64 + lane], a0, a1);
             z = mfma32(a0, a1, A.h1[2 * c], A.h1[2 * c + 1], z);
         }
-        const float4 b = vec4(vec, VEC_B2, ob, grp);
-        z[0] = z[0] + b.x;
-        z[1] = z[1] + b.y;
-        z[2] = z[2] + b.z;
-        z[3] = z[3] + b.w;
         A.h2[ob] = gelu4<GRAD>(z, A.d2[ob]);
         const float4 w = vec4(vec, VEC_W3, ob, grp);
-        partial = partial + w.x * bf2f((uint16_t)A.h2[ob][0]);
-        partial = partial + w.y * bf2f((uint16_t)A.h2[ob][1]);
-        partial = partial + w.z * bf2f((uint16_t)A.h2[ob][2]);
-        partial = partial + w.w * bf2f((uint16_t)A.h2[ob][3]);
+        partial = __builtin_fmaf(w.x, bf2f((uint16_t)A.h2[ob][0]), partial);
+        partial = __builtin_fmaf(w.y, bf2f((uint16_t)A.h2[ob][1]), partial);
+        partial = __builtin_fmaf(w.z, bf2f((uint16_t)A.h2[ob][2]), partial);
+        partial = __builtin_fmaf(w.w, bf2f((uint16_t)A.h2[ob][3]), partial);
     }
     // the four lane groups hold disjoint features of the same edge
     partial = partial + __shfl_xor(partial, 16);
@@ -386,19 +347,71 @@ __device__ __forceinline__ void transpose16_batch(const bf16x4 (&v)[T], bf16x4 (
     wave_lds_sync();
 }
 
+// Sum over the 16 lanes of each row of the wave (the 16 edges of a tile):
+// row_ror 8, 4, 2, 1, after which every lane of the row holds the total
+// (each lane in its own fixed order; callers use lane 0 of the row).
+__device__ __forceinline__ float row16_sum(float v) {
+    v = v + dpp_src<0x128, 0xf>(v);  // row_ror:8
+    v = v + dpp_src<0x124, 0xf>(v);  // row_ror:4
+    v = v + dpp_src<0x122, 0xf>(v);  // row_ror:2
+    v = v + dpp_src<0x121, 0xf>(v);  // row_ror:1
+    return v;
+}
+
+// Input tile of the backward pass as loaded from global memory (features
+// 4 (l/16) + 0..3 of edge l % 16 in lanes of groups 0 and 1; its label in
+// every lane), converted after the load has landed.
+struct TileIn {
+    float4 x;
+    float y;
+};
+
+__device__ __forceinline__ TileIn load_tile(const float* __restrict__ x, const float* __restrict__ y, int64_t e0,
+                                            int64_t count, int lane) {
+    const int64_t e = e0 + (lane & 15);
+    TileIn r{make_float4(0.f, 0.f, 0.f, 0.f), 0.0f};
+    if (e < count) {
+        if ((lane >> 4) < 2) r.x = *reinterpret_cast<const float4*>(x + (size_t)e * 8 + 4 * (lane >> 4));
+        r.y = y[e];
+    }
+    return r;
+}
+
 // Backward: binary cross-entropy with logits, positive weight `pos_weight`,
-// mean over `count` edges.  Every wave accumulates its tiles' gradients in
-// fp32 registers; the block's waves then add their rows in LDS in wave order
-// and the block stores one partial row (parameters, then the loss).
-__global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const float* __restrict__ x,
+// mean over `count` edges.  Every wave accumulates its tiles' gradients --
+// W1 and W2 (and b1, through a constant-one input column of the W1 product)
+// in MFMA accumulators, b2 and w3 in LDS slots owned by one lane each (read,
+// added to and written back once per tile) --
+// while the next tile's inputs load.  After the loop the block's waves
+// (MLP_BWD_WAVES, two per SIMD: at most 256 registers each) combine their
+// gradients in LDS -- the upper half of the waves store rows in accumulator
+// order with 16-byte stores, the lower half add theirs to them (wave w to
+// row w) -- and the block writes the sum of those rows, taken in row order,
+// as one partial row (layout MLP_ROW_*).
+__global__ __launch_bounds__(64 * MLP_BWD_WAVES) void k_mlp_backward(const MlpParams p, const float* __restrict__ x,
                                                       const float* __restrict__ y, int64_t count, float pos_weight,
                                                       float* __restrict__ partial) {
-    __shared__ __align__(16) uint16_t tr[4][8 * 256];
+    // The loop's LDS (W2 fragments, per-wave transpose scratch and b2 / w3
+    // sums) is reused for the gradient rows afterwards.
+    struct LoopShared {
+        W2Frags F;
+        uint16_t tr[MLP_BWD_WAVES][8 * 256];
+        float4 bias[MLP_BWD_WAVES][4][2][64];  // wave, ob, (b2, w3), lane: q = 0..3
+    };
+    union BlockShared {
+        LoopShared loop;
+        float4 rows[MLP_BWD_WAVES / 2][MLP_ROW / 4];
+    };
+    __shared__ __align__(16) BlockShared sh;
     __shared__ __align__(16) float vec[VEC_LEN];
-    __shared__ __align__(16) W2Frags F;
-    __shared__ __align__(16) float acc[MLP_STRIDE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane >> 4, col = lane & 15;
-    stage_w2(p, &F, true);
+    W2Frags* F = &sh.loop.F;
+    float4(*bias)[2][64] = sh.loop.bias[wave];
+    for (int ob = 0; ob < 4; ++ob) {
+        bias[ob][0][lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        bias[ob][1][lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    stage_w2(p, F, true);
     stage_vectors(p, vec);
     MlpRegs R;
     load_mlp(p, R, lane);
@@ -408,25 +421,37 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
         gW1[a] = zero;
         for (int b = 0; b < 4; ++b) gW2[a][b] = zero;
     }
-    float gb1[4][4], gb2[4][4], gw3[4][4], gb3 = 0.0f, lsum = 0.0f;
-    for (int a = 0; a < 4; ++a)
-        for (int q = 0; q < 4; ++q) gb1[a][q] = gb2[a][q] = gw3[a][q] = 0.0f;
+    float gb3 = 0.0f, lsum = 0.0f;
     const float inv = 1.0f / (float)count;
-    uint16_t* sc = tr[wave];
+    uint16_t* sc = sh.loop.tr[wave];
+    // input 8 of every edge is 1 (W1's padding columns are zero, so the
+    // forward pass does not see it): column 8 of the W1 gradient is b1's
+    const bf16x4 one8 = pack4(1.0f, 0.0f, 0.0f, 0.0f);
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave; t * 16 < count; t += waves) {
+    int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    TileIn next = load_tile(x, y, t * 16, count, lane);
+    for (; t * 16 < count; t += waves) {
         const int64_t e0 = t * 16;
         const bool valid = e0 + col < count;
-        const bf16x4 xb = load_x(x, e0, count, lane);
-        const float yy = valid ? y[e0 + col] : 0.0f;
+        const TileIn cur = next;
+        next = load_tile(x, y, (t + waves) * 16, count, lane);
+        const bf16x4 xb = grp == 2 ? one8 : pack4(cur.x.x, cur.x.y, cur.x.z, cur.x.w);
+        const float yy = cur.y;
         Activations A;
-        forward_tile<true>(R, &F, vec, xb, A, lane);
-        const float s = sigmoidf(A.out);
+        forward_tile<true>(R, F, vec, xb, A, lane);
+        // s = sigmoid(out) and the loss terms -log s = max(-out, 0) + L,
+        // -log(1 - s) = max(out, 0) + L with u = e^{-|out|}, L = log(1 + u),
+        // s = 1 / (1 + u) for out >= 0 and u / (1 + u) otherwise: one
+        // hardware exponential, reciprocal and logarithm
+        const float u = __builtin_amdgcn_exp2f(fabsf(A.out) * -LOG2E_F);
+        const float r = __builtin_amdgcn_rcpf(1.0f + u);
+        const float s = A.out >= 0.0f ? r : u * r;
         // d loss / d logit for -w y log s - (1-y) log(1-s)
         const float g = valid ? (s * (1.0f - yy + pos_weight * yy) - pos_weight * yy) * inv : 0.0f;
         if (grp == 0 && valid) {
-            const float sp_pos = fmaxf(-A.out, 0.0f) + log1pf(expf(-fabsf(A.out)));  // -log s
-            const float sp_neg = fmaxf(A.out, 0.0f) + log1pf(expf(-fabsf(A.out)));   // -log(1-s)
+            const float L = __builtin_amdgcn_logf(1.0f + u) * LN2_F;
+            const float sp_pos = fmaxf(-A.out, 0.0f) + L;  // -log s
+            const float sp_neg = fmaxf(A.out, 0.0f) + L;   // -log(1-s)
             lsum += (pos_weight * yy * sp_pos + (1.0f - yy) * sp_neg) * inv;
         }
         if (grp == 0) gb3 += g;
@@ -436,11 +461,18 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
             float dz[4];
             const float4 w = vec4(vec, VEC_W3, ob, grp);
             const float w3[4] = {w.x, w.y, w.z, w.w};
-            for (int q = 0; q < 4; ++q) {
-                gw3[ob][q] += g * bf2f((uint16_t)A.h2[ob][q]);
-                dz[q] = g * w3[q] * A.d2[ob][q];
-                gb2[ob][q] += dz[q];
-            }
+            for (int q = 0; q < 4; ++q) dz[q] = g * w3[q] * A.d2[ob][q];
+            float4 sb = bias[ob][0][lane], sw = bias[ob][1][lane];  // this lane's running sums
+            sb.x += dz[0];
+            sb.y += dz[1];
+            sb.z += dz[2];
+            sb.w += dz[3];
+            sw.x += g * bf2f((uint16_t)A.h2[ob][0]);
+            sw.y += g * bf2f((uint16_t)A.h2[ob][1]);
+            sw.z += g * bf2f((uint16_t)A.h2[ob][2]);
+            sw.w += g * bf2f((uint16_t)A.h2[ob][3]);
+            bias[ob][0][lane] = sb;
+            bias[ob][1][lane] = sw;
             dz2b[ob] = pack4(dz[0], dz[1], dz[2], dz[3]);
         }
         // dW2[o][i] = sum_e dz2[o][e] h1[i][e]: A = dz2 (o x e), B = h1^T
@@ -458,88 +490,142 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const MlpParams p, const f
             f32x4 dh = zero;
             for (int c = 0; c < 2; ++c) {
                 bf16x4 a0, a1;
-                frag_halves(F.bwd[(ib * 2 + c) * 64 + lane], a0, a1);
+                frag_halves(F->bwd[(ib * 2 + c) * 64 + lane], a0, a1);
                 dh = mfma32(a0, a1, dz2b[2 * c], dz2b[2 * c + 1], dh);
             }
-            float dz[4];
-            for (int q = 0; q < 4; ++q) {
-                dz[q] = dh[q] * A.d1[ib][q];
-                gb1[ib][q] += dz[q];
-            }
-            t2in[ib] = pack4(dz[0], dz[1], dz[2], dz[3]);
+            t2in[ib] = pack4(dh[0] * A.d1[ib][0], dh[1] * A.d1[ib][1], dh[2] * A.d1[ib][2], dh[3] * A.d1[ib][3]);
         }
         // dW1[o][i] = sum_e dz1[o][e] x[i][e]: B = x^T (e x i), i in 0..15
         t2in[4] = xb;
         transpose16_batch<5>(t2in, t2out, sc, lane);
         for (int ob = 0; ob < 4; ++ob) gW1[ob] = mfma16(t2out[ob], t2out[4], gW1[ob]);
     }
-    // bias / w3 gradients: reduce over the 16 edge lanes of each group
+    // b2 / w3 gradients: reduce each lane's LDS sums over the 16 edge lanes
+    float gb2[4][4], gw3[4][4];
+    for (int ob = 0; ob < 4; ++ob) {
+        const float4 sb = bias[ob][0][lane], sw = bias[ob][1][lane];
+        gb2[ob][0] = sb.x;
+        gb2[ob][1] = sb.y;
+        gb2[ob][2] = sb.z;
+        gb2[ob][3] = sb.w;
+        gw3[ob][0] = sw.x;
+        gw3[ob][1] = sw.y;
+        gw3[ob][2] = sw.z;
+        gw3[ob][3] = sw.w;
+    }
     for (int ob = 0; ob < 4; ++ob)
-        for (int q = 0; q < 4; ++q)
-            for (int off = 8; off >= 1; off >>= 1) {
-                gb1[ob][q] += __shfl_xor(gb1[ob][q], off);
-                gb2[ob][q] += __shfl_xor(gb2[ob][q], off);
-                gw3[ob][q] += __shfl_xor(gw3[ob][q], off);
-            }
+        for (int q = 0; q < 4; ++q) {
+            gb2[ob][q] = row16_sum(gb2[ob][q]);
+            gw3[ob][q] = row16_sum(gw3[ob][q]);
+        }
     for (int off = 32; off >= 1; off >>= 1) {
         gb3 += __shfl_xor(gb3, off);
         lsum += __shfl_xor(lsum, off);
     }
-    // the block's row: waves add theirs in LDS in wave order (every entry of
-    // a row has exactly one writing lane), then one coalesced store
-    for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) {
-        if (wave == wv) {
-            const bool first = wv == 0;
-            auto put = [&](int idx, float v) { acc[idx] = first ? v : acc[idx] + v; };
-            // D layout of a gradient tile: row 4grp+q, column col
-            for (int ob = 0; ob < 4; ++ob)
-                for (int q = 0; q < 4; ++q) {
-                    const int o = 16 * ob + 4 * grp + q;
-                    if (col < 8) put(MLP_W1 + o * 8 + col, gW1[ob][q]);
-                    for (int ib = 0; ib < 4; ++ib) put(MLP_W2 + o * 64 + 16 * ib + col, gW2[ob][ib][q]);
-                    if (col == 0) {
-                        put(MLP_B1 + o, gb1[ob][q]);
-                        put(MLP_B2 + o, gb2[ob][q]);
-                        put(MLP_W3 + o, gw3[ob][q]);
-                    }
-                }
-            if (lane == 0) {
-                put(MLP_B3, gb3);
-                put(MLP_PARAMS, lsum);
-            }
+    __syncthreads();  // every wave is past its last use of the loop's LDS
+    constexpr int HALF = MLP_BWD_WAVES / 2;
+    float4* row = sh.rows[wave % HALF];
+    // one wave's gradients into `row`: stored (upper half) or added (lower)
+    auto put = [&](int idx, float4 v, bool add) {
+        if (add) {
+            const float4 r = row[idx];
+            v = make_float4(r.x + v.x, r.y + v.y, r.z + v.z, r.w + v.w);
         }
-        __syncthreads();
+        row[idx] = v;
+    };
+    auto put_all = [&](bool add) {
+        for (int ob = 0; ob < 4; ++ob)
+            for (int ib = 0; ib < 4; ++ib) {
+                const f32x4 g4 = gW2[ob][ib];
+                put(MLP_ROW_W2 / 4 + (ob * 4 + ib) * 64 + lane, make_float4(g4[0], g4[1], g4[2], g4[3]), add);
+            }
+        for (int ob = 0; ob < 4; ++ob) {
+            const f32x4 g4 = gW1[ob];
+            put(MLP_ROW_W1 / 4 + ob * 64 + lane, make_float4(g4[0], g4[1], g4[2], g4[3]), add);
+        }
+        if (col == 0)  // lane 16 grp holds its group's row totals
+            for (int ob = 0; ob < 4; ++ob) {
+                put(MLP_ROW_B2 / 4 + grp * 4 + ob, make_float4(gb2[ob][0], gb2[ob][1], gb2[ob][2], gb2[ob][3]), add);
+                put(MLP_ROW_W3 / 4 + grp * 4 + ob, make_float4(gw3[ob][0], gw3[ob][1], gw3[ob][2], gw3[ob][3]), add);
+            }
+        if (lane == 0) put(MLP_ROW_B3 / 4, make_float4(gb3, lsum, 0.0f, 0.0f), add);
+    };
+    if (wave >= HALF) put_all(false);
+    __syncthreads();
+    if (wave < HALF) put_all(true);
+    __syncthreads();
+    float4* out = reinterpret_cast<float4*>(partial + (size_t)blockIdx.x * MLP_ROW);
+    for (int i = threadIdx.x; i < MLP_ROW / 4; i += blockDim.x) {
+        float4 sum = sh.rows[0][i];
+        for (int w = 1; w < HALF; ++w) {
+            const float4 r = sh.rows[w][i];
+            sum = make_float4(sum.x + r.x, sum.y + r.y, sum.z + r.z, sum.w + r.w);
+        }
+        out[i] = sum;
     }
-    float* row = partial + (size_t)blockIdx.x * MLP_STRIDE;
-    for (int i = threadIdx.x; i < MLP_STRIDE; i += blockDim.x) row[i] = acc[i];
 }
 
-// Sum of the per-wave partial rows in a fixed order, in two stages.  Stage 1:
-// block (x, y) sums rows 64 y + 16 g .. +15 (g = thread / 64) of 64 columns
-// into row 4 y + g of `stage`.  Stage 2: grad = sum of the stage rows in
-// order, loss += its last column.
-__global__ __launch_bounds__(256) void k_reduce_grad1(const float* __restrict__ partial, int32_t rows,
-                                                      float* __restrict__ stage) {
-    const int32_t col = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int32_t g = threadIdx.x >> 6;
-    if (col >= MLP_STRIDE) return;
-    const int32_t r0 = blockIdx.y * 64 + g * 16;
-    float acc = 0.0f;
-    for (int32_t i = 0; i < 16; ++i)
-        if (r0 + i < rows) acc += partial[(size_t)(r0 + i) * MLP_STRIDE + col];
-    stage[(size_t)(blockIdx.y * 4 + g) * MLP_STRIDE + col] = acc;
+// Parameter index of slot s of a partial row (layout MLP_ROW_*); MLP_PARAMS
+// for the loss, -1 for a padding slot.  Accumulator slots: lane l, value q
+// of block (ob, ib) is row o = 16 ob + 4 (l / 16) + q, column 16 ib + l % 16.
+__device__ __forceinline__ int32_t row_slot_param(int32_t s) {
+    if (s < MLP_ROW_W1) {
+        const int32_t lane = (s >> 2) & 63, q = s & 3, blk = s >> 8;
+        const int32_t o = 16 * (blk >> 2) + 4 * (lane >> 4) + q, i = 16 * (blk & 3) + (lane & 15);
+        return MLP_W2 + o * 64 + i;
+    }
+    if (s < MLP_ROW_B2) {
+        const int32_t t = s - MLP_ROW_W1, lane = (t >> 2) & 63, q = t & 3;
+        const int32_t o = 16 * (t >> 8) + 4 * (lane >> 4) + q, c = lane & 15;
+        return c < 8 ? MLP_W1 + o * 8 + c : (c == 8 ? MLP_B1 + o : -1);
+    }
+    if (s < MLP_ROW_B3) {
+        const int32_t t = (s - MLP_ROW_B2) & 63, g = t >> 4, ob = (t >> 2) & 3, q = t & 3;
+        return (s < MLP_ROW_W3 ? MLP_B2 : MLP_W3) + 16 * ob + 4 * g + q;
+    }
+    if (s == MLP_ROW_B3) return MLP_B3;
+    if (s == MLP_ROW_LOSS) return MLP_PARAMS;
+    return -1;
 }
 
-__global__ __launch_bounds__(256) void k_reduce_grad2(const float* __restrict__ stage, int32_t rows,
+// Sum of the partial rows in a fixed order: block b takes slots 32 b ..
+// 32 b + 31; its thread group g = thread / 32 (of 32) adds rows g, g + 32,
+// g + 64, ... in order (loads issued together, at most 8 per thread), and
+// the group sums are added in order g = 0..31.  grad gets the parameter
+// slots, loss accumulates the loss slot.
+enum : int {
+    REDUCE_COLS = 32,
+    REDUCE_GROUPS = 32,
+    REDUCE_ROWS_PER_GROUP = MLP_MAX_ROWS / REDUCE_GROUPS,
+};
+static_assert(REDUCE_GROUPS * REDUCE_ROWS_PER_GROUP == MLP_MAX_ROWS, "every partial row is summed");
+static_assert(REDUCE_COLS * REDUCE_GROUPS == 1024, "one 1024-thread block");
+
+__global__ __launch_bounds__(1024) void k_reduce_grad(const float* __restrict__ partial, int32_t rows,
                                                       float* __restrict__ grad, float* __restrict__ loss) {
-    const int32_t j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= MLP_STRIDE) return;
-    float acc = 0.0f;
-    for (int32_t r = 0; r < rows; ++r) acc += stage[(size_t)r * MLP_STRIDE + j];
-    if (j < MLP_PARAMS)
-        grad[j] = acc;
-    else
-        *loss += acc;
+    __shared__ float part[REDUCE_GROUPS][REDUCE_COLS];
+    const int32_t c = threadIdx.x % REDUCE_COLS, g = threadIdx.x / REDUCE_COLS;
+    const int32_t slot = blockIdx.x * REDUCE_COLS + c;
+    float v[REDUCE_ROWS_PER_GROUP];
+#pragma unroll
+    for (int k = 0; k < REDUCE_ROWS_PER_GROUP; ++k) {
+        const int32_t r = g + REDUCE_GROUPS * k;
+        v[k] = (slot < MLP_ROW && r < rows) ? partial[(size_t)r * MLP_ROW + slot] : 0.0f;
+    }
+    float acc = v[0];
+#pragma unroll
+    for (int k = 1; k < REDUCE_ROWS_PER_GROUP; ++k) acc += v[k];
+    part[g][c] = acc;
+    __syncthreads();
+    if (g == 0 && slot < MLP_ROW) {
+        float sum = part[0][c];
+        for (int k = 1; k < REDUCE_GROUPS; ++k) sum += part[k][c];
+        const int32_t j = row_slot_param(slot);
+        if (j >= 0 && j < MLP_PARAMS)
+            grad[j] = sum;
+        else if (j == MLP_PARAMS)
+            *loss += sum;
+    }
 }
 
 // Adam on the fp32 master parameters; bc1 = 1 - beta1^t and bc2 = 1 - beta2^t
@@ -561,13 +647,38 @@ __global__ __launch_bounds__(256) void k_adam(const MlpParams p, float* __restri
     }
 }
 
+__device__ __forceinline__ uint32_t bf_pair(float lo, float hi) {
+    return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+
+// bf16 operand fragments of W1 (64 x 8, padded to 16 inputs: row 16 ob +
+// l%16, inputs 4(l/16) + 0..3) and W2 (layout above) from the fp32 master.
 __global__ __launch_bounds__(256) void k_pack_bf16(MlpParams p) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < 64 * 16) {
-        const int o = i / 16, c = i % 16;
-        p.w1_bf16[i] = c < 8 ? f2bf(p.master[MLP_W1 + o * 8 + c]) : (uint16_t)0;
+    const float* m = p.master;
+    if (i < MLP_W1_FRAGS) {
+        const int ob = i >> 6, l = i & 63, o = 16 * ob + (l & 15), c = 4 * (l >> 4);
+        const float* w = m + MLP_W1 + o * 8;
+        p.w1_frag[i] = c < 8 ? make_uint2(bf_pair(w[c], w[c + 1]), bf_pair(w[c + 2], w[c + 3])) : make_uint2(0u, 0u);
     }
-    if (i < 64 * 64) p.w2_bf16[i] = f2bf(p.master[MLP_W2 + i]);
+    if (i < MLP_W2_FRAGS) {
+        const int f = i & 511, blk = f >> 7, c = (f >> 6) & 1, l = f & 63, r = l & 15, g = l >> 4;
+        const int k0 = 16 * (2 * c) + 4 * g, k1 = 16 * (2 * c + 1) + 4 * g;
+        const float* w = m + MLP_W2;  // 64 x 64, row = output feature
+        if (i < 512) {
+            const int o = 16 * blk + r;  // rows o, columns k0.. and k1..
+            p.w2_frag[i] = make_uint4(bf_pair(w[o * 64 + k0], w[o * 64 + k0 + 1]),
+                                      bf_pair(w[o * 64 + k0 + 2], w[o * 64 + k0 + 3]),
+                                      bf_pair(w[o * 64 + k1], w[o * 64 + k1 + 1]),
+                                      bf_pair(w[o * 64 + k1 + 2], w[o * 64 + k1 + 3]));
+        } else {
+            const int in = 16 * blk + r;  // column in, rows k0.. and k1..
+            p.w2_frag[i] = make_uint4(bf_pair(w[k0 * 64 + in], w[(k0 + 1) * 64 + in]),
+                                      bf_pair(w[(k0 + 2) * 64 + in], w[(k0 + 3) * 64 + in]),
+                                      bf_pair(w[k1 * 64 + in], w[(k1 + 1) * 64 + in]),
+                                      bf_pair(w[(k1 + 2) * 64 + in], w[(k1 + 3) * 64 + in]));
+        }
+    }
 }
 
 // Heuristic on candidate edges from the score l: transform 0 gives
@@ -613,18 +724,17 @@ hipError_t launch_mlp_forward(const MlpParams& p, const float* x, int64_t count,
 static unsigned backward_blocks(int64_t count) {
     const int64_t tiles = (count + 15) / 16;
     const int64_t waves = tiles < MLP_MAX_WAVES ? tiles : MLP_MAX_WAVES;
-    return (unsigned)((waves + 3) / 4 > 0 ? (waves + 3) / 4 : 1);
+    const int64_t blocks = (waves + MLP_BWD_WAVES - 1) / MLP_BWD_WAVES;
+    return (unsigned)(blocks > 0 ? blocks : 1);
 }
 
 hipError_t launch_mlp_backward(const MlpParams& p, const float* x, const float* y, int64_t count, float pos_weight,
                                float* partial, float* grad, float* loss, hipStream_t s) {
     const unsigned blocks = backward_blocks(count);  // one partial row per block
-    hipLaunchKernelGGL(k_mlp_backward, dim3(blocks), dim3(256), 0, s, p, x, y, count, pos_weight, partial);
-    const int32_t rows = (int32_t)blocks;
-    const int32_t chunks = (rows + 63) / 64;
-    float* stage = partial + (size_t)MLP_MAX_ROWS * MLP_STRIDE;
-    hipLaunchKernelGGL(k_reduce_grad1, dim3(cdiv(MLP_STRIDE, 64), chunks), dim3(256), 0, s, partial, rows, stage);
-    hipLaunchKernelGGL(k_reduce_grad2, dim3(cdiv(MLP_STRIDE, 256)), dim3(256), 0, s, stage, chunks * 4, grad, loss);
+    hipLaunchKernelGGL(k_mlp_backward, dim3(blocks), dim3(64 * MLP_BWD_WAVES), 0, s, p, x, y, count, pos_weight,
+                       partial);
+    hipLaunchKernelGGL(k_reduce_grad, dim3(cdiv(MLP_ROW, REDUCE_COLS)), dim3(1024), 0, s, partial, (int32_t)blocks,
+                       grad, loss);
     return hipGetLastError();
 }
 
@@ -636,7 +746,7 @@ hipError_t launch_adam(const MlpParams& p, float* grad, float* m, float* v, floa
 }
 
 hipError_t launch_pack_bf16(const MlpParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack_bf16, dim3(cdiv(64 * 64, 256)), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_pack_bf16, dim3(cdiv(MLP_W2_FRAGS, 256)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
@@ -659,8 +769,8 @@ struct aco_prior {
     aco_prior_config cfg;
     hipStream_t stream = nullptr;
     float* d_master = nullptr;
-    uint16_t* d_w1 = nullptr;
-    uint16_t* d_w2 = nullptr;
+    uint2* d_w1 = nullptr;  // bf16 operand fragments of W1 and W2
+    uint4* d_w2 = nullptr;
     float* d_m = nullptr;
     float* d_v = nullptr;
     float* d_grad = nullptr;
@@ -803,11 +913,11 @@ extern "C" int aco_prior_create(const aco_prior_config* cfg, aco_prior** out) {
         delete pr;
         return ACO_E_HIP;
     }
-    if ((rc = palloc(pr, &pr->d_master, ACO_PRIOR_PARAMS)) || (rc = palloc(pr, &pr->d_w1, 64 * 16)) ||
-        (rc = palloc(pr, &pr->d_w2, 64 * 64)) || (rc = palloc(pr, &pr->d_m, ACO_PRIOR_PARAMS)) ||
+    if ((rc = palloc(pr, &pr->d_master, ACO_PRIOR_PARAMS)) || (rc = palloc(pr, &pr->d_w1, aco::MLP_W1_FRAGS)) ||
+        (rc = palloc(pr, &pr->d_w2, aco::MLP_W2_FRAGS)) || (rc = palloc(pr, &pr->d_m, ACO_PRIOR_PARAMS)) ||
         (rc = palloc(pr, &pr->d_v, ACO_PRIOR_PARAMS)) || (rc = palloc(pr, &pr->d_grad, ACO_PRIOR_PARAMS)) ||
         (rc = palloc(pr, &pr->d_loss, 1)) || (rc = palloc(pr, &pr->d_sum, 1)) ||
-        (rc = palloc(pr, &pr->d_partial, (size_t)(aco::MLP_MAX_ROWS + aco::MLP_MAX_ROWS / 16) * aco::MLP_STRIDE))) {
+        (rc = palloc(pr, &pr->d_partial, (size_t)aco::MLP_MAX_ROWS * aco::MLP_ROW))) {
         snprintf(g_prior_err, sizeof(g_prior_err), "%s", pr->err);
         aco_prior_destroy(pr);
         return rc;

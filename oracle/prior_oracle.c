@@ -12,8 +12,9 @@
  *                        dictionaries and numpy float32 arithmetic
  *   orc_prior_labels     full candidate lists: exactly 2n labels, symmetric
  *   orc_gelu             scipy normal CDF: within the Abramowitz-Stegun bound
- *                        7.5e-8 |x| of x Phi(x); gradient = finite differences,
- *                        within 5.4e-7 of Phi(x) + x phi(x)
+ *                        7.5e-8 |x| of x Phi(x); gradient within 7.5e-8 of the
+ *                        exact derivative Phi(x) + x phi(x) and of central
+ *                        finite differences of orc_gelu
  *   orc_mlp_forward      zero network (= b3); one-path networks against
  *                        scalar closed forms (catches transposed operands)
  *   orc_mlp_grad         central finite differences of orc_mlp_forward's loss
@@ -95,33 +96,31 @@ double orc_bf16(double x) {
  * 1.5e-7, so |error of Phi| <= 7.5e-8): with u = |x| / sqrt(2),
  * erf(u) ~ 1 - P(t) e^{-u^2}, t = 1 / (1 + p u), P(t) = a1 t + ... + a5 t^5,
  * so Phi(x) = 1 - Q for x >= 0 and Q for x < 0, Q = P(t) e^{-x^2/2} / 2.
- * The gradient is the exact derivative of this function:
- * Phi'(x) = e^{-x^2/2} (c t^2 P'(t) + |x| P(t)) / 2 with c = p / sqrt(2).
- * The device evaluates the same expressions in fp32. */
+ * The gradient is GELU'(x) = Phi(x) + x phi(x) with this Phi and the exact
+ * normal density phi(x) = e^{-x^2/2} / sqrt(2 pi).  The device evaluates the
+ * same expressions in fp32. */
 static const double GELU_A[5] = {0.254829592, -0.284496736, 1.421413741, -1.453152027, 1.061405429};
 static const double GELU_P = 0.3275911;
 
-static void gelu_phi(double x, double *phi, double *dphi) {
+/* Phi(x) and e^{-x^2/2} */
+static double gelu_phi(double x, double *e) {
     const double a = fabs(x), c = GELU_P / sqrt(2.0);
-    const double e = exp(-0.5 * x * x);
+    *e = exp(-0.5 * x * x);
     const double t = 1.0 / (1.0 + c * a);
     const double P = t * (GELU_A[0] + t * (GELU_A[1] + t * (GELU_A[2] + t * (GELU_A[3] + t * GELU_A[4]))));
-    const double dP = GELU_A[0] + t * (2 * GELU_A[1] + t * (3 * GELU_A[2] + t * (4 * GELU_A[3] + t * 5 * GELU_A[4])));
-    const double q = 0.5 * P * e;
-    *phi = x >= 0 ? 1.0 - q : q;
-    *dphi = 0.5 * e * (c * t * t * dP + a * P);
+    const double q = 0.5 * P * *e;
+    return x >= 0 ? 1.0 - q : q;
 }
 
 double orc_gelu(double x) {
-    double phi, dphi;
-    gelu_phi(x, &phi, &dphi);
-    return x * phi;
+    double e;
+    return x * gelu_phi(x, &e);
 }
 
 double orc_gelu_grad(double x) {
-    double phi, dphi;
-    gelu_phi(x, &phi, &dphi);
-    return phi + x * dphi;
+    double e;
+    const double phi = gelu_phi(x, &e);
+    return phi + x * e / sqrt(2.0 * M_PI);
 }
 
 /* log(1 + e^x) without overflow */
@@ -166,8 +165,9 @@ void orc_mlp_forward(const float *params, const float *x, int64_t count, int32_t
 /* Mean weighted binary cross-entropy with logits over `count` edges and its
  * gradient (ACO_PRIOR_PARAMS doubles, overwritten); returns the loss.  With
  * bf16 != 0 the back-propagated pre-activation gradients dz2 and dz1 are
- * rounded to bfloat16 where they enter the weight-gradient and dh1 products
- * (the bias gradients take them unrounded), as the device does. */
+ * rounded to bfloat16 where they enter the weight-gradient and dh1 products,
+ * as the device does; b2's gradient takes dz2 unrounded, b1's comes out of
+ * the W1 product (a constant-one input) and takes dz1 rounded. */
 double orc_mlp_grad(const float *params, const float *x, const float *y, int64_t count, float pos_weight,
                     int32_t bf16, double *grad) {
     memset(grad, 0, sizeof(double) * 4801);
@@ -195,7 +195,7 @@ double orc_mlp_grad(const float *params, const float *x, const float *y, int64_t
             double dh = 0.0;
             for (int o = 0; o < H; ++o) dh += rnd((double)params[P_W2 + o * H + i], bf16) * dz2r[o];
             dz1[i] = dh * orc_gelu_grad(a.z1[i]);
-            grad[P_B1 + i] += dz1[i];
+            grad[P_B1 + i] += rnd(dz1[i], bf16);
         }
         for (int o = 0; o < H; ++o) {
             const double d = rnd(dz1[o], bf16);
