@@ -29,6 +29,7 @@ struct aco_ctx {
     int32_t* d_dist = nullptr;
     float* d_tau[2] = {nullptr, nullptr};
     float* d_choice = nullptr;
+    int32_t* d_xy = nullptr;  // coordinates, padded to a multiple of four cities (sweep distances)
     float* d_choice_nn = nullptr;
     float* d_tau_nn = nullptr;
     float* d_eta_nn = nullptr;
@@ -38,6 +39,7 @@ struct aco_ctx {
     float* d_eta_full = nullptr;  // aging: heuristic of every edge, pitched (young ants' fallback rows)
     int32_t eta_len = 0;
     int32_t eta_pow = 0;  // aco::eta_integer_power of (beta, largest distance)
+    int32_t sweep_eta_pow = 0;  // eta_pow when the sweep's own reciprocal matches the table, else 0
     float* d_mean = nullptr;
     int32_t* d_tours = nullptr;
     int64_t* d_len = nullptr;
@@ -115,7 +117,7 @@ extern "C" int aco_params_init(aco_params* p) {
 static void free_all(aco_ctx* c) {
     if (c->graph) hipGraphExecDestroy(c->graph);
     if (c->graph_long) hipGraphExecDestroy(c->graph_long);
-    void* ptrs[] = {c->d_dist, c->d_tau[0], c->d_tau[1], c->d_choice, c->d_choice_nn, c->d_nn, c->d_eta,
+    void* ptrs[] = {c->d_dist, c->d_tau[0], c->d_tau[1], c->d_choice, c->d_xy, c->d_choice_nn, c->d_nn, c->d_eta,
                     c->d_tau_nn, c->d_eta_nn, c->d_nn16,
                     c->d_mean, c->d_tours, c->d_len, c->d_prev_len, c->d_age, c->d_perf, c->d_valid,
                     c->d_best_tour, c->d_dep_ant, c->d_dep_val, c->d_state, c->d_scratch64, c->d_adj,
@@ -310,12 +312,24 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
         for (int32_t i = 0; i < 2 * c->n; ++i)
             if (xy[i] >= (1 << 29) || xy[i] <= -(1 << 29))
                 return bail(fail(c, ACO_E_RANGE, "coordinate %d out of range (|x| < 2^29)", xy[i]));
+        // Coordinates below XY_SWEEP_LIMIT in magnitude stay on the device:
+        // the sweep computes its rows' distances from them (exactly the
+        // matrix's values) instead of reading the n^2 matrix every iteration.
+        bool keep = true;
+        for (int32_t i = 0; i < 2 * c->n; ++i)
+            keep = keep && xy[i] < aco::XY_SWEEP_LIMIT && xy[i] > -aco::XY_SWEEP_LIMIT;
+        const size_t padded = (size_t)2 * (((size_t)c->n + 3) & ~(size_t)3);
         int32_t* d_xy = nullptr;
-        TRYB(dalloc(c, &d_xy, (size_t)2 * c->n));
-        hipError_t e = hipMemcpyAsync(d_xy, xy, sizeof(int32_t) * 2 * c->n, hipMemcpyHostToDevice, c->stream);
+        TRYB(dalloc(c, &d_xy, padded));
+        hipError_t e = hipMemsetAsync(d_xy, 0, sizeof(int32_t) * padded, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_xy, xy, sizeof(int32_t) * 2 * c->n, hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = aco::launch_dist_from_xy(d_xy, c->n, c->P, c->d_dist, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        hipFree(d_xy);
+        if (keep && e == hipSuccess)
+            c->d_xy = d_xy;
+        else
+            hipFree(d_xy);
         HIPB(e);
     } else {
         // The reference passes a device pointer in its GPU build and a host
@@ -350,6 +364,17 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     }
     TRYB(dalloc(c, &c->d_eta, (size_t)c->eta_len));
     HIPB(hipMemcpy(c->d_eta, eta.data(), sizeof(float) * eta.size(), hipMemcpyHostToDevice));
+    // The sweep computes 1 / d^beta with its own reciprocal sequence; it is
+    // used only if it reproduces every table entry.
+    if (c->eta_pow) {
+        int32_t* d_flag = reinterpret_cast<int32_t*>(c->d_scratch64);
+        int32_t flag = 1;
+        HIPB(hipMemsetAsync(d_flag, 0, sizeof(int32_t), c->stream));
+        HIPB(aco::launch_eta_check(c->eta_pow, c->d_eta, c->eta_len, d_flag, c->stream));
+        HIPB(hipMemcpyAsync(&flag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPB(hipStreamSynchronize(c->stream));
+        c->sweep_eta_pow = flag == 0 ? c->eta_pow : 0;
+    }
     // Aging colonies: young ants weigh a fallback row's every city with the
     // heuristic, a gather per city from the table; a pitched copy of the
     // table's values turns that into row loads (one memory round trip)
@@ -448,6 +473,7 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur, const aco::LaunchTiming& t)
     sa.dist = c->d_dist;
     sa.eta = c->d_eta;
     sa.eta_full = c->d_eta_full;
+    sa.xy = c->d_xy;
     sa.eta_len = c->eta_len;
     sa.nn = cand ? c->d_nn : nullptr;
     sa.k = c->k;
@@ -464,7 +490,7 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur, const aco::LaunchTiming& t)
     sa.alpha = c->p.alpha;
     sa.one_minus_rho = 1.0f - c->p.rho;
     sa.want_mean = c->p.aging;
-    sa.eta_pow = c->eta_pow;
+    sa.eta_pow = c->sweep_eta_pow;
     sa.state = c->d_state;
     sa.top = c->d_top;
     sa.top_thr = c->d_top_thr;

@@ -61,7 +61,12 @@ struct SweepArgs {
     uint16_t* top;           // n x TOP_K row top lists (null: none)
     float* top_thr;          // n: the TOP_K-th weight of the previous sweep
     const float* eta_full;   // pitched heuristic of every edge (read instead of dist; may be null)
+    const int32_t* xy;       // 2n interleaved coordinates, |x| < XY_SWEEP_LIMIT: distances computed, not read (may be null)
 };
+
+// Largest coordinate magnitude (exclusive) for which the sweep computes its
+// rows' distances from the coordinates (see euc2d_fast).
+constexpr int32_t XY_SWEEP_LIMIT = 1 << 20;
 
 struct ConstructArgs {
     const int32_t* nn;       // n x k (candidate) or n x n (full row)
@@ -246,6 +251,9 @@ int32_t batch_construct_blocks_per_cu(int32_t nmax, int32_t ants_per_block, int3
                                       size_t lds_bytes);
 
 hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s);
+// *flag |= 1 unless the sweep's computed heuristic 1 / d^eta_pow equals the
+// table eta[0 .. eta_len) bit for bit.
+hipError_t launch_eta_check(int32_t eta_pow, const float* eta, int32_t eta_len, int32_t* flag, hipStream_t s);
 // out (pitched n x P) = the heuristic table's value of every edge's distance
 hipError_t launch_eta_full(const int32_t* dist, const float* eta, int32_t eta_len, int32_t n, int32_t P, float* out,
                            hipStream_t s);

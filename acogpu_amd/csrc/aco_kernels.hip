@@ -95,6 +95,21 @@ __device__ __forceinline__ int32_t euc2d_exact(int64_t dx, int64_t dy) {
     return (int32_t)r;
 }
 
+// The same distance for coordinate differences below 2^21 in magnitude, in
+// fp64 without 64-bit integer products and without a loop: D = dx^2 + dy^2 <
+// 2^43 is exact; s = sqrt(fp32(D)) (v_sqrt_f32, 1 ulp) is within 0.53 of
+// sqrt(D) < 2^21.5, so r = rint(s) is within 1 of nint(sqrt(D)), and one step
+// with the exact residual e = D - r^2 corrects it (nint(sqrt(D)) = r iff
+// -r < e <= r, and r = 0 iff D = 0).
+__device__ __forceinline__ int32_t euc2d_fast(int32_t dx, int32_t dy) {
+    const double x = (double)dx, y = (double)dy;
+    const double D = __builtin_fma(x, x, y * y);
+    const double r = __builtin_rint((double)__builtin_amdgcn_sqrtf((float)D));
+    const double e = __builtin_fma(-r, r, D);
+    const double up = e > r ? 1.0 : 0.0, down = (r > 0.0 && e <= -r) ? 1.0 : 0.0;
+    return (int32_t)(r + up - down);
+}
+
 __global__ __launch_bounds__(256) void k_dist_from_xy(const int32_t* __restrict__ xy, int32_t n, int32_t P,
                                                       int32_t* __restrict__ dist) {
     const int32_t i = blockIdx.y;
@@ -346,105 +361,279 @@ __device__ __forceinline__ void top_finish(const SweepArgs& a, TopSel& t, int32_
     }
 }
 
-__device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf) {
+// One row's sweep state: pointers, the row sum (lane l adds its columns
+// 256 s + 4 l + q of every 1024-column chunk in s, q order) and the top-list
+// selection.
+struct SweepRow {
+    const float* __restrict__ trow;
+    const int32_t* __restrict__ drow;
+    const float* __restrict__ erow;  // eta_full row (null: from distances)
+    float* __restrict__ crow;
+    float* __restrict__ nrow;
+    int32_t xi, yi;                  // the row city's coordinates (a.xy set)
+    float acc;
+    TopSel top;
+};
+
+__device__ __forceinline__ SweepRow sweep_row_begin(const SweepArgs& a, int32_t i, uint64_t* topbuf) {
     const size_t row = (size_t)i * a.P;
-    const float* __restrict__ trow = a.tau + row;
-    const int32_t* __restrict__ drow = a.dist + row;
-    float* __restrict__ crow = a.choice + row;
-    float* __restrict__ nrow = a.tau_next + row;
-    float acc = 0.0f;
-    TopSel top{topbuf, 0.0f, 0};
+    SweepRow r;
+    r.trow = a.tau + row;
+    r.drow = a.dist + row;
+    r.erow = a.eta_full != nullptr ? a.eta_full + row : nullptr;
+    r.crow = a.choice + row;
+    r.nrow = a.tau_next + row;
+    r.xi = a.xy != nullptr ? a.xy[2 * i] : 0;
+    r.yi = a.xy != nullptr ? a.xy[2 * i + 1] : 0;
+    r.acc = 0.0f;
+    r.top = TopSel{topbuf, 0.0f, 0};
     if (a.top != nullptr) {
         // lower bound of this sweep's TOP_K-th weight: (1 - rho) w (1 - u)^3
         // >= fl(fl(w (1 - rho)) (1 - 2^-20)) for the unit roundoff u = 2^-24
-        top.thr = (a.top_thr[i] * a.one_minus_rho) * 0.99999904632568359375f;
+        r.top.thr = (a.top_thr[i] * a.one_minus_rho) * 0.99999904632568359375f;
     }
-    const float* __restrict__ erow = a.eta_full != nullptr ? a.eta_full + row : nullptr;
-    for (int32_t g0 = 0; g0 < a.n; g0 += 1024) {
-        float4 t[4];
-        int4 d[4];
-        float e[4][4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int32_t j0 = g0 + 256 * s + 4 * lane;
-            t[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-            d[s] = make_int4(0, 0, 0, 0);
-            e[s][0] = e[s][1] = e[s][2] = e[s][3] = 0.0f;
-            if (j0 < a.n) {
-                t[s] = *reinterpret_cast<const float4*>(trow + j0);
-                if (erow != nullptr) {
-                    const float4 ev = *reinterpret_cast<const float4*>(erow + j0);
-                    e[s][0] = ev.x;
-                    e[s][1] = ev.y;
-                    e[s][2] = ev.z;
-                    e[s][3] = ev.w;
-                } else {
-                    d[s] = *reinterpret_cast<const int4*>(drow + j0);
-                }
-            }
-        }
-        if (erow == nullptr) {
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int32_t dv[4] = {d[s].x, d[s].y, d[s].z, d[s].w};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    // 1 / d^beta computed (the table's value) rather than
-                    // gathered: the gathers were most of the sweep's L2 requests
-                    if (a.eta_pow)
-                        e[s][q] = (uint32_t)dv[q] < (uint32_t)a.eta_len ? eta_power_value(a.eta_pow, dv[q]) : 0.0f;
-                    else
-                        e[s][q] = eta_of(a.eta, a.eta_len, dv[q]);
-                }
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int32_t j0 = g0 + 256 * s + 4 * lane;
-            if (j0 >= a.n) continue;
-            const float tv[4] = {t[s].x, t[s].y, t[s].z, t[s].w};
-            float cv[4], ev[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool in = (j0 + q) < a.n;
-                const float x = in ? tv[q] : 0.0f;
-                acc = acc + x;
-                cv[q] = in ? apply_alpha(x, a.alpha) * e[s][q] : 0.0f;
-                ev[q] = in ? fmaxf(a.one_minus_rho * x, ACO_MIN_PHEROMONE) : 0.0f;
-            }
-            *reinterpret_cast<float4*>(crow + j0) = make_float4(cv[0], cv[1], cv[2], cv[3]);
-            *reinterpret_cast<float4*>(nrow + j0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
-        }
-        if (a.top != nullptr) {
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int32_t j0 = g0 + 256 * s + 4 * lane;
-                const float tv[4] = {t[s].x, t[s].y, t[s].z, t[s].w};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const bool in = (j0 + q) < a.n;
-                    // the weight just stored (alpha = 1: tau times eta)
-                    const float v = in ? tv[q] * e[s][q] : 0.0f;
-                    top_offer(top, v, j0 + q, in, lane);
-                }
-            }
-        }
-    }
-    if (a.top != nullptr) top_finish(a, top, i, lane);
+    return r;
+}
+
+__device__ __forceinline__ void sweep_row_end(const SweepArgs& a, int32_t i, int lane, SweepRow& r) {
+    if (a.top != nullptr) top_finish(a, r.top, i, lane);
     if (a.want_mean) {
-        const float total = readlane_f(wave_inclusive_scan(acc), 63);
+        const float total = readlane_f(wave_inclusive_scan(r.acc), 63);
         if (lane == 0) a.mean[i] = total / (float)a.n;
     }
     if (a.nn != nullptr) {
-        for (int32_t r = lane; r < a.k; r += 64) {
-            const int32_t j = a.nn[(size_t)i * a.k + r];
-            const float t = trow[j];
-            a.choice_nn[(size_t)i * a.k + r] = apply_alpha(t, a.alpha) * a.eta_nn[(size_t)i * a.k + r];
-            if (a.tau_nn != nullptr) a.tau_nn[(size_t)i * a.k + r] = t;
+        for (int32_t q = lane; q < a.k; q += 64) {
+            const int32_t j = a.nn[(size_t)i * a.k + q];
+            const float t = r.trow[j];
+            a.choice_nn[(size_t)i * a.k + q] = apply_alpha(t, a.alpha) * a.eta_nn[(size_t)i * a.k + q];
+            if (a.tau_nn != nullptr) a.tau_nn[(size_t)i * a.k + q] = t;
         }
     }
 }
 
+// Columns g0 .. g0 + 1023 of row i, any n (columns >= n are skipped): the
+// batch's sweep and the last chunk of a single colony's rows.
+__device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int32_t i, int32_t g0, int lane, SweepRow& r) {
+    // Distances from the (L2-resident) coordinates instead of the matrix
+    // row: 4n bytes of HBM reads become 8n bytes of L2 reads.
+    const bool from_xy = r.erow == nullptr && a.xy != nullptr;
+    float4 t[4];
+    int4 d[4];
+    float e[4][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int32_t j0 = g0 + 256 * s + 4 * lane;
+        t[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        d[s] = make_int4(0, 0, 0, 0);
+        e[s][0] = e[s][1] = e[s][2] = e[s][3] = 0.0f;
+        if (j0 < a.n) {
+            t[s] = *reinterpret_cast<const float4*>(r.trow + j0);
+            if (r.erow != nullptr) {
+                const float4 ev = *reinterpret_cast<const float4*>(r.erow + j0);
+                e[s][0] = ev.x;
+                e[s][1] = ev.y;
+                e[s][2] = ev.z;
+                e[s][3] = ev.w;
+            } else if (from_xy) {
+                // cities j0..j0+3 (the coordinate array is padded to a
+                // multiple of four cities)
+                const int4 p0 = *reinterpret_cast<const int4*>(a.xy + 2 * j0);
+                const int4 p1 = *reinterpret_cast<const int4*>(a.xy + 2 * j0 + 4);
+                const int32_t px[4] = {p0.x, p0.z, p1.x, p1.z}, py[4] = {p0.y, p0.w, p1.y, p1.w};
+                int32_t dv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int32_t dq = euc2d_fast(r.xi - px[q], r.yi - py[q]);
+                    dv[q] = (j0 + q == i) ? ACO_MAX_DISTANCE : dq;
+                }
+                d[s] = make_int4(dv[0], dv[1], dv[2], dv[3]);
+            } else {
+                d[s] = *reinterpret_cast<const int4*>(r.drow + j0);
+            }
+        }
+    }
+    if (r.erow == nullptr) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int32_t dv[4] = {d[s].x, d[s].y, d[s].z, d[s].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                // 1 / d^beta computed (the table's value) rather than
+                // gathered: the gathers were most of the sweep's L2 requests
+                if (a.eta_pow)
+                    e[s][q] = (uint32_t)dv[q] < (uint32_t)a.eta_len ? eta_power_value(a.eta_pow, dv[q]) : 0.0f;
+                else
+                    e[s][q] = eta_of(a.eta, a.eta_len, dv[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int32_t j0 = g0 + 256 * s + 4 * lane;
+        if (j0 >= a.n) continue;
+        const float tv[4] = {t[s].x, t[s].y, t[s].z, t[s].w};
+        float cv[4], ev[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool in = (j0 + q) < a.n;
+            const float x = in ? tv[q] : 0.0f;
+            r.acc = r.acc + x;
+            cv[q] = in ? apply_alpha(x, a.alpha) * e[s][q] : 0.0f;
+            ev[q] = in ? fmaxf(a.one_minus_rho * x, ACO_MIN_PHEROMONE) : 0.0f;
+        }
+        *reinterpret_cast<float4*>(r.crow + j0) = make_float4(cv[0], cv[1], cv[2], cv[3]);
+        *reinterpret_cast<float4*>(r.nrow + j0) = make_float4(ev[0], ev[1], ev[2], ev[3]);
+    }
+    if (a.top != nullptr) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int32_t j0 = g0 + 256 * s + 4 * lane;
+            const float tv[4] = {t[s].x, t[s].y, t[s].z, t[s].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool in = (j0 + q) < a.n;
+                // the weight just stored (alpha = 1: tau times eta)
+                const float v = in ? tv[q] * e[s][q] : 0.0f;
+                top_offer(r.top, v, j0 + q, in, lane);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf) {
+    SweepRow r = sweep_row_begin(a, i, topbuf);
+    for (int32_t g0 = 0; g0 < a.n; g0 += 1024) sweep_chunk(a, i, g0, lane, r);
+    sweep_row_end(a, i, lane, r);
+}
+
+// eta_power_value(PW, d) for d < eta_len (0 beyond: the diagonal), with the
+// reciprocal of p = d^PW (an integer in [1, 2^53)) as the compiler's IEEE
+// division computes it minus its scaling and special-case steps, which never
+// act on such p: two Newton steps from v_rcp_f64, then the final
+// residual-corrected product, rounded once to fp32.  The library checks it
+// against the host table at creation (k_eta_check) and uses the table
+// instead on any difference.
+template <int PW>
+__device__ __forceinline__ float eta_pow_fast(int32_t d, int32_t eta_len) {
+    const double dd = (double)(d > 0 ? d : 1);
+    double p = dd;
+    if (PW >= 2) p = p * dd;
+    if (PW >= 3) p = p * dd;
+    double y = __builtin_amdgcn_rcp(p);
+    double e = __builtin_fma(-p, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-p, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double res = __builtin_fma(-p, y, 1.0);
+    const float v = (float)__builtin_fma(res, y, y);
+    const float w = d == 0 ? (float)(1 << PW) : v;
+    return (uint32_t)d < (uint32_t)eta_len ? w : 0.0f;
+}
+
+// Columns g0 .. g0 + 1023 < n of row i of a single colony, without range
+// tests or branches per element: every load of the chunk is issued first;
+// the heuristic comes from the eta_full row (SRC 0), the distance matrix
+// row (SRC 1) or the coordinates (SRC 2), as eta_pow_fast<PW> (PW 1..3) or
+// a table gather (PW 0).  Same values, stores and row-sum order as
+// sweep_chunk.
+template <int SRC, int PW, bool ALPHA1>
+__device__ __forceinline__ void sweep_chunk_full(const SweepArgs& a, int32_t i, int32_t g0, int lane, SweepRow& r) {
+    // Non-temporal loads and stores for the row streams (each read or
+    // written once per sweep); the coordinates are re-read by every row and
+    // stay cached.
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    typedef int i4v __attribute__((ext_vector_type(4)));
+    float4 t[4], ef[4];
+    int4 dl[4], p0[4], p1[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int32_t j0 = g0 + 256 * s + 4 * lane;
+        const f4v tv = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(r.trow + j0));
+        t[s] = make_float4(tv.x, tv.y, tv.z, tv.w);
+        if (SRC == 0) {
+            const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(r.erow + j0));
+            ef[s] = make_float4(v.x, v.y, v.z, v.w);
+        }
+        if (SRC == 1) {
+            const i4v v = __builtin_nontemporal_load(reinterpret_cast<const i4v*>(r.drow + j0));
+            dl[s] = make_int4(v.x, v.y, v.z, v.w);
+        }
+        if (SRC == 2) {
+            p0[s] = *reinterpret_cast<const int4*>(a.xy + 2 * j0);
+            p1[s] = *reinterpret_cast<const int4*>(a.xy + 2 * j0 + 4);
+        }
+    }
+    float w[4][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int32_t j0 = g0 + 256 * s + 4 * lane;
+        const float tv[4] = {t[s].x, t[s].y, t[s].z, t[s].w};
+        float e[4];
+        if (SRC == 0) {
+            e[0] = ef[s].x;
+            e[1] = ef[s].y;
+            e[2] = ef[s].z;
+            e[3] = ef[s].w;
+        } else {
+            int32_t dv[4];
+            if (SRC == 1) {
+                dv[0] = dl[s].x;
+                dv[1] = dl[s].y;
+                dv[2] = dl[s].z;
+                dv[3] = dl[s].w;
+            } else {
+                const int32_t px[4] = {p0[s].x, p0[s].z, p1[s].x, p1[s].z};
+                const int32_t py[4] = {p0[s].y, p0[s].w, p1[s].y, p1[s].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int32_t dq = euc2d_fast(r.xi - px[q], r.yi - py[q]);
+                    dv[q] = (j0 + q == i) ? ACO_MAX_DISTANCE : dq;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                e[q] = PW > 0 ? eta_pow_fast<PW>(dv[q], a.eta_len) : eta_of(a.eta, a.eta_len, dv[q]);
+        }
+        float cv[4], ev[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float x = tv[q];
+            r.acc = r.acc + x;
+            cv[q] = (ALPHA1 ? x : pow_alpha(x, a.alpha)) * e[q];
+            ev[q] = fmaxf(a.one_minus_rho * x, ACO_MIN_PHEROMONE);
+            w[s][q] = x * e[q];
+        }
+        __builtin_nontemporal_store(f4v{cv[0], cv[1], cv[2], cv[3]}, reinterpret_cast<f4v*>(r.crow + j0));
+        __builtin_nontemporal_store(f4v{ev[0], ev[1], ev[2], ev[3]}, reinterpret_cast<f4v*>(r.nrow + j0));
+    }
+    if (a.top != nullptr) {
+        // one ballot for the chunk: offers only where some weight reaches
+        // the threshold (the kept set, the TOP_K largest keys, does not
+        // depend on the order of the offers)
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) any = any || w[s][q] >= r.top.thr;
+        if (__ballot(any)) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) top_offer(r.top, w[s][q], g0 + 256 * s + 4 * lane + q, true, lane);
+        }
+    }
+}
+
+template <int SRC, int PW, bool ALPHA1>
+__device__ __forceinline__ void sweep_row_fast(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf) {
+    SweepRow r = sweep_row_begin(a, i, topbuf);
+    const int32_t full_end = a.n & ~1023;
+    for (int32_t g0 = 0; g0 < full_end; g0 += 1024) sweep_chunk_full<SRC, PW, ALPHA1>(a, i, g0, lane, r);
+    if (full_end < a.n) sweep_chunk(a, i, full_end, lane, r);
+    sweep_row_end(a, i, lane, r);
+}
+
+template <int SRC, int PW, bool ALPHA1>
 __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
     extern __shared__ __align__(16) uint64_t topbuf[];  // TOP_BUF keys per wave when a.top is set
     const int lane = threadIdx.x & 63;
@@ -461,7 +650,16 @@ __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
         for (int32_t k = blockIdx.x * 256 + threadIdx.x; k < a.m; k += gridDim.x * 256) a.age[k] += 1;
     const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= a.n) return;
-    sweep_row(a, i, lane, topbuf + (size_t)(threadIdx.x >> 6) * TOP_BUF);
+    sweep_row_fast<SRC, PW, ALPHA1>(a, i, lane, topbuf + (size_t)(threadIdx.x >> 6) * TOP_BUF);
+}
+
+// Bit-for-bit check of eta_pow_fast<PW> against the table of
+// eta_power_value: flag = 1 on any difference.
+template <int PW>
+__global__ __launch_bounds__(256) void k_eta_check(const float* __restrict__ eta, int32_t eta_len,
+                                                   int32_t* __restrict__ flag) {
+    const int32_t d = blockIdx.x * 256 + threadIdx.x;
+    if (d < eta_len && __float_as_uint(eta_pow_fast<PW>(d, eta_len)) != __float_as_uint(eta[d])) atomicOr(flag, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2656,9 +2854,40 @@ static inline void launch_k(F kernel, dim3 grid, dim3 block, size_t lds, hipStre
         hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
 }
 
+template <int SRC, int PW>
+static void launch_sweep_t(const SweepArgs& a, size_t lds, hipStream_t s, const LaunchTiming& t) {
+    if (a.alpha == 1.0f)
+        launch_k(k_sweep<SRC, PW, true>, dim3(cdiv(a.n, 4)), dim3(256), lds, s, t.start, t.stop, a);
+    else
+        launch_k(k_sweep<SRC, PW, false>, dim3(cdiv(a.n, 4)), dim3(256), lds, s, t.start, t.stop, a);
+}
+
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t s, const LaunchTiming& t) {
     const size_t lds = a.top != nullptr ? (size_t)4 * TOP_BUF * sizeof(uint64_t) : 0;
-    launch_k(k_sweep, dim3(cdiv(a.n, 4)), dim3(256), lds, s, t.start, t.stop, a);
+    // heuristic source: eta_full row, else distances from the coordinates or
+    // the matrix, as 1 / d^eta_pow or from the table
+    if (a.eta_full != nullptr) {
+        launch_sweep_t<0, 0>(a, lds, s, t);
+    } else {
+        const bool xy = a.xy != nullptr;
+        switch (a.eta_pow) {
+            case 1: xy ? launch_sweep_t<2, 1>(a, lds, s, t) : launch_sweep_t<1, 1>(a, lds, s, t); break;
+            case 2: xy ? launch_sweep_t<2, 2>(a, lds, s, t) : launch_sweep_t<1, 2>(a, lds, s, t); break;
+            case 3: xy ? launch_sweep_t<2, 3>(a, lds, s, t) : launch_sweep_t<1, 3>(a, lds, s, t); break;
+            default: xy ? launch_sweep_t<2, 0>(a, lds, s, t) : launch_sweep_t<1, 0>(a, lds, s, t); break;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_eta_check(int32_t eta_pow, const float* eta, int32_t eta_len, int32_t* flag, hipStream_t s) {
+    const dim3 g(cdiv(eta_len, 256));
+    switch (eta_pow) {
+        case 1: hipLaunchKernelGGL(k_eta_check<1>, g, dim3(256), 0, s, eta, eta_len, flag); break;
+        case 2: hipLaunchKernelGGL(k_eta_check<2>, g, dim3(256), 0, s, eta, eta_len, flag); break;
+        case 3: hipLaunchKernelGGL(k_eta_check<3>, g, dim3(256), 0, s, eta, eta_len, flag); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -249,10 +249,12 @@ def bench_c3(args):
         built = (fb1[2] - fb0[2]) / 3
         roof = hbm_roofline("k_construct_cand", construct_bytes(n, k, built, scanned), kms[1])
         pitch = (n + 63) // 64 * 64
-        sweep_bytes = 16.0 * n * pitch
+        # HBM per element: tau read, choice and evaporated tau written (the
+        # distances come from the coordinates, L2-resident: 8 B of L2 reads)
+        sweep_bytes = 12.0 * n * pitch
         roof["sweep"] = {"bytes_per_launch": sweep_bytes, "kernel_ms": kms[0],
                          "achieved": sweep_bytes / (kms[0] * 1e-3) / 1e9, "frac": sweep_bytes / (kms[0] * 1e-3) / 1e9
-                         / PEAK_HBM_GBS}
+                         / PEAK_HBM_GBS, "bytes_per_element": "4 read + 8 written (distances from coordinates)"}
         ms_it = wall * 1e3 / (iters - warm)
         log(f"bench: c3 {ms_it:.3f} ms/iteration, kernels {kms}, fallbacks/launch {(fb1[0] - fb0[0]) / 3:.0f} "
             f"({(fb1[3] - fb0[3]) / 3:.0f} from row top lists), init {init_s:.2f} s")

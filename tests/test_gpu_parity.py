@@ -145,6 +145,35 @@ def test_as_iteration_replay_large_n(n, m, k, path, seed):
             assert listed == 0
 
 
+@pytest.mark.parametrize("beta", [1.0, 2.0, 3.0])
+@pytest.mark.parametrize("span", [(1 << 20) - 1, 1 << 20])
+def test_sweep_distances_from_extreme_coordinates(beta, span):
+    """The sweep computes its rows' distances from the coordinates when every
+    |coordinate| < 2^20 (fp32 square root, one exact fp64 correction) and
+    reads the matrix otherwise; either way choice_info, tours and pheromone
+    equal the oracle's, which uses the exact integer distances.  Coordinates
+    reach +-span (corners included) with duplicate cities; n = 1100 covers a
+    full 1024-column chunk and a tail; beta 1 and 2 use the sweep's own
+    reciprocal, beta 3 (d^3 beyond 2^53) the table."""
+    n, m, k = 1100, 8, 16
+    say(f"case: sweep distances, coordinates up to {span}, beta {beta}")
+    rng = np.random.default_rng(int(beta) * 7 + (span & 1))
+    xy = rng.integers(-span, span + 1, size=(n, 2)).astype(np.int32)
+    xy[:4] = [[-span, -span], [span, span], [-span, span], [span, -span]]
+    xy[10:14] = xy[20]  # duplicates: distance 0
+    xy[30] = xy[31] + np.array([1, 0], np.int32)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=k, beta=beta, rho=0.2, seed=77)
+    with make(p, xy=xy) as col:
+        np.testing.assert_array_equal(col.dist(), dist)
+        col.set_pheromone(random_tau(rng, n))
+        for _ in range(2):
+            snap = snapshot(col)
+            col.iterate()
+            ref = replay(p, dist, snap)
+            check_iteration(col, p, dist, snap, ref)
+
+
 @pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("n,k,m", [(2, 1, 1), (3, 2, 3), (5, 4, 7), (33, 32, 5), (65, 64, 4)])
 def test_tiny_instances_replay(n, k, m, path):
