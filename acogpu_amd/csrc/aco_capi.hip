@@ -65,6 +65,9 @@ struct aco_ctx {
     uint32_t phases_async = 0;  // bit p: phase p enqueued without waiting since the iteration began
     hipEvent_t tev[5] = {};     // timed iteration: start of its first kernel, then the stop of each phase
     int64_t* h_stats = nullptr; // coherent pinned host memory: {iteration best, best so far, valid ants}
+    int64_t* h_stamp = nullptr; // coherent pinned host memory: a timed iteration's five wall-clock stamps
+    uint64_t* d_stamp = nullptr;  // the phases' start stamps and the end stamp (device)
+    double wall_khz = 0.0;      // wall_clock64 ticks per millisecond
     char err[512] = {0};
 };
 
@@ -131,6 +134,8 @@ static void free_all(aco_ctx* c) {
     for (auto& e : c->tev)
         if (e) hipEventDestroy(e);
     if (c->h_stats) hipHostFree(c->h_stats);
+    if (c->h_stamp) hipHostFree(c->h_stamp);
+    if (c->d_stamp) hipFree(c->d_stamp);
     if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -269,6 +274,15 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     for (auto& e : c->pev) HIPB(hipEventCreate(&e));
     for (auto& e : c->tev) HIPB(hipEventCreate(&e));
     HIPB(hipHostMalloc(reinterpret_cast<void**>(&c->h_stats), 4 * sizeof(int64_t), hipHostMallocCoherent));
+    HIPB(hipHostMalloc(reinterpret_cast<void**>(&c->h_stamp), 8 * sizeof(int64_t), hipHostMallocCoherent));
+    std::memset(c->h_stamp, 0, 8 * sizeof(int64_t));
+    {
+        int dev = 0, khz = 0;
+        HIPB(hipGetDevice(&dev));
+        HIPB(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        if (khz <= 0) return bail(fail(c, ACO_E_HIP, "device reports no wall-clock rate"));
+        c->wall_khz = (double)khz;
+    }
     std::memset(c->h_stats, 0, 4 * sizeof(int64_t));
     const size_t nP = (size_t)c->n * c->P;
     TRYB(dalloc(c, &c->d_dist, nP));
@@ -300,6 +314,7 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     }
     TRYB(dalloc(c, &c->d_state, 1));
     TRYB(dalloc(c, &c->d_scratch64, 4));
+    TRYB(dalloc(c, &c->d_stamp, 8));
     HIPB(hipMemsetAsync(c->d_dist, 0, nP * sizeof(int32_t), c->stream));
     HIPB(hipMemsetAsync(c->d_tau[0], 0, nP * sizeof(float), c->stream));
     HIPB(hipMemsetAsync(c->d_tau[1], 0, nP * sizeof(float), c->stream));
@@ -492,6 +507,7 @@ static hipError_t enqueue_sweep(aco_ctx* c, int cur, const aco::LaunchTiming& t)
     sa.want_mean = c->p.aging;
     sa.eta_pow = c->sweep_eta_pow;
     sa.state = c->d_state;
+    sa.stamp = t.stamp;
     sa.top = c->d_top;
     sa.top_thr = c->d_top_thr;
     if ((e = aco::launch_sweep(sa, c->stream, t)) != hipSuccess) return e;
@@ -524,6 +540,7 @@ static hipError_t enqueue_construct(aco_ctx* c, int cur, const aco::LaunchTiming
     ca.age = c->d_age;
     ca.valid = c->d_valid;
     ca.state = c->d_state;
+    ca.stamp = t.stamp;
     ca.n = c->n;
     ca.P = c->P;
     ca.m = c->m;
@@ -560,6 +577,7 @@ static hipError_t enqueue_stats(aco_ctx* c, int cur, bool mirror, const aco::Lau
     st.dep_ant = c->d_dep_ant;
     st.dep_val = c->d_dep_val;
     st.state = c->d_state;
+    st.stamp = t.stamp;
     st.n = c->n;
     st.m = c->m;
     st.variant = c->p.variant;
@@ -587,6 +605,7 @@ static hipError_t enqueue_deposit(aco_ctx* c, int cur, const aco::LaunchTiming& 
     da.dep_ant = c->d_dep_ant;
     da.dep_val = c->d_dep_val;
     da.state = c->d_state;
+    da.stamp = t.stamp;
     da.tau_next = tau_next;
     da.n = c->n;
     da.P = c->P;
@@ -625,20 +644,23 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     return hipSuccess;
 }
 
-// One timed iteration: the four phases launched in order, the first kernel
-// recording the start and each phase's last kernel its stop from its own
-// dispatch (no marker packets), the statistics kernel writing the
-// iteration's statistics to h_stats; one wait.
+// One timed iteration: the four phases launched in order with plain
+// launches, each phase's first kernel storing its start time (wall_clock64)
+// in d_stamp[phase], a one-thread kernel the end time and all five stamps to
+// h_stamp; the statistics kernel writes the iteration's statistics to
+// h_stats; one wait.  (Launches that carry timestamp events cost about 12 us
+// per iteration more at n = 1000.)
 static hipError_t enqueue_timed_iteration(aco_ctx* c, int cur) {
     static const int order[4] = {ACO_PHASE_PROBABILITIES, ACO_PHASE_TOUR, ACO_PHASE_ANT_DISTANCE,
                                  ACO_PHASE_UPDATE_PHEROMONE};
     hipError_t e = hipSuccess;
     for (int i = 0; i < 4 && e == hipSuccess; ++i) {
         aco::LaunchTiming t;
-        t.start = i == 0 ? c->tev[0] : nullptr;
-        t.stop = c->tev[i + 1];
+        t.stamp = c->d_stamp + i;
         e = enqueue_phase(c, cur, order[i], true, t);
     }
+    if (e == hipSuccess) e = aco::launch_stamp_end(c->d_stamp, c->h_stamp, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->tev[4], c->stream);
     return e;
 }
 
@@ -670,8 +692,11 @@ extern "C" int aco_iterate_stats(aco_ctx* c, float* timers, int64_t* iter_best, 
     c->cur ^= 1;
     HIP_TRY(c, hipEventSynchronize(c->tev[4]));
     if (timers) {
-        float t[4] = {0, 0, 0, 0};
-        for (int i = 0; i < 4; ++i) HIP_TRY(c, hipEventElapsedTime(&t[i], c->tev[i], c->tev[i + 1]));
+        // phase i: from its first kernel's start to the next phase's (the
+        // last one to the end stamp)
+        const volatile int64_t* h = c->h_stamp;
+        float t[4];
+        for (int i = 0; i < 4; ++i) t[i] = (float)((double)(h[i + 1] - h[i]) / c->wall_khz);
         fill_timers(timers, t);
     }
     read_host_stats(c, iter_best, glob_best, valid_ants);

@@ -62,6 +62,7 @@ struct SweepArgs {
     float* top_thr;          // n: the TOP_K-th weight of the previous sweep
     const float* eta_full;   // pitched heuristic of every edge (read instead of dist; may be null)
     const int32_t* xy;       // 2n interleaved coordinates, |x| < XY_SWEEP_LIMIT: distances computed, not read (may be null)
+    uint64_t* stamp;         // set: block 0 writes the kernel's start (wall_clock64) here
 };
 
 // Largest coordinate magnitude (exclusive) for which the sweep computes its
@@ -93,6 +94,7 @@ struct ConstructArgs {
     int32_t young_blocks;    // trailing blocks reserved for young aging ants (staged path)
     const uint16_t* top;     // n x TOP_K row top lists (null: none)
     const float* eta_full;   // pitched heuristic of every edge (aging: young ants' fallback rows; null: gathered)
+    uint64_t* stamp;         // set: block 0 writes the kernel's start (wall_clock64) here
 };
 
 struct StatsArgs {
@@ -112,6 +114,7 @@ struct StatsArgs {
     // (coherent pinned) memory after the statistics, so the host reads them
     // after the iteration with no copy
     int64_t* host_stats;
+    uint64_t* stamp;         // set: block 0 writes the kernel's start (wall_clock64) here
 };
 
 struct DepositArgs {
@@ -127,6 +130,7 @@ struct DepositArgs {
     // predecessor (2a+1) of x.
     uint16_t* adj;
     int32_t m;
+    uint64_t* stamp;         // set: block 0 writes the kernel's start (wall_clock64) here
 };
 
 // Heuristic eta^beta of an integer distance d.  beta = 1, 2 or 3 with
@@ -251,6 +255,9 @@ int32_t batch_construct_blocks_per_cu(int32_t nmax, int32_t ants_per_block, int3
                                       size_t lds_bytes);
 
 hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s);
+// End stamp of a timed iteration: stamp[4] = now, then stamp[0..4] copied to
+// host (coherent host memory, system fence).
+hipError_t launch_stamp_end(uint64_t* stamp, int64_t* host, hipStream_t s);
 // *flag |= 1 unless the sweep's computed heuristic 1 / d^eta_pow equals the
 // table eta[0 .. eta_len) bit for bit.
 hipError_t launch_eta_check(int32_t eta_pow, const float* eta, int32_t eta_len, int32_t* flag, hipStream_t s);
@@ -272,7 +279,14 @@ hipError_t launch_greedy(const int32_t* dist, int32_t n, int32_t P, int32_t star
 struct LaunchTiming {
     hipEvent_t start = nullptr;
     hipEvent_t stop = nullptr;
+    uint64_t* stamp = nullptr;  // the phase's first kernel stores its start time (wall_clock64) here
 };
+
+// Start stamp of a phase's first kernel: block 0's first thread, before any
+// early exit (a plain store; the next kernel on the stream sees it).
+__device__ __forceinline__ void stamp_start(uint64_t* stamp) {
+    if (stamp != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *stamp = wall_clock64();
+}
 
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());
 // Full-row construction: chunks of 64 candidates whose list entries and

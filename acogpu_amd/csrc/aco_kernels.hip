@@ -637,6 +637,7 @@ template <int SRC, int PW, bool ALPHA1>
 __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
     extern __shared__ __align__(16) uint64_t topbuf[];  // TOP_BUF keys per wave when a.top is set
     const int lane = threadIdx.x & 63;
+    stamp_start(a.stamp);
     // A halted run skips whole iterations: the sweep, first kernel of an
     // iteration, publishes the decision for the other three.
     if (a.state != nullptr) {
@@ -984,6 +985,7 @@ __device__ __forceinline__ void construct_cand_ant(const ConstructArgs& a, int32
 
 __global__ __launch_bounds__(512) void k_construct_cand(const ConstructArgs a) {
     extern __shared__ __align__(16) uint32_t lds[];
+    stamp_start(a.stamp);
     if (a.state->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -1347,6 +1349,7 @@ __device__ __forceinline__ void stage_young(float* wl, const ConstructArgs& a, f
 __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a) {
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ int32_t s_ant;
+    stamp_start(a.stamp);
     if (a.state->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -1472,6 +1475,7 @@ __device__ __forceinline__ void stage_weight_row(const ConstructArgs& a, int32_t
 
 __global__ __launch_bounds__(512) void k_construct_full(const ConstructArgs a) {
     extern __shared__ __align__(16) uint32_t lds[];
+    stamp_start(a.stamp);
     if (a.state->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -1950,6 +1954,7 @@ __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
     extern __shared__ uint32_t taken[];  // RBAS: ants already ranked
     __shared__ LenKey red_rank[16 * RANK_MAX];
     __shared__ StatsShared sh;
+    stamp_start(a.stamp);
     if (a.state->skip) return;
     // the ants that built a tour this iteration: the valid ones (validity
     // changes only below, after the deposit list is made)
@@ -1986,6 +1991,7 @@ __device__ __forceinline__ void deposit_entry(const DepositArgs& a, int32_t e) {
 }
 
 __global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
+    stamp_start(a.stamp);
     if (a.state->skip) return;
     deposit_entry(a, blockIdx.x);
 }
@@ -2145,6 +2151,7 @@ __device__ __forceinline__ void adj_apply_row(const DepositArgs& a, int32_t x, f
 // rows band X .. band X + X - 1.  16 waves: one or two tours per wave.
 __global__ __launch_bounds__(1024) void k_adj_scatter_staged(const DepositArgs a, int32_t A, int32_t X) {
     extern __shared__ __align__(16) uint16_t adj_stage[];
+    stamp_start(a.stamp);
     if (a.state->skip) return;
     const int32_t groups = (a.m + A - 1) / A;
     const int32_t g = blockIdx.x % groups, band = blockIdx.x / groups;
@@ -2803,6 +2810,20 @@ __global__ __launch_bounds__(256) void kb_deposit(const BatchBuffers b) {
 // ---------------------------------------------------------------------------
 // Launchers.
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+__global__ __launch_bounds__(64) void k_stamp_end(uint64_t* stamp, int64_t* host) {
+    if (threadIdx.x == 0) {
+        stamp[4] = wall_clock64();
+        volatile int64_t* h = host;
+        for (int i = 0; i < 5; ++i) h[i] = (int64_t)stamp[i];
+        __threadfence_system();
+    }
+}
+
+hipError_t launch_stamp_end(uint64_t* stamp, int64_t* host, hipStream_t s) {
+    hipLaunchKernelGGL(k_stamp_end, dim3(1), dim3(64), 0, s, stamp, host);
+    return hipGetLastError();
+}
 
 hipError_t launch_dist_from_xy(const int32_t* xy, int32_t n, int32_t P, int32_t* dist, hipStream_t s) {
     hipLaunchKernelGGL(k_dist_from_xy, dim3(cdiv(P, 256), n), dim3(256), 0, s, xy, n, P, dist);

@@ -103,8 +103,9 @@ int aco_initialize(aco_ctx *c);
 
 /* Colony::constructSolution (src/Colony.cu:537-594): one iteration, phase
  * times in milliseconds (timers may be NULL), each from the start of the
- * phase's first kernel (after the previous phase) to the stop of its last,
- * recorded by the kernels' own dispatches; one wait. */
+ * phase's first kernel to the start of the next phase's (the last phase: to
+ * the end of the iteration), from wall-clock stamps the kernels store
+ * themselves; one wait. */
 int aco_iterate(aco_ctx *c, float *timers);
 
 /* aco_iterate plus what Colony::constructSolution's caller reads next:
