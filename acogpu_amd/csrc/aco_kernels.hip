@@ -758,7 +758,7 @@ __device__ __forceinline__ int32_t fallback_argmax(const ConstructArgs& a, const
         const uint16_t* tl = a.top + (size_t)c * TOP_K;
         int32_t e[TOP_K / 64];
 #pragma unroll
-        for (int q = 0; q < TOP_K / 64; ++q) e[q] = tl[64 * q + lane];
+        for (int q = 0; q < TOP_K / 64; ++q) e[q] = __builtin_nontemporal_load(tl + 64 * q + lane);
         if (readlane_i(e[0], 0) != (int32_t)TOP_INVALID) {
 #pragma unroll
             for (int q = 0; q < TOP_K / 64; ++q) {
@@ -977,7 +977,9 @@ __device__ __forceinline__ void construct_cand_ant(const ConstructArgs& a, int32
             tbuf = (lane == s - s0) ? next : tbuf;
             c = next;
         }
-        if (lane < s_end - s0) tour[s0 + lane] = tbuf;
+        // non-temporal: the tours (m x n x 4 bytes per iteration) would
+        // otherwise stream through L2 past the candidate rows every step reads
+        if (lane < s_end - s0) __builtin_nontemporal_store(tbuf, tour + s0 + lane);
     }
     finish_counters(a, fb, scanned, lane, listed);
     store_tour_length(a, tour, ant, lane);
