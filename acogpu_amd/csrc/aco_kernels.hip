@@ -1011,51 +1011,56 @@ __global__ __launch_bounds__(512) void k_construct_cand(const ConstructArgs a) {
 // blocks stage choice_nn and run W ants each; the trailing young_blocks
 // blocks each take one aging ant younger than min_age after convergence and
 // stage that ant's pulled-toward-the-mean weights instead (they differ per
-// age), so young ants also run at LDS speed.  Tours are kept in LDS (16-bit)
-// and written, with their exact lengths, after the last step.
+// age), so young ants also run at LDS speed.  Each 64-step chunk of a tour
+// is written when it ends, and the lengths of its edges are loaded then and
+// summed a chunk later (exact int64 tour lengths).
 
-// Argmax fallback for the register bitmap layout: lane l owns cities
-// 32l .. 32l+31 (its bitmap word), reads them as eight 16-byte loads and
-// keeps the largest weight (lowest index on ties).
-__device__ __forceinline__ int32_t fallback_argmax_vreg(const ConstructArgs& a, uint32_t vword, int32_t c, bool young,
-                                                        float ratio, int lane, unsigned long long& scanned) {
+// Argmax fallback with the ant's bitmap in registers (lane src + w holds word
+// w), spread over the whole wave: lane l reads cities CPL l .. CPL l + CPL - 1
+// (CPL / 4 16-byte loads per row; n <= 64 CPL; rows are padded to a multiple
+// of 64) and takes their bits from the owner lane with one bpermute.  The
+// weight is choice, or for a young aging ant the pulled-toward-the-mean
+// pheromone times the heuristic (per-edge heuristic rows when the colony
+// keeps them, else the distance row and table gathers).  Each lane keeps
+// the largest weight of its unvisited cities (strictly greater in ascending
+// order: lowest index on ties, a NaN weight never wins) and the wave takes
+// the largest (weight, lowest index) key: the same result as the sequential
+// scan.  Call with every lane active.
+template <int CPL>
+__device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a, uint32_t vreg, int src, int32_t c,
+                                                          bool young, float ratio, int lane,
+                                                          unsigned long long& scanned) {
+    const int32_t j0 = CPL * lane;
+    const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((src + (j0 >> 5)) << 2, (int)vreg);
+    uint32_t free_bits = 0u;
     float best = -1.0f;
     int32_t bj = 0x7fffffff;
-    const int32_t j0 = 32 * lane;
     if (j0 < a.n) {
-        uint32_t free_bits = ~vword;
-        if (a.n - j0 < 32) free_bits &= (1u << (a.n - j0)) - 1u;
+        constexpr uint32_t FULL = CPL == 32 ? 0xffffffffu : ((1u << CPL) - 1u);
+        free_bits = ~(word >> (j0 & 31)) & FULL;
+        if (a.n - j0 < CPL) free_bits &= (1u << (a.n - j0)) - 1u;
         scanned += __popc(free_bits);
         const size_t row = (size_t)c * a.P + j0;
+        float v[CPL];
         if (!young) {
-            float x[32];
-            const float4* src = reinterpret_cast<const float4*>(a.choice + row);
+            const float4* src4 = reinterpret_cast<const float4*>(a.choice + row);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const float4 v = src[q];
-                x[4 * q] = v.x;
-                x[4 * q + 1] = v.y;
-                x[4 * q + 2] = v.z;
-                x[4 * q + 3] = v.w;
-            }
-#pragma unroll
-            for (int q = 0; q < 32; ++q) {
-                if (((free_bits >> q) & 1u) && x[q] > best) {
-                    best = x[q];
-                    bj = j0 + q;
-                }
+            for (int q = 0; q < CPL / 4; ++q) {
+                const float4 t = src4[q];
+                v[4 * q] = t.x;
+                v[4 * q + 1] = t.y;
+                v[4 * q + 2] = t.z;
+                v[4 * q + 3] = t.w;
             }
         } else {
-            // all loads first (16-byte tau and heuristic rows, or tau and
-            // dist followed by the heuristic gathers), then the
-            // pulled-toward-the-mean weights
+            // all loads first, then the weights
             const float mc = a.mean[c];
-            float t[32], e[32];
+            float t[CPL], e[CPL];
             const float4* t4 = reinterpret_cast<const float4*>(a.tau + row);
-            if (a.eta_full != nullptr) {  // one round trip
+            if (a.eta_full != nullptr) {
                 const float4* e4 = reinterpret_cast<const float4*>(a.eta_full + row);
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
+                for (int q = 0; q < CPL / 4; ++q) {
                     const float4 tv = t4[q];
                     const float4 ev = e4[q];
                     t[4 * q] = tv.x;
@@ -1068,10 +1073,10 @@ __device__ __forceinline__ int32_t fallback_argmax_vreg(const ConstructArgs& a, 
                     e[4 * q + 3] = ev.w;
                 }
             } else {
-                int32_t d[32];
+                int32_t d[CPL];
                 const int4* d4 = reinterpret_cast<const int4*>(a.dist + row);
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
+                for (int q = 0; q < CPL / 4; ++q) {
                     const float4 tv = t4[q];
                     const int4 dv = d4[q];
                     t[4 * q] = tv.x;
@@ -1084,69 +1089,21 @@ __device__ __forceinline__ int32_t fallback_argmax_vreg(const ConstructArgs& a, 
                     d[4 * q + 3] = dv.w;
                 }
 #pragma unroll
-                for (int q = 0; q < 32; ++q) e[q] = eta_of(a.eta, a.eta_len, d[q]);
+                for (int q = 0; q < CPL; ++q) e[q] = eta_of(a.eta, a.eta_len, d[q]);
             }
 #pragma unroll
-            for (int q = 0; q < 32; ++q) {
-                if ((free_bits >> q) & 1u) {
-                    const float x = young_formula(t[q], mc, ratio, e[q], a.alpha);
-                    if (x > best) {
-                        best = x;
-                        bj = j0 + q;
-                    }
-                }
+            for (int q = 0; q < CPL; ++q)
+                v[q] = ((free_bits >> q) & 1u) ? young_formula(t[q], mc, ratio, e[q], a.alpha) : -1.0f;
+        }
+        // ascending cities, strictly greater: the lowest index among equal
+        // maxima, and a NaN weight never wins
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            if (((free_bits >> q) & 1u) && v[q] > best) {
+                best = v[q];
+                bj = j0 + q;
             }
         }
-    }
-    int32_t next = wave_argmax_index(best, bj);
-    if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
-        uint32_t free_bits = (32 * lane < a.n) ? ~vword : 0u;
-        if (32 * lane < a.n && a.n - 32 * lane < 32) free_bits &= (1u << (a.n - 32 * lane)) - 1u;
-        const unsigned long long m = __ballot(free_bits != 0u);
-        const int l = __ffsll((long long)m) - 1;
-        next = 32 * l + __ffs(readlane_i((int)free_bits, l)) - 1;
-    }
-    return next;
-}
-
-// Argmax fallback with the ant's bitmap in registers (lane src + w holds word
-// w), spread over the whole wave: lane l reads cities CPL l .. CPL l + CPL - 1
-// (CPL / 4 16-byte loads; n <= 64 CPL) and takes their bits from the owner
-// lane with one bpermute.  Visited cities become -1, the lane's maximum
-// comes from fmaxf (maxNum: a NaN weight never wins, as in the sequential
-// scan), and the lowest index holding it is the lane's candidate: the same
-// result as the sequential scan.  Call with every lane active.
-template <int CPL>
-__device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a, uint32_t vreg, int src, int32_t c,
-                                                          int lane, unsigned long long& scanned) {
-    const int32_t j0 = CPL * lane;
-    const uint32_t word = (uint32_t)__builtin_amdgcn_ds_bpermute((src + (j0 >> 5)) << 2, (int)vreg);
-    uint32_t free_bits = 0u;
-    float best = -1.0f;
-    int32_t bj = 0x7fffffff;
-    if (j0 < a.n) {
-        constexpr uint32_t FULL = CPL == 32 ? 0xffffffffu : ((1u << CPL) - 1u);
-        free_bits = ~(word >> (j0 & 31)) & FULL;
-        if (a.n - j0 < CPL) free_bits &= (1u << (a.n - j0)) - 1u;
-        scanned += __popc(free_bits);
-        float v[CPL];
-        const float4* src4 = reinterpret_cast<const float4*>(a.choice + (size_t)c * a.P + j0);
-#pragma unroll
-        for (int q = 0; q < CPL / 4; ++q) {
-            const float4 t = src4[q];
-            v[4 * q] = t.x;
-            v[4 * q + 1] = t.y;
-            v[4 * q + 2] = t.z;
-            v[4 * q + 3] = t.w;
-        }
-#pragma unroll
-        for (int q = 0; q < CPL; ++q) v[q] = ((free_bits >> q) & 1u) ? v[q] : -1.0f;
-        float m = v[0];
-#pragma unroll
-        for (int q = 1; q < CPL; ++q) m = fmaxf(m, v[q]);
-#pragma unroll
-        for (int q = CPL - 1; q >= 0; --q) bj = (v[q] == m) ? j0 + q : bj;
-        best = m;
     }
     int32_t next = wave_argmax_index(best, bj);
     if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
@@ -1157,6 +1114,18 @@ __device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a
     return next;
 }
 
+// The register-bitmap fallback at the narrowest spread that covers the row
+// (n <= 64 MAXCPL; wave-uniform).
+template <int MAXCPL = 32>
+__device__ __forceinline__ int32_t fallback_argmax_regs(const ConstructArgs& a, uint32_t vreg, int src, int32_t c,
+                                                        bool young, float ratio, int lane,
+                                                        unsigned long long& scanned) {
+    if (MAXCPL == 8 || a.n <= 512) return fallback_argmax_spread<8>(a, vreg, src, c, young, ratio, lane, scanned);
+    if (MAXCPL == 16 || a.n <= 1024)
+        return fallback_argmax_spread<16>(a, vreg, src, c, young, ratio, lane, scanned);
+    return fallback_argmax_spread<32>(a, vreg, src, c, young, ratio, lane, scanned);
+}
+
 // One ant on LDS-staged candidate data.  VREG_VIS: the visited set lives in
 // one VGPR per lane (lane l holds cities 32l..32l+31; n <= 2048) and is read
 // across lanes with ds_bpermute; otherwise an LDS bitmap.  K32: k <= 32, the
@@ -1165,7 +1134,7 @@ __device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a
 template <bool VREG_VIS, bool K32>
 __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int32_t ant, bool young, float ratio,
                                                      const float* wl, const uint16_t* nl, uint32_t* vis_lds,
-                                                     uint16_t* t16, uint32_t iter, int lane) {
+                                                     uint32_t iter, int lane) {
     uint32_t vreg = 0u;
     if (!VREG_VIS)
         for (int32_t w = lane; w < a.words; w += 64) vis_lds[w] = 0u;
@@ -1196,6 +1165,10 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
     // steps in chunks of 64: a chunk's random words sit one per lane in
     // rng.cur and its cities one per lane in tbuf, so the step itself tests
     // neither the refill nor the tour flush
+    int32_t* tour = a.tours + (size_t)ant * a.n;
+    int32_t prev_last = start;  // the city before the chunk's first position
+    int32_t dpend = 0;          // the previous chunk's edge lengths, loads in flight
+    int64_t len = 0;
     for (int32_t s0 = 0; s0 < a.n; s0 += 64) {
         if (s0 > 0) rng_step(rng, s0, (uint32_t)ant, iter, a, lane);
         const float uc = word_to_uniform(rng.cur);  // the chunk's uniforms, one per lane
@@ -1222,7 +1195,7 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             const int32_t next = roulette_pick<K32>(w, j, u, [&]() {
                 ++fb;
                 if (!VREG_VIS) return fallback_argmax(a, vis_lds, c, young, ratio, lane, scanned, listed);
-                return fallback_argmax_vreg(a, vreg, c, young, ratio, lane, scanned);
+                return fallback_argmax_regs(a, vreg, 0, c, young, ratio, lane, scanned);
             });
             const int32_t jn = *reinterpret_cast<const uint16_t*>(nlb + __umul24((uint32_t)next, kk2));
             const float cwn = *reinterpret_cast<const float*>(wlb + __umul24((uint32_t)next, kk4));
@@ -1231,18 +1204,22 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
             j = jn;
             cw = cwn;
         }
-        if (lane < s_end - s0) t16[s0 + lane] = (uint16_t)tbuf;
+        // The chunk's tour entries and the lengths of its edges (position
+        // p - 1 -> p, one per lane) are requested now and summed at the end
+        // of the next chunk, so the loads complete during its steps instead
+        // of one memory round trip per 64 edges after the last step.
+        const int32_t cnt = s_end - s0;
+        int32_t pred = __shfl_up(tbuf, 1);
+        pred = lane == 0 ? prev_last : pred;
+        len += dpend;
+        dpend = (lane < cnt && s0 + lane > 0) ? a.dist[(size_t)pred * a.P + tbuf] : 0;
+        if (lane < cnt) __builtin_nontemporal_store(tbuf, tour + s0 + lane);
+        prev_last = readlane_i(tbuf, cnt - 1);
     }
     finish_counters(a, fb, scanned, lane, listed);
-    // tour out (coalesced) and exact length, edge i -> i+1 per lane
-    int32_t* tour = a.tours + (size_t)ant * a.n;
-    int64_t len = 0;
-    for (int32_t i = lane; i < a.n; i += 64) {
-        const int32_t x = t16[i];
-        const int32_t y = t16[(i + 1 == a.n) ? 0 : i + 1];
-        tour[i] = x;
-        len += a.dist[(size_t)x * a.P + y];
-    }
+    // the closing edge, last city -> start
+    len += dpend;
+    if (lane == 0) len += a.dist[(size_t)prev_last * a.P + start];
     for (int off = 32; off >= 1; off >>= 1) {
         const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)len, off);
         const uint32_t hi = __shfl_xor((uint32_t)((uint64_t)len >> 32), off);
@@ -1254,17 +1231,17 @@ __device__ __forceinline__ void construct_staged_ant(const ConstructArgs& a, int
 // Dispatch on the register-bitmap and 32-lane cases (wave-uniform).
 __device__ __forceinline__ void construct_staged_dispatch(const ConstructArgs& a, int32_t ant, bool young,
                                                           float ratio, const float* wl, const uint16_t* nl,
-                                                          uint32_t* vis, uint16_t* t16, uint32_t iter, int lane) {
+                                                          uint32_t* vis, uint32_t iter, int lane) {
     if (a.n <= 2048) {
         if (a.k <= 32)
-            construct_staged_ant<true, true>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+            construct_staged_ant<true, true>(a, ant, young, ratio, wl, nl, vis, iter, lane);
         else
-            construct_staged_ant<true, false>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+            construct_staged_ant<true, false>(a, ant, young, ratio, wl, nl, vis, iter, lane);
     } else {
         if (a.k <= 32)
-            construct_staged_ant<false, true>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+            construct_staged_ant<false, true>(a, ant, young, ratio, wl, nl, vis, iter, lane);
         else
-            construct_staged_ant<false, false>(a, ant, young, ratio, wl, nl, vis, t16, iter, lane);
+            construct_staged_ant<false, false>(a, ant, young, ratio, wl, nl, vis, iter, lane);
     }
 }
 
@@ -1311,16 +1288,44 @@ __device__ __forceinline__ void stage_vec(V* d4, const V* __restrict__ s4, int32
     for (; x < v; x += stride) d4[x] = s4[x];
 }
 
-__device__ __forceinline__ void stage_f32(float* dst, const float* __restrict__ src, int32_t count) {
-    const int32_t v = count / 4;
-    stage_vec(reinterpret_cast<float4*>(dst), reinterpret_cast<const float4*>(src), v);
-    for (int32_t x = 4 * v + threadIdx.x; x < count; x += blockDim.x) dst[x] = src[x];
-}
-
 __device__ __forceinline__ void stage_u16(uint16_t* dst, const uint16_t* __restrict__ src, int32_t count) {
     const int32_t v = count / 8;
     stage_vec(reinterpret_cast<uint4*>(dst), reinterpret_cast<const uint4*>(src), v);
     for (int32_t x = 8 * v + threadIdx.x; x < count; x += blockDim.x) dst[x] = src[x];
+}
+
+// The candidate lists (count uint16) and their weights (count fp32) in one
+// pass: up to 16 loads of 16 bytes per thread in flight, from both tables,
+// before the first LDS store, so a block of 512 threads stages an n = 1000,
+// k = 20 instance (120 KB) in one memory round trip instead of one per four
+// loads per thread and table.
+__device__ __forceinline__ void stage_lists_weights(uint16_t* nl, const uint16_t* __restrict__ nn16, float* wl,
+                                                    const float* __restrict__ w, int32_t count) {
+    constexpr int U = 16;
+    const int32_t v1 = count / 8, v2 = count / 4, v = v1 + v2;
+    const uint4* s1 = reinterpret_cast<const uint4*>(nn16);
+    const uint4* s2 = reinterpret_cast<const uint4*>(w);
+    uint4* d1 = reinterpret_cast<uint4*>(nl);
+    uint4* d2 = reinterpret_cast<uint4*>(wl);
+    const int32_t stride = blockDim.x;
+    for (int32_t x0 = threadIdx.x; x0 < v; x0 += U * stride) {
+        uint4 t[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {  // indices past the end reload the last vector
+            const int32_t x = min(x0 + q * stride, v - 1);
+            t[q] = *(x < v1 ? s1 + x : s2 + (x - v1));
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int32_t x = x0 + q * stride;
+            if (x < v1)
+                d1[x] = t[q];
+            else if (x < v)
+                d2[x - v1] = t[q];
+        }
+    }
+    for (int32_t x = 8 * v1 + threadIdx.x; x < count; x += stride) nl[x] = nn16[x];
+    for (int32_t x = 4 * v2 + threadIdx.x; x < count; x += stride) wl[x] = w[x];
 }
 
 // Young-ant weights for the whole instance, four candidates per thread per
@@ -1360,54 +1365,50 @@ __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a)
     const StagedLayout L = staged_layout(lds, nk);
     float* wl = L.wl;
     uint16_t* nl = L.nl;
-    uint32_t* per = L.per;
-    const int32_t per_words = a.words + (a.n + 1) / 2;
-    uint32_t* vis = per + (size_t)wave * per_words;
-    uint16_t* t16 = reinterpret_cast<uint16_t*>(vis + a.words);
+    uint32_t* vis = L.per + (size_t)wave * a.words;
     const uint32_t iter = a.state->iter;
     const int32_t converged = a.state->converged;
     const int32_t normal_blocks = gridDim.x - a.young_blocks;
     const bool young_block = (int32_t)blockIdx.x >= normal_blocks;
     if (young_block && (!a.aging || !converged)) return;
 
+    // Normal blocks run one round (W ants).
+    if (!young_block) {
+        stage_lists_weights(nl, a.nn16, wl, a.choice_nn, nk);
+        staged_zero_entry(L, nk);
+        __syncthreads();
+        const int32_t ant = blockIdx.x * nwaves + wave;
+        if (ant < a.m && !(a.aging && (!a.valid[ant] || ant_is_young(a, ant, converged))))
+            construct_staged_dispatch(a, ant, false, 0.0f, wl, nl, vis, iter, lane);
+        return;
+    }
+
+    // Young block y runs the young ants y, y + young_blocks, ... one per
+    // round on wave 0.
     stage_u16(nl, a.nn16, nk);
     staged_zero_entry(L, nk);
-
-    // Normal blocks run one round (W ants); young block y runs the young
-    // ants y, y + young_blocks, ... one per round on wave 0.
     for (int32_t y = blockIdx.x - normal_blocks;; y += a.young_blocks) {
-        int32_t ant;
-        float ratio = 0.0f;
-        bool active;
-        if (!young_block) {
-            ant = blockIdx.x * nwaves + wave;
-            stage_f32(wl, a.choice_nn, nk);
-            active = ant < a.m && !(a.aging && (!a.valid[ant] || ant_is_young(a, ant, converged)));
-        } else {
-            if (wave == 0) {
-                int32_t found = -1, seen = 0;
-                for (int32_t a0 = 0; a0 < a.m && found < 0; a0 += 64) {
-                    const int32_t k = a0 + lane;
-                    unsigned long long mk = __ballot(k < a.m && ant_is_young(a, k, converged));
-                    const int cnt = __popcll(mk);
-                    if (seen + cnt > y) {
-                        for (int t = 0; t < y - seen; ++t) mk &= mk - 1;
-                        found = a0 + __ffsll((long long)mk) - 1;
-                    }
-                    seen += cnt;
+        if (wave == 0) {
+            int32_t found = -1, seen = 0;
+            for (int32_t a0 = 0; a0 < a.m && found < 0; a0 += 64) {
+                const int32_t k = a0 + lane;
+                unsigned long long mk = __ballot(k < a.m && ant_is_young(a, k, converged));
+                const int cnt = __popcll(mk);
+                if (seen + cnt > y) {
+                    for (int t = 0; t < y - seen; ++t) mk &= mk - 1;
+                    found = a0 + __ffsll((long long)mk) - 1;
                 }
-                if (lane == 0) s_ant = found;
+                seen += cnt;
             }
-            __syncthreads();
-            ant = s_ant;
-            if (ant < 0) return;
-            ratio = (float)a.age[ant] / (float)a.min_age;
-            stage_young(wl, a, ratio, nk);
-            active = wave == 0;
+            if (lane == 0) s_ant = found;
         }
         __syncthreads();
-        if (active) construct_staged_dispatch(a, ant, young_block, ratio, wl, nl, vis, t16, iter, lane);
-        if (!young_block) return;
+        const int32_t ant = s_ant;
+        if (ant < 0) return;
+        const float ratio = (float)a.age[ant] / (float)a.min_age;
+        stage_young(wl, a, ratio, nk);
+        __syncthreads();
+        if (wave == 0) construct_staged_dispatch(a, ant, true, ratio, wl, nl, vis, iter, lane);
         __syncthreads();
     }
 }
@@ -2369,18 +2370,16 @@ __device__ __forceinline__ void kb_construct_body(const BatchBuffers& b, uint32_
         const StagedLayout L = staged_layout(lds, nk);
         float* wl = L.wl;
         uint16_t* nl = L.nl;
-        uint32_t* vis = L.per + (size_t)wave * (im.words + (im.n + 1) / 2);
-        uint16_t* t16 = reinterpret_cast<uint16_t*>(vis + im.words);
-        stage_u16(nl, a.nn16, nk);
-        stage_f32(wl, a.choice_nn, nk);
+        uint32_t* vis = L.per + (size_t)wave * im.words;
+        stage_lists_weights(nl, a.nn16, wl, a.choice_nn, nk);
         staged_zero_entry(L, nk);
         __syncthreads();
         const int32_t ant = it.y + wave;
         if (ant < im.m) {
             if (a.k <= 32)
-                construct_staged_ant<VREG_VIS, true>(a, ant, false, 0.0f, wl, nl, vis, t16, a.state->iter, lane);
+                construct_staged_ant<VREG_VIS, true>(a, ant, false, 0.0f, wl, nl, vis, a.state->iter, lane);
             else
-                construct_staged_ant<VREG_VIS, false>(a, ant, false, 0.0f, wl, nl, vis, t16, a.state->iter, lane);
+                construct_staged_ant<VREG_VIS, false>(a, ant, false, 0.0f, wl, nl, vis, a.state->iter, lane);
         }
         __syncthreads();
     }
@@ -2547,12 +2546,9 @@ __device__ __forceinline__ void construct_group_wave(const ConstructArgs& a, int
                         const bool vg = ant0 + gg < a.m;
                         fb += vg ? 1 : 0;
                         unsigned long long sc = 0;
-                        if constexpr (G == 4) {  // eight cities per lane over the whole wave
-                            ng = fallback_argmax_spread<8>(a, vreg, gg * L, readlane_i(c, gg * L), lane, sc);
-                        } else {  // 32 cities on each of the group's lanes' words
-                            const uint32_t vword = (uint32_t)bperm_i((gg * L + lane) & 63, (int32_t)vreg);
-                            ng = fallback_argmax_vreg(a, vword, readlane_i(c, gg * L), false, 0.0f, lane, sc);
-                        }
+                        // the group's row over the whole wave (n <= 1024 for G = 2, 512 for G = 4)
+                        ng = fallback_argmax_regs<G == 4 ? 8 : 16>(a, vreg, gg * L, readlane_i(c, gg * L), false,
+                                                                   0.0f, lane, sc);
                         scanned += vg ? sc : 0ull;
                     }
                     next = (g == gg) ? ng : next;
@@ -2916,8 +2912,7 @@ hipError_t launch_eta_check(int32_t eta_pow, const float* eta, int32_t eta_len, 
 
 size_t staged_lds_bytes(int32_t n, int32_t k, int32_t words, int32_t ants_per_block) {
     const size_t nk = (size_t)n * k;
-    return ((nk + 4) & ~(size_t)3) * 4 + ((nk + 8) & ~(size_t)7) * 2 +
-           (size_t)ants_per_block * (words + (n + 1) / 2) * 4;
+    return ((nk + 4) & ~(size_t)3) * 4 + ((nk + 8) & ~(size_t)7) * 2 + (size_t)ants_per_block * words * 4;
 }
 
 hipError_t launch_construct(const ConstructArgs& a, int32_t path, int32_t ants_per_block, hipStream_t s,

@@ -237,7 +237,7 @@ def test_tune_k_picks_fastest_k_within_tolerance():
     k, rep = capi.tune_k(p, sizes, xys, ks, trial_iterations=8, tolerance=0.01)
     say(f"chosen k={k}; report {rep.tolist()}")
     assert k in ks
-    assert (rep[:, 0] > 0).all() and (rep[:, 1] > 0).all() and set(rep[:, 2].tolist()) <= {1.0, 2.0, 4.0, 8.0}
+    assert (rep[:, 0] > 0).all() and (rep[:, 1] > 0).all() and set(rep[:, 2].tolist()) <= {1.0, 2.0, 4.0, 8.0, 16.0}
     ok = rep[:, 1] <= rep[:, 1].min() * 1.01 * (1 + 1e-6)
     assert ok[ks.index(k)]
     assert rep[ks.index(k), 0] <= rep[ok, 0].min() * (1 + 1e-6)
