@@ -1091,9 +1091,14 @@ __device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a
 #pragma unroll
                 for (int q = 0; q < CPL; ++q) e[q] = eta_of(a.eta, a.eta_len, d[q]);
             }
+            if (a.alpha == 1.0f) {  // every city's weight without branches, the visited ones masked below
 #pragma unroll
-            for (int q = 0; q < CPL; ++q)
-                v[q] = ((free_bits >> q) & 1u) ? young_formula(t[q], mc, ratio, e[q], a.alpha) : -1.0f;
+                for (int q = 0; q < CPL; ++q) v[q] = young_formula(t[q], mc, ratio, e[q], 1.0f);
+            } else {
+#pragma unroll
+                for (int q = 0; q < CPL; ++q)
+                    v[q] = ((free_bits >> q) & 1u) ? young_formula(t[q], mc, ratio, e[q], a.alpha) : -1.0f;
+            }
         }
         // ascending cities, strictly greater: the lowest index among equal
         // maxima, and a NaN weight never wins
@@ -1328,29 +1333,75 @@ __device__ __forceinline__ void stage_lists_weights(uint16_t* nl, const uint16_t
     for (int32_t x = 4 * v2 + threadIdx.x; x < count; x += stride) wl[x] = w[x];
 }
 
-// Young-ant weights for the whole instance, four candidates per thread per
-// pass (16-byte tau and eta loads).
+// Young-ant weights for the whole instance, four candidates per 16-byte
+// vector.  A thread first issues the loads of U vectors (tau and heuristic)
+// and of their rows' means, then computes and stores them: one memory round
+// trip per U vectors per thread instead of one per mean, which kept the
+// young ant's block from starting its tour for tens of microseconds.
 __device__ __forceinline__ void stage_young(float* wl, const ConstructArgs& a, float ratio, int32_t nk) {
+    constexpr int U = 8;
     const int32_t v = nk / 4;
     const float4* t4 = reinterpret_cast<const float4*>(a.tau_nn);
     const float4* e4 = reinterpret_cast<const float4*>(a.eta_nn);
     float4* w4 = reinterpret_cast<float4*>(wl);
-    for (int32_t x = threadIdx.x; x < v; x += blockDim.x) {
-        const float4 t = t4[x];
-        const float4 e = e4[x];
-        const int32_t r0 = (4 * x) / a.k;
-        const int32_t r1 = (4 * x + 1) / a.k;
-        const int32_t r2 = (4 * x + 2) / a.k;
-        const int32_t r3 = (4 * x + 3) / a.k;
-        float4 w;
-        w.x = young_formula(t.x, a.mean[r0], ratio, e.x, a.alpha);
-        w.y = young_formula(t.y, a.mean[r1], ratio, e.y, a.alpha);
-        w.z = young_formula(t.z, a.mean[r2], ratio, e.z, a.alpha);
-        w.w = young_formula(t.w, a.mean[r3], ratio, e.w, a.alpha);
-        w4[x] = w;
+    const int32_t stride = blockDim.x;
+    for (int32_t x0 = threadIdx.x; x0 < v; x0 += U * stride) {
+        float4 t[U], e[U];
+        float mc[U][4];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {  // indices past the end reload the last vector
+            const int32_t x = min(x0 + q * stride, v - 1);
+            t[q] = t4[x];
+            e[q] = e4[x];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) mc[q][i] = a.mean[(4 * x + i) / a.k];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int32_t x = x0 + q * stride;
+            if (x < v) {
+                float4 w;
+                w.x = young_formula(t[q].x, mc[q][0], ratio, e[q].x, a.alpha);
+                w.y = young_formula(t[q].y, mc[q][1], ratio, e[q].y, a.alpha);
+                w.z = young_formula(t[q].z, mc[q][2], ratio, e[q].z, a.alpha);
+                w.w = young_formula(t[q].w, mc[q][3], ratio, e[q].w, a.alpha);
+                w4[x] = w;
+            }
+        }
     }
-    for (int32_t x = 4 * v + threadIdx.x; x < nk; x += blockDim.x)
+    for (int32_t x = 4 * v + threadIdx.x; x < nk; x += stride)
         wl[x] = young_formula(a.tau_nn[x], a.mean[x / a.k], ratio, a.eta_nn[x], a.alpha);
+}
+
+// Index of the y-th young ant (ant order), -1 if there are at most y; wave
+// 0 only.  Each lane loads the validity and age of 16 ants at a time before
+// any ballot, one memory round trip per 1024 ants.
+__device__ __forceinline__ int32_t find_young_ant(const ConstructArgs& a, int32_t y, int32_t converged, int lane) {
+    int32_t seen = 0;
+    for (int32_t a0 = 0; a0 < a.m; a0 += 1024) {
+        int32_t vl[16], ag[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {  // both loads unconditional (ants past m read ant m - 1)
+            const int32_t k = min(a0 + 64 * q + lane, a.m - 1);
+            vl[q] = a.valid[k];
+            ag[q] = a.age[k];
+        }
+        bool yg[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)  // ant_is_young
+            yg[q] = a0 + 64 * q + lane < a.m && a.aging && converged && vl[q] && ag[q] < a.min_age;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            unsigned long long mk = __ballot(yg[q]);
+            const int cnt = __popcll(mk);
+            if (seen + cnt > y) {
+                for (int t = 0; t < y - seen; ++t) mk &= mk - 1;
+                return a0 + 64 * q + __ffsll((long long)mk) - 1;
+            }
+            seen += cnt;
+        }
+    }
+    return -1;
 }
 
 __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a) {
@@ -1389,17 +1440,7 @@ __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a)
     staged_zero_entry(L, nk);
     for (int32_t y = blockIdx.x - normal_blocks;; y += a.young_blocks) {
         if (wave == 0) {
-            int32_t found = -1, seen = 0;
-            for (int32_t a0 = 0; a0 < a.m && found < 0; a0 += 64) {
-                const int32_t k = a0 + lane;
-                unsigned long long mk = __ballot(k < a.m && ant_is_young(a, k, converged));
-                const int cnt = __popcll(mk);
-                if (seen + cnt > y) {
-                    for (int t = 0; t < y - seen; ++t) mk &= mk - 1;
-                    found = a0 + __ffsll((long long)mk) - 1;
-                }
-                seen += cnt;
-            }
+            const int32_t found = find_young_ant(a, y, converged, lane);
             if (lane == 0) s_ant = found;
         }
         __syncthreads();
