@@ -1100,15 +1100,20 @@ __device__ __forceinline__ int32_t fallback_argmax_spread(const ConstructArgs& a
                     v[q] = ((free_bits >> q) & 1u) ? young_formula(t[q], mc, ratio, e[q], a.alpha) : -1.0f;
             }
         }
-        // ascending cities, strictly greater: the lowest index among equal
-        // maxima, and a NaN weight never wins
+        // visited cities weigh -1 (never above the initial best); ascending
+        // cities, strictly greater: the lowest index among equal maxima, and
+        // a NaN weight never wins.  Selects, not branches: a branch per city
+        // costs two exec-mask updates and a jump.
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) v[q] = ((free_bits >> q) & 1u) ? v[q] : -1.0f;
+        int32_t bq = 0;  // offset in the lane's cities (inline constants; j0 added once)
 #pragma unroll
         for (int q = 0; q < CPL; ++q) {
-            if (((free_bits >> q) & 1u) && v[q] > best) {
-                best = v[q];
-                bj = j0 + q;
-            }
+            const bool gt = v[q] > best;
+            best = gt ? v[q] : best;
+            bq = gt ? q : bq;
         }
+        bj = j0 + bq;  // read only when best > -1
     }
     int32_t next = wave_argmax_index(best, bj);
     if (next == 0x7fffffff) {  // only NaN weights left: lowest unvisited city
