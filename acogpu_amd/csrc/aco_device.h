@@ -92,10 +92,10 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// 64-bit wave reductions through DPP: the scan pattern of
-// wave_inclusive_scan (row shifts 1, 2, 4, 8, then the two row broadcasts)
-// with min / max; sources outside a row keep the identity.  Lane 63 ends
-// with the result, which every lane receives through v_readlane.
+// 64-bit wave minimum through DPP: the scan pattern of wave_inclusive_scan
+// (row shifts 1, 2, 4, 8, then the two row broadcasts); sources outside a
+// row keep the identity.  Lane 63 ends with the result, which every lane
+// receives through v_readlane.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ uint64_t dpp_u64(uint64_t v, uint32_t identity) {
     const uint32_t lo =
@@ -121,30 +121,40 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
     return lane63_u64(v);
 }
 
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
-    v = max(v, dpp_u64<0x111, 0xf>(v, 0u));
-    v = max(v, dpp_u64<0x112, 0xf>(v, 0u));
-    v = max(v, dpp_u64<0x114, 0xf>(v, 0u));
-    v = max(v, dpp_u64<0x118, 0xf>(v, 0u));
-    v = max(v, dpp_u64<0x142, 0xa>(v, 0u));
-    v = max(v, dpp_u64<0x143, 0xc>(v, 0u));
-    return lane63_u64(v);
+// DPP source with the lane's own value where the pattern has no source lane
+// (row shifts past the row start, rows outside ROW_MASK).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int32_t dpp_keep_i(int32_t v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xf, false);
 }
 
 // Wave argmax of per-lane (best, index) pairs: the largest value, ties to the
 // lowest index; best = -1 means "no candidate" (weights above -1 only; NaN
-// never becomes a lane's best).  Packed as (order-preserving float bits,
-// inverted index) so one 64-bit max decides both.  0x7fffffff if no lane has
-// a candidate.
+// never becomes a lane's best).  The maximum comes from a 32-bit float max
+// over the scan pattern of wave_inclusive_scan (one v_max per step); the
+// lanes holding it are one ballot, and when several do (equal weights) a
+// minimum over their indices decides.  -0 and +0 compare equal, as in the
+// sequential scan.  0x7fffffff if no lane has a candidate.
 __device__ __forceinline__ int32_t wave_argmax_index(float best, int32_t idx) {
-    uint64_t key = 0;
-    if (best > -1.0f) {
-        const uint32_t u = __float_as_uint(best + 0.0f);  // -0 -> +0: equal floats, equal keys
-        const uint32_t ord = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-        key = ((uint64_t)ord << 32) | (uint32_t)(0xffffffffu - (uint32_t)idx);
-    }
-    key = wave_max_u64(key);
-    return key ? (int32_t)(0xffffffffu - (uint32_t)key) : 0x7fffffff;
+    float m = best;
+    m = fmaxf(m, __int_as_float(dpp_keep_i<0x111, 0xf>(__float_as_int(m))));  // row_shr:1
+    m = fmaxf(m, __int_as_float(dpp_keep_i<0x112, 0xf>(__float_as_int(m))));  // row_shr:2
+    m = fmaxf(m, __int_as_float(dpp_keep_i<0x114, 0xf>(__float_as_int(m))));  // row_shr:4
+    m = fmaxf(m, __int_as_float(dpp_keep_i<0x118, 0xf>(__float_as_int(m))));  // row_shr:8
+    m = fmaxf(m, __int_as_float(dpp_keep_i<0x142, 0xa>(__float_as_int(m))));  // row_bcast:15
+    m = fmaxf(m, __int_as_float(dpp_keep_i<0x143, 0xc>(__float_as_int(m))));  // row_bcast:31
+    m = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), 63));
+    if (!(m > -1.0f)) return 0x7fffffff;
+    const unsigned long long hit = __builtin_amdgcn_ballot_w64(best == m);
+    if (__builtin_expect(__popcll(hit) == 1, 1)) return __builtin_amdgcn_readlane(idx, __ffsll((long long)hit) - 1);
+    int32_t v = best == m ? idx : 0x7fffffff;
+    v = min(v, dpp_keep_i<0x111, 0xf>(v));
+    v = min(v, dpp_keep_i<0x112, 0xf>(v));
+    v = min(v, dpp_keep_i<0x114, 0xf>(v));
+    v = min(v, dpp_keep_i<0x118, 0xf>(v));
+    v = min(v, dpp_keep_i<0x142, 0xa>(v));
+    v = min(v, dpp_keep_i<0x143, 0xc>(v));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Heuristic lookup: table by integer distance, 0 outside (diagonal).
