@@ -1815,6 +1815,23 @@ __device__ __forceinline__ uint64_t block_min_u64(uint64_t v, uint64_t* red) {
     return b;
 }
 
+// Wave minimum of 32-bit keys: one v_min_u32 per step of the scan pattern
+// (sources outside the pattern read ~0), lane 63 read by every lane.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_min_src_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)~0u, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = min(v, dpp_min_src_u32<0x111, 0xf>(v));
+    v = min(v, dpp_min_src_u32<0x112, 0xf>(v));
+    v = min(v, dpp_min_src_u32<0x114, 0xf>(v));
+    v = min(v, dpp_min_src_u32<0x118, 0xf>(v));
+    v = min(v, dpp_min_src_u32<0x142, 0xa>(v));
+    v = min(v, dpp_min_src_u32<0x143, 0xc>(v));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // (len, ant) keys packed as len << 20 | ant: the packed order is the
 // (len, ant) order for ant < 2^20 and len < 2^44 (n <= 2^18 cities of
 // distance <= 2^26).  NO_KEY marks "no ant".
@@ -1822,6 +1839,43 @@ constexpr uint64_t NO_KEY = ~0ull;
 __device__ __forceinline__ uint64_t pack_key(int64_t len, int32_t ant) { return ((uint64_t)len << 20) | (uint32_t)ant; }
 __device__ __forceinline__ int64_t key_len(uint64_t k) { return (int64_t)(k >> 20); }
 __device__ __forceinline__ int32_t key_ant(uint64_t k) { return (int32_t)(k & 0xfffff); }
+
+// The same (len, ant) order in the key width the statistics block works in:
+// 64-bit keys in general, 32-bit keys (len << 10 | ant) when m <= 1024 and
+// every valid ant's length is below 2^22, where each reduction step is one
+// v_min_u32 instead of a 64-bit compare and two selects.
+template <typename K>
+struct StatKey;
+template <>
+struct StatKey<uint64_t> {
+    static constexpr uint64_t NONE = ~0ull;
+    static __device__ __forceinline__ uint64_t pack(int64_t len, int32_t ant) { return pack_key(len, ant); }
+    static __device__ __forceinline__ int64_t len(uint64_t k) { return key_len(k); }
+    static __device__ __forceinline__ int32_t ant(uint64_t k) { return key_ant(k); }
+    static __device__ __forceinline__ uint64_t wave_min(uint64_t v) { return wave_min_u64(v); }
+};
+template <>
+struct StatKey<uint32_t> {
+    static constexpr uint32_t NONE = ~0u;
+    static constexpr int64_t LEN_LIMIT = 1 << 22;
+    static __device__ __forceinline__ uint32_t pack(int64_t len, int32_t ant) {
+        return ((uint32_t)len << 10) | (uint32_t)ant;
+    }
+    static __device__ __forceinline__ int64_t len(uint32_t k) { return (int64_t)(k >> 10); }
+    static __device__ __forceinline__ int32_t ant(uint32_t k) { return (int32_t)(k & 1023u); }
+    static __device__ __forceinline__ uint32_t wave_min(uint32_t v) { return wave_min_u32(v); }
+};
+
+template <typename K>
+__device__ __forceinline__ K block_min_key(K v, K* red) {
+    v = StatKey<K>::wave_min(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    K b = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = min(b, red[w]);
+    return b;
+}
 
 // The same statistics with every ant's length, validity and age loaded into
 // registers once (at most four ants per thread) and the iteration state kept
@@ -1832,48 +1886,35 @@ struct StatsShared {
     int32_t gperf, num_valid, improved, recruit;
 };
 
-__device__ __forceinline__ void stats_block_fast(const StatsArgs& a, uint64_t* red, uint64_t* red_rank,
+// Per-thread view of its (at most four) ants, loaded once.
+struct StatsAnts {
+    int64_t L[4];
+    int32_t AG[4];
+    bool IN[4], V[4];
+};
+
+template <typename K>
+__device__ __forceinline__ void stats_block_fast(const StatsArgs& a, const StatsAnts& s, K* red, K* red_rank,
                                                  StatsShared& sh) {
+    using SK = StatKey<K>;
     DevState* st = a.state;
     const int32_t B = blockDim.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (threadIdx.x == 0) {
-        sh.glob_best = st->glob_best;
-        sh.gperf = st->gperf;
-        sh.num_valid = st->num_valid;
-    }
-    int64_t L[4], PL[4];
-    int32_t AG[4], PF[4];
-    bool IN[4], V[4];
+    bool V[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int32_t k = threadIdx.x + q * B;
-        IN[q] = k < a.m;
-        L[q] = IN[q] ? a.lengths[k] : 0;
-        V[q] = IN[q] && (!a.aging || a.valid[k]);
-        AG[q] = (IN[q] && a.aging) ? a.age[k] : 0;
-        PL[q] = (IN[q] && a.aging) ? a.prev_len[k] : 0;
-        PF[q] = (IN[q] && a.aging) ? a.perf[k] : 0;
-    }
-    uint64_t mine = NO_KEY;
+    for (int q = 0; q < 4; ++q) V[q] = s.V[q];
+    K mine = SK::NONE;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (!V[q]) continue;
-        const int32_t k = threadIdx.x + q * B;
-        if (a.aging) {
-            a.perf[k] = (L[q] == PL[q]) ? PF[q] + 1 : 0;
-            a.prev_len[k] = L[q];
-        }
-        mine = min(mine, pack_key(L[q], k));
-    }
-    const uint64_t ib = block_min_u64(mine, red);
+    for (int q = 0; q < 4; ++q)
+        if (V[q]) mine = min(mine, SK::pack(s.L[q], (int32_t)(threadIdx.x + q * B)));
+    const K ib = block_min_key(mine, red);
     // iterBestDist starts at MAX_DISTANCE - 1 and only a strictly smaller
     // length is taken (src/Colony.cu:872,1015).
-    const bool found = ib != NO_KEY && key_len(ib) < (int64_t)ACO_MAX_DISTANCE - 1;
-    const int64_t ibest = found ? key_len(ib) : (int64_t)ACO_MAX_DISTANCE - 1;
+    const bool found = ib != SK::NONE && SK::len(ib) < (int64_t)ACO_MAX_DISTANCE - 1;
+    const int64_t ibest = found ? SK::len(ib) : (int64_t)ACO_MAX_DISTANCE - 1;
     if (threadIdx.x == 0) {
         st->iter_best = ibest;
-        st->iter_best_ant = found ? key_ant(ib) : -1;
+        st->iter_best_ant = found ? SK::ant(ib) : -1;
         const bool improved = found && ibest < sh.glob_best;
         if (improved) {
             sh.glob_best = ibest;
@@ -1883,7 +1924,7 @@ __device__ __forceinline__ void stats_block_fast(const StatsArgs& a, uint64_t* r
         }
         st->glob_best = sh.glob_best;
         st->gperf = sh.gperf;
-        sh.improved = improved ? key_ant(ib) : -1;
+        sh.improved = improved ? SK::ant(ib) : -1;
     }
     __syncthreads();
     const int32_t imp = sh.improved;
@@ -1893,41 +1934,41 @@ __device__ __forceinline__ void stats_block_fast(const StatsArgs& a, uint64_t* r
     if (a.variant == 0) {  // AS: dense, one entry per ant
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (IN[q]) a.dep_val[threadIdx.x + q * B] = V[q] ? (float)(1.0 / (double)L[q]) : 0.0f;
+            if (s.IN[q]) a.dep_val[threadIdx.x + q * B] = V[q] ? (float)(1.0 / (double)s.L[q]) : 0.0f;
         if (threadIdx.x == 0) st->ndep = a.m;
     } else {
         // rank-based: the w-1 smallest keys in two rounds (every wave its
         // own, then wave 0 over the waves' candidates)
         const int32_t t = a.w - 1;
-        uint64_t ks[4];
+        K ks[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) ks[q] = V[q] ? pack_key(L[q], (int32_t)(threadIdx.x + q * B)) : NO_KEY;
+        for (int q = 0; q < 4; ++q) ks[q] = V[q] ? SK::pack(s.L[q], (int32_t)(threadIdx.x + q * B)) : SK::NONE;
         for (int32_t r = 0; r < t; ++r) {
-            const uint64_t wmin = wave_min_u64(min(min(ks[0], ks[1]), min(ks[2], ks[3])));
+            const K wmin = SK::wave_min(min(min(ks[0], ks[1]), min(ks[2], ks[3])));
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (ks[q] == wmin) ks[q] = NO_KEY;
+                if (ks[q] == wmin) ks[q] = SK::NONE;
             if (lane == 0) red_rank[wave * RANK_MAX + r] = wmin;
         }
         __syncthreads();
         if (wave == 0) {
             const int32_t nq = (int32_t)(B >> 6) * RANK_MAX / 64;  // 1..4 candidates per lane
-            uint64_t cs[4];
+            K cs[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int32_t x = lane + 64 * q;
-                cs[q] = (q < nq && (x % RANK_MAX) < t) ? red_rank[x] : NO_KEY;
+                cs[q] = (q < nq && (x % RANK_MAX) < t) ? red_rank[x] : SK::NONE;
             }
             int32_t nd = 0;
             for (int32_t r = 0; r < t; ++r) {
-                const uint64_t wmin = wave_min_u64(min(min(cs[0], cs[1]), min(cs[2], cs[3])));
-                if (wmin == NO_KEY) break;  // fewer valid ants than w-1
+                const K wmin = SK::wave_min(min(min(cs[0], cs[1]), min(cs[2], cs[3])));
+                if (wmin == SK::NONE) break;  // fewer valid ants than w-1
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (cs[q] == wmin) cs[q] = NO_KEY;
+                    if (cs[q] == wmin) cs[q] = SK::NONE;
                 if (lane == 0) {
-                    a.dep_ant[nd] = key_ant(wmin);
-                    a.dep_val[nd] = (float)((double)(a.w - (r + 1)) / (double)key_len(wmin));
+                    a.dep_ant[nd] = SK::ant(wmin);
+                    a.dep_val[nd] = (float)((double)(a.w - (r + 1)) / (double)SK::len(wmin));
                 }
                 ++nd;
             }
@@ -1943,40 +1984,47 @@ __device__ __forceinline__ void stats_block_fast(const StatsArgs& a, uint64_t* r
     }
 
     if (a.aging) {
-        // retireAnts: first valid ant among the first m/2 at max_age.
-        uint64_t cand = NO_KEY;
+        // retireAnts: the first valid ant among the first m/2 at max_age;
+        // recruiteAnts: the first invalid ant (the retired one included),
+        // when the colony stagnates or is small.  Both minima in one block
+        // reduction: the first invalid ant after the retirement is the
+        // smaller of the first one before it and the retired ant.
+        uint32_t cand = ~0u, inv = ~0u;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int32_t k = threadIdx.x + q * B;
-            if (k < a.m / 2 && V[q] && AG[q] >= a.max_age) cand = min(cand, (uint64_t)k);
+            const uint32_t k = threadIdx.x + q * B;
+            if ((int32_t)k < a.m / 2 && V[q] && s.AG[q] >= a.max_age) cand = min(cand, k);
+            if (s.IN[q] && !V[q]) inv = min(inv, k);
         }
-        const uint64_t ret = block_min_u64(cand, red);
-        if (ret != NO_KEY) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if ((uint64_t)(threadIdx.x + q * B) == ret) V[q] = false;
-            if (threadIdx.x == 0) {
+        cand = wave_min_u32(cand);
+        inv = wave_min_u32(inv);
+        uint32_t* red32 = reinterpret_cast<uint32_t*>(red);
+        const int nw = (int)(blockDim.x >> 6);
+        __syncthreads();
+        if (lane == 0) {
+            red32[wave] = cand;
+            red32[nw + wave] = inv;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t ret = red32[0], first_inv = red32[nw];
+            for (int w = 1; w < nw; ++w) {
+                ret = min(ret, red32[w]);
+                first_inv = min(first_inv, red32[nw + w]);
+            }
+            if (ret != ~0u) {
                 a.valid[ret] = 0;
                 sh.num_valid -= 1;
                 st->converged = 1;
+                first_inv = min(first_inv, ret);
             }
-        }
-        // recruiteAnts: first invalid ant, when the colony stagnates or is small.
-        if (threadIdx.x == 0) sh.recruit = (sh.gperf >= a.global_perf || sh.num_valid < a.min_pop);
-        __syncthreads();
-        if (sh.recruit) {
-            uint64_t inv = NO_KEY;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (IN[q] && !V[q]) inv = min(inv, (uint64_t)(threadIdx.x + q * B));
-            const uint64_t rc = block_min_u64(inv, red);
-            if (threadIdx.x == 0 && rc != NO_KEY) {
-                a.valid[rc] = 1;
-                a.age[rc] = 0;
+            if ((sh.gperf >= a.global_perf || sh.num_valid < a.min_pop) && first_inv != ~0u) {
+                a.valid[first_inv] = 1;
+                a.age[first_inv] = 0;
                 sh.num_valid += 1;
             }
+            st->num_valid = sh.num_valid;
         }
-        if (threadIdx.x == 0) st->num_valid = sh.num_valid;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1987,11 +2035,53 @@ __device__ __forceinline__ void stats_block_fast(const StatsArgs& a, uint64_t* r
     }
 }
 
+// Loads the ants once, updates the stagnation counters and picks the key
+// width (a block-wide vote on the 32-bit limits).
+__device__ __forceinline__ void stats_block_fast_entry(const StatsArgs& a, uint64_t* red, uint64_t* red_rank,
+                                                       StatsShared& sh) {
+    DevState* st = a.state;
+    const int32_t B = blockDim.x;
+    if (threadIdx.x == 0) {
+        sh.glob_best = st->glob_best;
+        sh.gperf = st->gperf;
+        sh.num_valid = st->num_valid;
+    }
+    StatsAnts s;
+    int64_t PL[4];
+    int32_t PF[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int32_t k = threadIdx.x + q * B;
+        s.IN[q] = k < a.m;
+        s.L[q] = s.IN[q] ? a.lengths[k] : 0;
+        s.V[q] = s.IN[q] && (!a.aging || a.valid[k]);
+        s.AG[q] = (s.IN[q] && a.aging) ? a.age[k] : 0;
+        PL[q] = (s.IN[q] && a.aging) ? a.prev_len[k] : 0;
+        PF[q] = (s.IN[q] && a.aging) ? a.perf[k] : 0;
+    }
+    bool wide = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (!s.V[q]) continue;
+        const int32_t k = threadIdx.x + q * B;
+        if (a.aging) {
+            a.perf[k] = (s.L[q] == PL[q]) ? PF[q] + 1 : 0;
+            a.prev_len[k] = s.L[q];
+        }
+        wide = wide || s.L[q] < 0 || s.L[q] >= StatKey<uint32_t>::LEN_LIMIT;
+    }
+    wide = __syncthreads_or(wide ? 1 : 0) != 0 || a.m > 1024;
+    if (wide)
+        stats_block_fast<uint64_t>(a, s, red, red_rank, sh);
+    else
+        stats_block_fast<uint32_t>(a, s, reinterpret_cast<uint32_t*>(red), reinterpret_cast<uint32_t*>(red_rank), sh);
+}
+
 __device__ __forceinline__ void stats_block(const StatsArgs& a, LenKey* red, int32_t* redi, uint32_t* taken, int32_t& s_improved,
                             int32_t& s_recruit, LenKey* red_rank, StatsShared& sh) {
     if (a.m <= 4 * (int32_t)blockDim.x && a.m < (1 << 20) && a.n <= (1 << 18) &&
         (a.variant == 0 || a.w - 1 <= RANK_MAX))
-        stats_block_fast(a, reinterpret_cast<uint64_t*>(red), reinterpret_cast<uint64_t*>(red_rank), sh);
+        stats_block_fast_entry(a, reinterpret_cast<uint64_t*>(red), reinterpret_cast<uint64_t*>(red_rank), sh);
     else
         stats_block_general(a, red, redi, taken, s_improved, s_recruit, red_rank);
 }
