@@ -2115,28 +2115,45 @@ __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
 // Deposit: one block per list entry, fp32 atomics on both (i,j) and (j,i) of
 // every tour edge (dev_update_pheromones src/AntSystem.cu:105-128, which only
 // updated one direction; the CPU branch src/AntSystem.cu:296-316 is symmetric).
-__device__ __forceinline__ void deposit_entry(const DepositArgs& a, int32_t e) {
-    if (e >= a.state->ndep) return;
+// One depositing tour per block.  The iteration state and the block's
+// deposit entry are read together (the entry arrays hold m + w + 2 slots per
+// colony, so every block's slot exists), and each thread loads the ends of
+// up to U edges before its atomics: two memory round trips before the first
+// atomic instead of one per phase and per edge batch.
+__device__ __forceinline__ void deposit_tour(const DepositArgs& a, int32_t e, bool check_skip) {
+    constexpr int U = 8;
+    const int32_t skip = check_skip ? a.state->skip : 0;
+    const int32_t ndep = a.state->ndep;
     const int32_t ant = a.dense ? e : a.dep_ant[e];
     const float d = a.dep_val[e];
-    if (d == 0.0f) return;
+    if (skip || e >= ndep || d == 0.0f) return;
     const int32_t* t = ant >= 0 ? a.tours + (size_t)ant * a.n : a.best_tour;
-    for (int32_t i = threadIdx.x; i < a.n; i += blockDim.x) {
-        const int32_t x = t[i];
-        const int32_t y = t[(i + 1 == a.n) ? 0 : i + 1];
-        unsafeAtomicAdd(a.tau_next + (size_t)x * a.P + y, d);
-        unsafeAtomicAdd(a.tau_next + (size_t)y * a.P + x, d);
+    const int32_t stride = blockDim.x;
+    for (int32_t i0 = threadIdx.x; i0 < a.n; i0 += U * stride) {
+        int32_t x[U], y[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {  // indices past the end reload the last edge
+            const int32_t i = min(i0 + q * stride, a.n - 1);
+            x[q] = t[i];
+            y[q] = t[(i + 1 == a.n) ? 0 : i + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            if (i0 + q * stride < a.n) {
+                unsafeAtomicAdd(a.tau_next + (size_t)x[q] * a.P + y[q], d);
+                unsafeAtomicAdd(a.tau_next + (size_t)y[q] * a.P + x[q], d);
+            }
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
     stamp_start(a.stamp);
-    if (a.state->skip) return;
-    deposit_entry(a, blockIdx.x);
+    deposit_tour(a, blockIdx.x, true);
 }
 
 // Ant System deposit in two passes with no global atomics; the same sum as
-// deposit_entry (src/AntSystem.cu:295-317): tau_xy += dep_val[a] for every
+// deposit_tour (src/AntSystem.cu:295-317): tau_xy += dep_val[a] for every
 // ant a whose tour uses edge (x, y), in both directions.
 // Pass 1: ant a writes succ_a(x) to slot 2a and pred_a(y) to slot 2a+1 of
 // rows x and y of its tour edges (x, y), one wave per ant, 64 edges per
@@ -2938,7 +2955,7 @@ __global__ __launch_bounds__(256) void kb_deposit(const BatchBuffers b) {
         e = blockIdx.x % (b.w + 1);
     }
     const InstMeta im = b.meta[inst];
-    deposit_entry(batch_deposit_args(b, inst, im), e);
+    deposit_tour(batch_deposit_args(b, inst, im), e, false);
 }
 
 // ---------------------------------------------------------------------------
