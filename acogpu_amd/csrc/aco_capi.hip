@@ -63,7 +63,7 @@ struct aco_ctx {
     hipEvent_t ev[6] = {};
     hipEvent_t pev[6] = {};     // asynchronous phases: iteration start, then the end of phase p at 1 + p
     uint32_t phases_async = 0;  // bit p: phase p enqueued without waiting since the iteration began
-    hipEvent_t tev[5] = {};     // timed iteration: start of its first kernel, then the stop of each phase
+    hipEvent_t tev_end = nullptr;  // end of a timed iteration (phase times come from in-kernel stamps)
     int64_t* h_stats = nullptr; // coherent pinned host memory: {iteration best, best so far, valid ants}
     int64_t* h_stamp = nullptr; // coherent pinned host memory: a timed iteration's five wall-clock stamps
     uint64_t* d_stamp = nullptr;  // the phases' start stamps and the end stamp (device)
@@ -131,8 +131,7 @@ static void free_all(aco_ctx* c) {
         if (e) hipEventDestroy(e);
     for (auto& e : c->pev)
         if (e) hipEventDestroy(e);
-    for (auto& e : c->tev)
-        if (e) hipEventDestroy(e);
+    if (c->tev_end) hipEventDestroy(c->tev_end);
     if (c->h_stats) hipHostFree(c->h_stats);
     if (c->h_stamp) hipHostFree(c->h_stamp);
     if (c->d_stamp) hipFree(c->d_stamp);
@@ -272,7 +271,7 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     HIPB(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPB(hipEventCreate(&e));
     for (auto& e : c->pev) HIPB(hipEventCreate(&e));
-    for (auto& e : c->tev) HIPB(hipEventCreate(&e));
+    HIPB(hipEventCreate(&c->tev_end));
     HIPB(hipHostMalloc(reinterpret_cast<void**>(&c->h_stats), 4 * sizeof(int64_t), hipHostMallocCoherent));
     HIPB(hipHostMalloc(reinterpret_cast<void**>(&c->h_stamp), 8 * sizeof(int64_t), hipHostMallocCoherent));
     std::memset(c->h_stamp, 0, 8 * sizeof(int64_t));
@@ -660,7 +659,7 @@ static hipError_t enqueue_timed_iteration(aco_ctx* c, int cur) {
         e = enqueue_phase(c, cur, order[i], true, t);
     }
     if (e == hipSuccess) e = aco::launch_stamp_end(c->d_stamp, c->h_stamp, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(c->tev[4], c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->tev_end, c->stream);
     return e;
 }
 
@@ -690,7 +689,7 @@ extern "C" int aco_iterate_stats(aco_ctx* c, float* timers, int64_t* iter_best, 
     if (c->next_phase != ACO_PHASE_PROBABILITIES) return fail(c, ACO_E_STATE, "aco_iterate inside a phased iteration");
     HIP_TRY(c, enqueue_timed_iteration(c, c->cur));
     c->cur ^= 1;
-    HIP_TRY(c, hipEventSynchronize(c->tev[4]));
+    HIP_TRY(c, hipEventSynchronize(c->tev_end));
     if (timers) {
         // phase i: from its first kernel's start to the next phase's (the
         // last one to the end stamp)
@@ -827,25 +826,18 @@ extern "C" int aco_run_until_time(aco_ctx* c, int32_t max_iterations, int64_t st
 extern "C" int aco_profile(aco_ctx* c, int32_t iterations, float* kernel_ms) {
     if (!c || !kernel_ms || iterations < 1) return ACO_E_ARG;
     if (!c->initialized) return fail(c, ACO_E_STATE, "aco_profile before aco_initialize");
-    // each phase's duration from its own dispatches (start of its first
-    // kernel to the stop of its last)
-    static const int order[4] = {ACO_PHASE_PROBABILITIES, ACO_PHASE_TOUR, ACO_PHASE_ANT_DISTANCE,
-                                 ACO_PHASE_UPDATE_PHEROMONE};
+    if (c->next_phase != ACO_PHASE_PROBABILITIES) return fail(c, ACO_E_STATE, "aco_profile inside a phased iteration");
+    // each phase from its first kernel's start to the next phase's first
+    // kernel's start (the last one to the end stamp), from the wall-clock
+    // stamps the kernels themselves store: events around the launches add
+    // their own dispatch overhead to every interval
     double acc[4] = {0, 0, 0, 0};
     for (int32_t it = 0; it < iterations; ++it) {
-        for (int i = 0; i < 4; ++i) {
-            aco::LaunchTiming t;
-            t.start = c->pev[i];
-            t.stop = c->tev[i];
-            HIP_TRY(c, enqueue_phase(c, c->cur, order[i], false, t));
-        }
+        HIP_TRY(c, enqueue_timed_iteration(c, c->cur));
         c->cur ^= 1;
-        HIP_TRY(c, hipEventSynchronize(c->tev[3]));
-        for (int i = 0; i < 4; ++i) {
-            float t = 0;
-            HIP_TRY(c, hipEventElapsedTime(&t, c->pev[i], c->tev[i]));
-            acc[i] += t;
-        }
+        HIP_TRY(c, hipEventSynchronize(c->tev_end));
+        const volatile int64_t* h = c->h_stamp;
+        for (int i = 0; i < 4; ++i) acc[i] += (double)(h[i + 1] - h[i]) / c->wall_khz;
     }
     for (int i = 0; i < 4; ++i) kernel_ms[i] = (float)(acc[i] / iterations);
     return ACO_OK;

@@ -206,9 +206,10 @@ int aco_get_counters(aco_ctx *c, uint64_t *counters);
  * reserved for young aging ants, LDS bytes per block available}. */
 int aco_get_config(aco_ctx *c, int32_t *config);
 
-/* Runs `iterations` iterations with HIP events around every kernel on the
- * ctx stream; kernel_ms[0..3] = average ms of {sweep, construct, stats,
- * deposit}. */
+/* Runs `iterations` iterations on the ctx stream, each phase's first kernel
+ * storing its start time (device wall clock) and a last one-thread kernel the
+ * end; kernel_ms[0..3] = average ms of {sweep, construct, stats, deposit},
+ * each from its kernel's start to the next one's. */
 int aco_profile(aco_ctx *c, int32_t iterations, float *kernel_ms);
 
 /* Latency floor of one construction step at this colony's k: a single wave
