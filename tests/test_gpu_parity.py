@@ -252,6 +252,34 @@ def test_rbas_replay_forced_young_state(path):
         check_iteration(col, p, dist, snap, replay(p, dist, snap))
 
 
+# the register-bitmap fallback at each of its widths (8, 16 and 32 cities per
+# lane for n <= 512, 1024, 2048), for ordinary and young ants, with few
+# candidates so that most ants fall back many times
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("n", [509, 700, 1024, 1500, 2048])
+def test_rbas_replay_forced_young_state_fallback_widths(n, path):
+    m, k = 64, 6
+    say(f"case: RBAS aging forced state n={n} m={m} k={k} (fallback heavy) path={path}")
+    rng = np.random.default_rng(n)
+    xy = instances.uniform(n, 20000, seed=n)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=k, w=4, beta=2.0, rho=0.25, variant=capi.VARIANT_RBAS,
+                            aging=1, min_age=10, max_age=12, global_perf=5, min_pop=60, seed=n + 3,
+                            construct_path=path)
+    with make(p, xy=xy) as col:
+        col.set_pheromone(random_tau(rng, n))
+        age = rng.integers(0, 14, size=m).astype(np.int32)
+        valid = (rng.random(m) < 0.8).astype(np.uint8)
+        col.set_aging(age=age, valid=valid, perf=np.zeros(m, np.int32), prev_len=np.zeros(m, np.int64),
+                      scalars=np.array([int(valid.sum()), 1, 4, 17], np.int32))
+        col.set_best(10 ** 12, np.arange(n, dtype=np.int32))
+        snap = snapshot(col)
+        col.iterate()
+        check_iteration(col, p, dist, snap, replay(p, dist, snap))
+        fb = col.counters()[0]
+        assert fb > 0
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_heuristic_override_replay(path):
     """A candidate-edge heuristic replaces eta^beta on NN edges (old and young
