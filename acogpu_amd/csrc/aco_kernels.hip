@@ -373,6 +373,8 @@ struct SweepRow {
     int32_t xi, yi;                  // the row city's coordinates (a.xy set)
     float acc;
     TopSel top;
+    int32_t nnj;                     // candidate `lane` of the row and its heuristic, loaded
+    float nne;                       // before the row (lanes < k)
 };
 
 __device__ __forceinline__ SweepRow sweep_row_begin(const SweepArgs& a, int32_t i, uint64_t* topbuf) {
@@ -403,9 +405,10 @@ __device__ __forceinline__ void sweep_row_end(const SweepArgs& a, int32_t i, int
     }
     if (a.nn != nullptr) {
         for (int32_t q = lane; q < a.k; q += 64) {
-            const int32_t j = a.nn[(size_t)i * a.k + q];
+            const int32_t j = q < 64 ? r.nnj : a.nn[(size_t)i * a.k + q];
+            const float e = q < 64 ? r.nne : a.eta_nn[(size_t)i * a.k + q];
             const float t = r.trow[j];
-            a.choice_nn[(size_t)i * a.k + q] = apply_alpha(t, a.alpha) * a.eta_nn[(size_t)i * a.k + q];
+            a.choice_nn[(size_t)i * a.k + q] = apply_alpha(t, a.alpha) * e;
             if (a.tau_nn != nullptr) a.tau_nn[(size_t)i * a.k + q] = t;
         }
     }
@@ -502,6 +505,12 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int32_t i, int32
 
 __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf) {
     SweepRow r = sweep_row_begin(a, i, topbuf);
+    r.nnj = 0;
+    r.nne = 0.0f;
+    if (a.nn != nullptr && lane < a.k) {
+        r.nnj = a.nn[(size_t)i * a.k + lane];
+        r.nne = a.eta_nn[(size_t)i * a.k + lane];
+    }
     for (int32_t g0 = 0; g0 < a.n; g0 += 1024) sweep_chunk(a, i, g0, lane, r);
     sweep_row_end(a, i, lane, r);
 }
@@ -625,8 +634,11 @@ __device__ __forceinline__ void sweep_chunk_full(const SweepArgs& a, int32_t i, 
 }
 
 template <int SRC, int PW, bool ALPHA1>
-__device__ __forceinline__ void sweep_row_fast(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf) {
+__device__ __forceinline__ void sweep_row_fast(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf,
+                                               int32_t nnj, float nne) {
     SweepRow r = sweep_row_begin(a, i, topbuf);
+    r.nnj = nnj;
+    r.nne = nne;
     const int32_t full_end = a.n & ~1023;
     for (int32_t g0 = 0; g0 < full_end; g0 += 1024) sweep_chunk_full<SRC, PW, ALPHA1>(a, i, g0, lane, r);
     if (full_end < a.n) sweep_chunk(a, i, full_end, lane, r);
@@ -638,6 +650,15 @@ __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
     extern __shared__ __align__(16) uint64_t topbuf[];  // TOP_BUF keys per wave when a.top is set
     const int lane = threadIdx.x & 63;
     stamp_start(a.stamp);
+    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the row's candidate entries are requested first: their round trip
+    // overlaps the halted flag's and the row's instead of following them
+    int32_t nnj = 0;
+    float nne = 0.0f;
+    if (a.nn != nullptr && i < a.n && lane < a.k) {
+        nnj = a.nn[(size_t)i * a.k + lane];
+        nne = a.eta_nn[(size_t)i * a.k + lane];
+    }
     // A halted run skips whole iterations: the sweep, first kernel of an
     // iteration, publishes the decision for the other three.
     if (a.state != nullptr) {
@@ -649,9 +670,8 @@ __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
     // start of an iteration, before construction reads the ages.
     if (a.age != nullptr)
         for (int32_t k = blockIdx.x * 256 + threadIdx.x; k < a.m; k += gridDim.x * 256) a.age[k] += 1;
-    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= a.n) return;
-    sweep_row_fast<SRC, PW, ALPHA1>(a, i, lane, topbuf + (size_t)(threadIdx.x >> 6) * TOP_BUF);
+    sweep_row_fast<SRC, PW, ALPHA1>(a, i, lane, topbuf + (size_t)(threadIdx.x >> 6) * TOP_BUF, nnj, nne);
 }
 
 // Bit-for-bit check of eta_pow_fast<PW> against the table of
