@@ -252,6 +252,35 @@ def test_rbas_replay_forced_young_state(path):
         check_iteration(col, p, dist, snap, replay(p, dist, snap))
 
 
+def test_c2_full_size_settled_iterations_replay():
+    """The n = 1000 rank-based + aging workload at its full size (10 Gaussian
+    clusters, m = 1000, k = 20, w = 6, beta 3, rho 0.1, min_age 30, max_age
+    50), settled past the retirement age like the benchmark's colony, then
+    two iterations replayed through the oracle: young ants with their own
+    staged blocks, fallbacks of both kinds, the rank-based deposit."""
+    n, m = 1000, 1000
+    say(f"case: C2 full size n={n} m={m} k=20 w=6 RBAS + aging, 505 iterations then 2 replayed")
+    xy = instances.clustered(n, clusters=10, side=10000, sigma=400.0, seed=1)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(variant=capi.VARIANT_RBAS, mode=capi.MODE_CANDIDATE, n=n, m=m, k=20, w=6,
+                            alpha=1.0, beta=3.0, rho=0.1, tau0_policy=capi.TAU0_RBAS, aging=1,
+                            min_age=30, max_age=50, local_perf=10, global_perf=10, min_pop=40, max_pop=100,
+                            seed=20241203)
+    with make(p, xy=xy) as col:
+        assert col.config()["path"] == 2  # LDS-staged construction
+        col.run(505)
+        young_seen = False
+        for _ in range(2):
+            snap = snapshot(col)
+            young_seen = young_seen or bool(
+                snap["converged"] and np.any((snap["age"] + 1 < p.min_age) & (snap["valid"] > 0)))
+            fb0 = col.counters()[0]
+            col.iterate()
+            check_iteration(col, p, dist, snap, replay(p, dist, snap))
+            assert col.counters()[0] > fb0
+        assert young_seen, "no young ant in the settled colony"
+
+
 # the register-bitmap fallback at each of its widths (8, 16 and 32 cities per
 # lane for n <= 512, 1024, 2048), for ordinary and young ants, with few
 # candidates so that most ants fall back many times
