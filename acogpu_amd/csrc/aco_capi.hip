@@ -590,14 +590,10 @@ static hipError_t enqueue_stats(aco_ctx* c, int cur, bool mirror, const aco::Lau
     return hipSuccess;
 }
 
+// Ant System: the adjacency deposit where its buffer exists, else fp32
+// atomics; rank-based: the ordered row deposit.
 static hipError_t enqueue_deposit(aco_ctx* c, int cur, const aco::LaunchTiming& t) {
-    hipError_t e;
-    float* tau = c->d_tau[cur];
     float* tau_next = c->d_tau[1 - cur];
-    const bool cand = c->p.mode == ACO_MODE_CANDIDATE;
-    (void)tau;
-    (void)tau_next;
-    (void)cand;
     aco::DepositArgs da{};
     da.tours = c->d_tours;
     da.best_tour = c->d_best_tour;
@@ -611,10 +607,10 @@ static hipError_t enqueue_deposit(aco_ctx* c, int cur, const aco::LaunchTiming& 
     da.dense = c->p.variant == ACO_VARIANT_AS;
     da.adj = c->d_adj;
     da.m = c->m;
+    da.list_cap = c->m + c->p.w + 2;
     if (da.dense && c->d_adj) return aco::launch_deposit_adj(da, c->stream, t);
-    const int32_t grid = da.dense ? c->m : c->p.w + 1;
-    if ((e = aco::launch_deposit(da, grid, c->stream, t)) != hipSuccess) return e;
-    return hipSuccess;
+    if (!da.dense) return aco::launch_deposit_rows(da, c->stream, t);
+    return aco::launch_deposit(da, c->m, c->stream, t);
 }
 
 // mirror: the statistics kernel also writes its results to h_stats; t: the

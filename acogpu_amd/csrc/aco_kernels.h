@@ -130,6 +130,7 @@ struct DepositArgs {
     // predecessor (2a+1) of x.
     uint16_t* adj;
     int32_t m;
+    int32_t list_cap;        // slots of dep_ant / dep_val (m + w + 2)
     uint64_t* stamp;         // set: block 0 writes the kernel's start (wall_clock64) here
 };
 
@@ -310,6 +311,9 @@ hipError_t launch_nn_aux(const int32_t* nn, const int32_t* dist, const float* et
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());
 hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s,
                           const LaunchTiming& t = LaunchTiming());
+// Rank-based deposit (list entries in order, one writer per cell, no
+// atomics): the same fp32 sums on every launch.
+hipError_t launch_deposit_rows(const DepositArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());
 // Ant System deposit without global atomics: tours scattered into adj, then
 // every pheromone row gathers its deposits in LDS (needs a.adj, a.m, n <= ADJ_MAX_N).
 hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());

@@ -1854,8 +1854,7 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 
 // (len, ant) keys packed as len << 20 | ant: the packed order is the
 // (len, ant) order for ant < 2^20 and len < 2^44 (n <= 2^18 cities of
-// distance <= 2^26).  NO_KEY marks "no ant".
-constexpr uint64_t NO_KEY = ~0ull;
+// distance <= 2^26).
 __device__ __forceinline__ uint64_t pack_key(int64_t len, int32_t ant) { return ((uint64_t)len << 20) | (uint32_t)ant; }
 __device__ __forceinline__ int64_t key_len(uint64_t k) { return (int64_t)(k >> 20); }
 __device__ __forceinline__ int32_t key_ant(uint64_t k) { return (int32_t)(k & 0xfffff); }
@@ -2172,6 +2171,134 @@ __global__ __launch_bounds__(256) void k_deposit(const DepositArgs a) {
     deposit_tour(a, blockIdx.x, true);
 }
 
+// Rank-based deposit with one writer per pheromone cell.  A block owns a
+// band of rows x0 .. x0 + rows - 1 (rows <= ROWS) of tau_next.  For each
+// pass of up to DEP_PASS list entries, the waves read the entries' tours 64
+// positions at a time and note, for every city of the band, its predecessor
+// and successor in each tour (neighbours by shuffle, the wave's two end
+// lanes load theirs).  Then four lanes per row add the pass's deposits to
+// the row's cells in list order, each cell loaded once and stored once by
+// one lane: ((tau + d_0) + d_1) + ..., the order of the CPU loop that adds
+// the ranked tours one after another (src/AntSystem.cu:296-316 per tour;
+// the list is built by the statistics kernel).  No atomics, so the sums, and
+// with them the colony's run, are the same from launch to launch.  Small
+// bands spread the cell loads, which miss in L2, over many CUs; every block
+// reads the whole tours.  Every thread of a 1024-thread block calls it (it
+// synchronises the block).
+constexpr int32_t DEP_PASS = 8;
+template <int ROWS>
+struct DepositRowsShared {
+    int32_t nb[DEP_PASS][ROWS][2];
+    float dv[DEP_PASS];
+    const int32_t* tp[DEP_PASS];
+};
+template <int ROWS>
+__device__ __forceinline__ void deposit_rows_band(const DepositArgs& a, int32_t x0, int32_t rows, bool check_skip,
+                                                  DepositRowsShared<ROWS>& sh) {
+    static_assert(4 * ROWS <= 1024, "four lanes per row");
+    int32_t(&nb)[DEP_PASS][ROWS][2] = sh.nb;
+    float(&dv)[DEP_PASS] = sh.dv;
+    const int32_t*(&tp)[DEP_PASS] = sh.tp;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int32_t n = a.n;
+    const int32_t nseg = (n + 63) >> 6;
+    // the first pass's list slots are read with the list length and the skip
+    // flag, in one round trip (slots past ndep are read but never used)
+    int32_t ant = -1;
+    float dval = 0.0f;
+    if ((int32_t)threadIdx.x < min(DEP_PASS, a.list_cap)) {
+        ant = a.dep_ant[threadIdx.x];
+        dval = a.dep_val[threadIdx.x];
+    }
+    const int32_t ndep = a.state->ndep;
+    if (check_skip && a.state->skip) return;
+    for (int32_t e0 = 0; e0 < ndep; e0 += DEP_PASS) {
+        const int32_t ne = min(DEP_PASS, ndep - e0);
+        if (e0 > 0 && (int32_t)threadIdx.x < ne) {
+            ant = a.dep_ant[e0 + threadIdx.x];
+            dval = a.dep_val[e0 + threadIdx.x];
+        }
+        if ((int32_t)threadIdx.x < ne) {
+            dv[threadIdx.x] = dval;
+            tp[threadIdx.x] = ant >= 0 ? a.tours + (size_t)ant * n : a.best_tour;
+        }
+        __syncthreads();
+        // wave w takes positions 64 s + lane, s = w, w + 16, ..., of every
+        // entry of the pass at once (DEP_PASS loads in flight per lane)
+        for (int32_t s0 = wave; s0 < nseg; s0 += nw) {
+            const int32_t i = (s0 << 6) + lane;
+            const bool last = lane == 63 || i == n - 1;  // successor outside the wave's 64
+            int32_t c[DEP_PASS], pe[DEP_PASS], se[DEP_PASS];
+#pragma unroll
+            for (int e = 0; e < DEP_PASS; ++e) {
+                const bool in = e < ne && i < n && dv[e] != 0.0f;
+                const int32_t* t = tp[e];
+                c[e] = in ? t[i] : -1;
+                pe[e] = (in && lane == 0) ? t[i == 0 ? n - 1 : i - 1] : -1;
+                se[e] = (in && last) ? t[i + 1 == n ? 0 : i + 1] : -1;
+            }
+#pragma unroll
+            for (int e = 0; e < DEP_PASS; ++e) {
+                const int32_t up = __shfl_up(c[e], 1), down = __shfl_down(c[e], 1);
+                const int32_t r = c[e] - x0;
+                if (c[e] >= 0 && (uint32_t)r < (uint32_t)rows) {
+                    nb[e][r][0] = lane == 0 ? pe[e] : up;
+                    nb[e][r][1] = last ? se[e] : down;
+                }
+            }
+        }
+        __syncthreads();
+        if ((int32_t)threadIdx.x < 4 * rows) {
+            // the row's 2 ne (target, deposit) slots; a target's first slot
+            // loads the cell, sums every slot naming it in list order and
+            // stores it; the row's k-th first slot is lane k mod 4's
+            const int32_t r = threadIdx.x >> 2, sub = threadIdx.x & 3;
+            float* row = a.tau_next + (size_t)(x0 + r) * a.P;
+            int32_t j[2 * DEP_PASS];
+            float d[2 * DEP_PASS], v[2 * DEP_PASS];
+#pragma unroll
+            for (int q = 0; q < 2 * DEP_PASS; ++q) {
+                const int e = q >> 1;
+                const bool on = e < ne && dv[e] != 0.0f;
+                j[q] = on ? nb[e][r][q & 1] : -1;
+                d[q] = on ? dv[e] : 0.0f;
+            }
+            uint32_t mine = 0;
+            int32_t cnt = 0;
+#pragma unroll
+            for (int q = 0; q < 2 * DEP_PASS; ++q) {
+                bool f = j[q] >= 0;
+#pragma unroll
+                for (int q2 = 0; q2 < q; ++q2) f = f && j[q2] != j[q];
+                mine |= (uint32_t)(f && (cnt & 3) == sub) << q;
+                cnt += f;
+            }
+#pragma unroll
+            for (int q = 0; q < 2 * DEP_PASS; ++q) v[q] = ((mine >> q) & 1) ? row[j[q]] : 0.0f;
+#pragma unroll
+            for (int q = 0; q < 2 * DEP_PASS; ++q) {
+                if (!((mine >> q) & 1)) continue;
+                float s = v[q];
+#pragma unroll
+                for (int q2 = q; q2 < 2 * DEP_PASS; ++q2)
+                    if (j[q2] == j[q]) s += d[q2];
+                row[j[q]] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Bands of 16 rows up to 4096 cities (every block rereads the whole tours
+// from L2), 256 rows beyond.
+template <int ROWS>
+__global__ __launch_bounds__(1024) void k_deposit_rows(const DepositArgs a) {
+    stamp_start(a.stamp);
+    __shared__ DepositRowsShared<ROWS> sh;
+    const int32_t x0 = blockIdx.x * ROWS;
+    deposit_rows_band<ROWS>(a, x0, min(ROWS, a.n - x0), true, sh);
+}
+
 // Ant System deposit in two passes with no global atomics; the same sum as
 // deposit_tour (src/AntSystem.cu:295-317): tau_xy += dep_val[a] for every
 // ant a whose tour uses edge (x, y), in both directions.
@@ -2428,6 +2555,7 @@ __device__ __forceinline__ DepositArgs batch_deposit_args(const BatchBuffers& b,
     a.dense = b.variant == 0;
     a.adj = b.adj ? b.adj + im.adj_off : nullptr;
     a.m = im.m;
+    a.list_cap = im.m + b.w + 2;
     return a;
 }
 
@@ -2965,17 +3093,29 @@ __global__ __launch_bounds__(256) void kb_adj_gather(const BatchBuffers b, int32
     }
 }
 
+// Ant System batch deposit with fp32 atomics (no adjacency buffer): one
+// block per ant.
 __global__ __launch_bounds__(256) void kb_deposit(const BatchBuffers b) {
-    int32_t inst, e;
-    if (b.variant == 0) {
-        inst = b.ant_inst[blockIdx.x];
-        e = blockIdx.x - b.meta[inst].ant_off;
-    } else {
-        inst = blockIdx.x / (b.w + 1);
-        e = blockIdx.x % (b.w + 1);
-    }
+    const int32_t inst = b.ant_inst[blockIdx.x];
+    const int32_t e = blockIdx.x - b.meta[inst].ant_off;
     const InstMeta im = b.meta[inst];
     deposit_tour(batch_deposit_args(b, inst, im), e, false);
+}
+
+// Rank-based batch deposit: block b takes global rows 64 b .. 64 b + 63,
+// one band per instance those rows belong to, each deposited as the single
+// colony's (deposit_rows_band): the same ordered fp32 sums per instance.
+constexpr int32_t BATCH_DEP_ROWS = 64;
+__global__ __launch_bounds__(1024) void kb_deposit_rows(const BatchBuffers b) {
+    __shared__ DepositRowsShared<BATCH_DEP_ROWS> sh;
+    const int32_t g0 = blockIdx.x * BATCH_DEP_ROWS, g1 = min(g0 + BATCH_DEP_ROWS, b.total_rows);
+    for (int32_t g = g0; g < g1;) {
+        const int32_t inst = b.row_inst[g];
+        const InstMeta im = b.meta[inst];
+        const int32_t x0 = g - im.city_off, rows = min(g1 - g, im.n - x0);
+        deposit_rows_band<BATCH_DEP_ROWS>(batch_deposit_args(b, inst, im), x0, rows, false, sh);
+        g += rows;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3122,6 +3262,15 @@ hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s, con
     return hipGetLastError();
 }
 
+hipError_t launch_deposit_rows(const DepositArgs& a, hipStream_t s, const LaunchTiming& t) {
+    if (a.dense) return hipErrorInvalidValue;
+    if (a.n <= 4096)
+        launch_k(k_deposit_rows<16>, dim3(cdiv(a.n, 16)), dim3(1024), 0, s, t.start, t.stop, a);
+    else
+        launch_k(k_deposit_rows<256>, dim3(cdiv(a.n, 256)), dim3(1024), 0, s, t.start, t.stop, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_deposit_adj(const DepositArgs& a, hipStream_t s, const LaunchTiming& t) {
     if (!a.adj || !a.dense || a.n > ADJ_MAX_N) return hipErrorInvalidValue;
     const int32_t A = adj_stage_ants(a.n);
@@ -3211,9 +3360,10 @@ hipError_t launch_batch_iteration_kernel(const BatchBuffers& b, int32_t which, i
                     hipLaunchKernelGGL(kb_adj_gather<4>, dim3(cdiv(b.total_rows, 16)), dim3(256), glds, s, b, nmax);
                 else
                     hipLaunchKernelGGL(kb_adj_gather<1>, dim3(cdiv(b.total_rows, 4)), dim3(256), glds, s, b, nmax);
+            } else if (b.variant == 0) {
+                hipLaunchKernelGGL(kb_deposit, dim3(b.total_ants), dim3(256), 0, s, b);
             } else {
-                hipLaunchKernelGGL(kb_deposit, dim3(b.variant == 0 ? b.total_ants : b.count * (b.w + 1)), dim3(256),
-                                   0, s, b);
+                hipLaunchKernelGGL(kb_deposit_rows, dim3(cdiv(b.total_rows, BATCH_DEP_ROWS)), dim3(1024), 0, s, b);
             }
     }
     return hipGetLastError();

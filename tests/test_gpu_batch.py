@@ -284,6 +284,29 @@ def test_islands_exchange_equals_single_colonies_given_the_shared_best():
                 np.testing.assert_array_equal(c.best_tour(), best2[i])
 
 
+@pytest.mark.parametrize("w", [6, 11])
+def test_ranked_batch_equals_single_colonies(w):
+    """Rank-based AS: the batch deposit adds each instance's ranked tours in
+    list order, as the single colony does, so whole runs of uneven instances
+    (several sharing one 64-row deposit band, and w = 11 taking two deposit
+    passes) equal single colonies bit for bit."""
+    sizes = np.array([50, 57, 61, 130, 333], dtype=np.int32)
+    say(f"case: RBAS w={w} batch n={sizes.tolist()}, 15 iterations vs single colonies")
+    xys = [instances.uniform(int(n), 1000, seed=300 + i) for i, n in enumerate(sizes)]
+    kw = dict(variant=capi.VARIANT_RBAS, w=w, k=12, tau0_policy=capi.TAU0_RBAS, rho=0.1)
+    with capi.Batch(batch_params(**kw), sizes, xys) as b:
+        b.initialize()
+        b.run(15)
+        gb, best = b.best()
+        for i, n in enumerate(sizes):
+            with capi.Colony(batch_params(n=int(n), m=int(n), stream_id=i, **kw), xy=xys[i]) as c:
+                c.initialize()
+                c.run(15)
+                np.testing.assert_array_equal(b.tours(i), c.tours(), err_msg=f"instance {i}")
+                assert c.stats()[1] == gb[i]
+                np.testing.assert_array_equal(c.best_tour(), best[i])
+
+
 def test_run_islands_ends_with_a_shared_best():
     say("case: run_islands 8 islands n=120, 12 iterations, exchange every 4")
     xy = instances.uniform(120, 1000, seed=78)

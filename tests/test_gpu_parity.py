@@ -529,6 +529,62 @@ def test_deterministic_run_matches_oracle_run(mode, path):
         assert np.isfinite(tau).all()
 
 
+# w = 12 and 17: lists longer than one deposit pass (8 entries)
+@pytest.mark.parametrize("mode,path,aging,w", [(capi.MODE_CANDIDATE, 2, 0, 6), (capi.MODE_CANDIDATE, 2, 1, 6),
+                                               (capi.MODE_CANDIDATE, 1, 1, 6), (capi.MODE_FULLROW, 0, 1, 6),
+                                               (capi.MODE_CANDIDATE, 2, 0, 12), (capi.MODE_CANDIDATE, 2, 1, 17)])
+def test_ranked_run_matches_oracle_run(mode, path, aging, w):
+    """Rank-based AS: the deposit adds the w ranked tours to every pheromone
+    cell in list order, as the oracle's CPU loop does, so the whole run
+    (with aging: retirements, recruits and young ants) equals the oracle's
+    fp32 run bit for bit, and two runs on the GPU are identical."""
+    n, m, iters = 150, 40, 45
+    say(f"case: w={w} RBAS graph run {iters} iterations mode={mode} path={path} aging={aging}")
+    xy = instances.clustered(n, seed=21)
+    dist = orc.dist_from_int_coords(xy)
+    ag = dict(min_age=5, max_age=8, global_perf=3, min_pop=20)
+    p = capi.default_params(n=n, m=m, k=12, w=w, beta=3.0, rho=0.1, variant=capi.VARIANT_RBAS, mode=mode,
+                            aging=aging, seed=7, tau0=0.02, construct_path=path, **ag)
+    res = []
+    for _ in range(2):
+        with make(p, xy=xy) as col:
+            col.run(iters)
+            ib, gb, nv = col.stats()
+            res.append((ib, gb, col.best_tour(), col.pheromone()))
+    assert res[0][:2] == res[1][:2]
+    np.testing.assert_array_equal(res[0][3], res[1][3])
+    secs, ogb, oib, obest = orc.run(dist, m, variant=orc.RBAS,
+                                    mode=orc.MODE_CANDIDATE if mode == capi.MODE_CANDIDATE else orc.MODE_FULLROW,
+                                    k=12, w=w, aging=bool(aging), iterations=iters, alpha=1.0, beta=3.0, rho=0.1,
+                                    tau0=0.02, seed=7, **ag)
+    ib, gb, best, _ = res[0]
+    assert (gb, ib) == (ogb, oib)
+    np.testing.assert_array_equal(best, obest)
+
+
+def test_ranked_run_large_n_matches_oracle_run():
+    """n > 4096 cities: the ordered deposit in 256-row bands; a short whole
+    run equals the oracle's fp32 run, with one iteration also replayed
+    against the FP64 pheromone."""
+    n, m, iters = 5000, 16, 4
+    say(f"case: w=6 RBAS n={n} m={m} {iters} iterations vs oracle run")
+    xy = instances.uniform(n, 100000, seed=11)
+    dist = orc.dist_from_int_coords(xy)
+    p = capi.default_params(n=n, m=m, k=8, w=6, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS, seed=12,
+                            tau0=0.001)
+    with make(p, xy=xy) as col:
+        col.run(iters - 1)
+        snap = snapshot(col)
+        col.iterate()
+        check_iteration(col, p, dist, snap, replay(p, dist, snap))
+        ib, gb, _ = col.stats()
+        best = col.best_tour()
+    _, ogb, oib, obest = orc.run(dist, m, variant=orc.RBAS, mode=orc.MODE_CANDIDATE, k=8, w=6, iterations=iters,
+                                 alpha=1.0, beta=2.0, rho=0.2, tau0=0.001, seed=12)
+    assert (gb, ib) == (ogb, oib)
+    np.testing.assert_array_equal(best, obest)
+
+
 def test_graph_and_eager_paths_agree():
     say("case: graph vs eager, AS m=1, 6 iterations")
     xy = instances.uniform(90, 1000, seed=8)
