@@ -632,6 +632,32 @@ def test_c2_full_size_properties():
         check_iteration(col, p, dist, snap, ref)
 
 
+def test_c2_whole_run_matches_oracle_fixture():
+    """The n = 1000 rank-based + aging workload at full size (m = 1000,
+    k = 20, w = 6) through the hipGraph run path: stopped after 20, 820 and
+    2000 iterations, the GPU's best-so-far length, last iteration's best and
+    best-so-far tour equal the oracle's whole fp32 run
+    (golden/c2_oracle_run.json, written by golden/make_c2_oracle_run.py)."""
+    import json
+    import os
+
+    fx = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_oracle_run.json")))
+    say("case: C2 full size whole run vs oracle fixture, stops " + str([st["iterations"] for st in fx["stops"]]))
+    xy = instances.clustered(1000, clusters=10, side=10000, sigma=400.0, seed=1)
+    p = capi.default_params(n=1000, m=1000, k=20, w=6, alpha=1.0, beta=3.0, rho=0.1, variant=capi.VARIANT_RBAS,
+                            mode=capi.MODE_CANDIDATE, tau0_policy=capi.TAU0_RBAS, aging=1, min_age=30, max_age=50,
+                            local_perf=10, global_perf=10, min_pop=40, max_pop=100, seed=fx["seed"])
+    with make(p, xy=xy) as col:
+        assert col.pheromone()[0, 1] == np.float32(fx["tau0"])
+        done = 0
+        for st in fx["stops"]:
+            col.run(st["iterations"] - done)
+            done = st["iterations"]
+            ib, gb, _ = col.stats()
+            assert (gb, ib) == (st["glob_best"], st["iter_best"]), f"after {done} iterations"
+            np.testing.assert_array_equal(col.best_tour(), np.array(st["best_tour"], dtype=np.int32))
+
+
 def test_c3_full_size_properties():
     """The n = 20000 Ant System workload at full size (m = 1024, k = 32, row
     top lists on): every tour a permutation with its exact length, the best

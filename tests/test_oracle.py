@@ -410,6 +410,30 @@ def test_c1_refseq_fixture_reproduces():
     assert mk.refseq_best(dist, tau0, ref["seeds"][0]) == ref["best"][0]
 
 
+def test_c2_oracle_run_fixture_reproduces():
+    """The committed whole-run fixture of the n = 1000 rank-based + aging
+    workload is what the oracle computes (its 20-iteration stop re-run,
+    about 15 s), and every stop's best tour is a permutation whose EUC_2D
+    length is the recorded best."""
+    import json
+    import os
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_c2_oracle_run as mk
+
+    ref = json.load(open(os.path.join(here, "golden", "c2_oracle_run.json")))
+    _, dist, tau0 = mk.c2_instance()
+    assert np.float32(tau0) == np.float32(ref["tau0"])
+    for st in ref["stops"]:
+        t = np.array(st["best_tour"], dtype=np.int32)
+        assert sorted(t.tolist()) == list(range(mk.N))
+        assert instances.tour_length(dist, t) == st["glob_best"] <= st["iter_best"]
+    first = ref["stops"][0]
+    assert mk.oracle_stop(first["iterations"]) == first
+
+
 # ----------------------------------------------------------------- closed-form / brute-force pins
 @pytest.mark.parametrize("name", sorted(pins.CHECKS))
 def test_oracle_pins(name):
