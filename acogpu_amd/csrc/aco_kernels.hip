@@ -1912,9 +1912,139 @@ struct StatsAnts {
     bool IN[4], V[4];
 };
 
+// Rank-based statistics with one reduction pass: every wave leaves its
+// smallest max(w - 1, 1) keys and its aging candidates in LDS, then wave 0
+// takes the colony's smallest keys in order (the first is the iteration
+// best), updates the best-so-far, writes the deposit list (one division per
+// lane) and makes the aging decision; after one more barrier the block
+// copies an improved tour.  The same results as the per-phase reductions of
+// the Ant System path (src/Colony.cu:996-1043 order).
+template <typename K>
+__device__ __forceinline__ void stats_block_ranked(const StatsArgs& a, const StatsAnts& s, K* red, K* red_rank,
+                                                   StatsShared& sh) {
+    using SK = StatKey<K>;
+    DevState* st = a.state;
+    const int32_t B = blockDim.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = (int)(B >> 6);
+    const int32_t t = a.w - 1;     // ranked ants in the deposit list (<= RANK_MAX)
+    const int32_t tt = max(t, 1);  // keys taken per wave; the first is also the iteration best
+    K ks[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ks[q] = s.V[q] ? SK::pack(s.L[q], (int32_t)(threadIdx.x + q * B)) : SK::NONE;
+    for (int32_t r = 0; r < tt; ++r) {
+        const K wmin = SK::wave_min(min(min(ks[0], ks[1]), min(ks[2], ks[3])));
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ks[q] == wmin) ks[q] = SK::NONE;
+        if (lane == 0) red_rank[wave * RANK_MAX + r] = wmin;
+    }
+    // retireAnts / recruiteAnts candidates (see the Ant System path below)
+    uint32_t* red32 = reinterpret_cast<uint32_t*>(red);
+    if (a.aging) {
+        uint32_t cand = ~0u, inv = ~0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t k = threadIdx.x + q * B;
+            if ((int32_t)k < a.m / 2 && s.V[q] && s.AG[q] >= a.max_age) cand = min(cand, k);
+            if (s.IN[q] && !s.V[q]) inv = min(inv, k);
+        }
+        cand = wave_min_u32(cand);
+        inv = wave_min_u32(inv);
+        if (lane == 0) {
+            red32[wave] = cand;
+            red32[nw + wave] = inv;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const int32_t nc = nw * RANK_MAX;  // candidate slots, up to four per lane
+        K cs[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t x = lane + 64 * q;
+            cs[q] = (x < nc && (x % RANK_MAX) < tt) ? red_rank[x] : SK::NONE;
+        }
+        K mykey = SK::NONE, ib = SK::NONE;  // lane r: the r-th smallest key
+        int32_t found = 0;
+        for (int32_t r = 0; r < tt; ++r) {
+            const K wmin = SK::wave_min(min(min(cs[0], cs[1]), min(cs[2], cs[3])));
+            if (wmin == SK::NONE) break;  // fewer valid ants
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (cs[q] == wmin) cs[q] = SK::NONE;
+            mykey = lane == r ? wmin : mykey;
+            ib = r == 0 ? wmin : ib;
+            ++found;
+        }
+        // iterBestDist starts at MAX_DISTANCE - 1 and only a strictly smaller
+        // length is taken (src/Colony.cu:872,1015)
+        const bool fnd = ib != SK::NONE && SK::len(ib) < (int64_t)ACO_MAX_DISTANCE - 1;
+        const int64_t ibest = fnd ? SK::len(ib) : (int64_t)ACO_MAX_DISTANCE - 1;
+        const bool improved = fnd && ibest < sh.glob_best;
+        const int64_t gb = improved ? ibest : sh.glob_best;
+        const int32_t gperf = improved ? 0 : sh.gperf + 1;
+        // deposit list: the t best ranked ants with weights w - r, then the
+        // best-so-far tour with weight w
+        const int32_t nr = min(t, found);
+        if (lane < nr) {
+            a.dep_ant[lane] = SK::ant(mykey);
+            a.dep_val[lane] = (float)((double)(a.w - (lane + 1)) / (double)SK::len(mykey));
+        }
+        if (lane == 0) {
+            st->iter_best = ibest;
+            st->iter_best_ant = fnd ? SK::ant(ib) : -1;
+            st->glob_best = gb;
+            st->gperf = gperf;
+            sh.glob_best = gb;
+            sh.gperf = gperf;
+            sh.improved = improved ? SK::ant(ib) : -1;
+            int32_t nd = nr;
+            if (gb < (int64_t)ACO_MAX_DISTANCE) {
+                a.dep_ant[nd] = -1;
+                a.dep_val[nd] = (float)((double)a.w / (double)gb);
+                ++nd;
+            }
+            st->ndep = nd;
+            if (a.aging) {
+                uint32_t ret = red32[0], first_inv = red32[nw];
+                for (int w = 1; w < nw; ++w) {
+                    ret = min(ret, red32[w]);
+                    first_inv = min(first_inv, red32[nw + w]);
+                }
+                int32_t num_valid = sh.num_valid;
+                if (ret != ~0u) {
+                    a.valid[ret] = 0;
+                    num_valid -= 1;
+                    st->converged = 1;
+                    first_inv = min(first_inv, ret);
+                }
+                if ((gperf >= a.global_perf || num_valid < a.min_pop) && first_inv != ~0u) {
+                    a.valid[first_inv] = 1;
+                    a.age[first_inv] = 0;
+                    num_valid += 1;
+                }
+                sh.num_valid = num_valid;
+                st->num_valid = num_valid;
+            }
+            st->iter += 1;
+            // TSP::terminationCondition's optimum test (src/TSP.cpp:168-171)
+            if (st->stop_at >= 0 && ibest <= st->stop_at) st->halted = 1;
+        }
+    }
+    __syncthreads();
+    const int32_t imp = sh.improved;
+    if (imp >= 0)
+        for (int32_t i = threadIdx.x; i < a.n; i += B) a.best_tour[i] = a.tours[(size_t)imp * a.n + i];
+}
+
 template <typename K>
 __device__ __forceinline__ void stats_block_fast(const StatsArgs& a, const StatsAnts& s, K* red, K* red_rank,
                                                  StatsShared& sh) {
+    if (a.variant != 0) {
+        stats_block_ranked<K>(a, s, red, red_rank, sh);
+        return;
+    }
     using SK = StatKey<K>;
     DevState* st = a.state;
     const int32_t B = blockDim.x;
@@ -1950,57 +2080,11 @@ __device__ __forceinline__ void stats_block_fast(const StatsArgs& a, const Stats
     if (imp >= 0)
         for (int32_t i = threadIdx.x; i < a.n; i += B) a.best_tour[i] = a.tours[(size_t)imp * a.n + i];
 
-    if (a.variant == 0) {  // AS: dense, one entry per ant
+    // AS: dense, one entry per ant
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (s.IN[q]) a.dep_val[threadIdx.x + q * B] = V[q] ? (float)(1.0 / (double)s.L[q]) : 0.0f;
-        if (threadIdx.x == 0) st->ndep = a.m;
-    } else {
-        // rank-based: the w-1 smallest keys in two rounds (every wave its
-        // own, then wave 0 over the waves' candidates)
-        const int32_t t = a.w - 1;
-        K ks[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) ks[q] = V[q] ? SK::pack(s.L[q], (int32_t)(threadIdx.x + q * B)) : SK::NONE;
-        for (int32_t r = 0; r < t; ++r) {
-            const K wmin = SK::wave_min(min(min(ks[0], ks[1]), min(ks[2], ks[3])));
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (ks[q] == wmin) ks[q] = SK::NONE;
-            if (lane == 0) red_rank[wave * RANK_MAX + r] = wmin;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            const int32_t nq = (int32_t)(B >> 6) * RANK_MAX / 64;  // 1..4 candidates per lane
-            K cs[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int32_t x = lane + 64 * q;
-                cs[q] = (q < nq && (x % RANK_MAX) < t) ? red_rank[x] : SK::NONE;
-            }
-            int32_t nd = 0;
-            for (int32_t r = 0; r < t; ++r) {
-                const K wmin = SK::wave_min(min(min(cs[0], cs[1]), min(cs[2], cs[3])));
-                if (wmin == SK::NONE) break;  // fewer valid ants than w-1
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (cs[q] == wmin) cs[q] = SK::NONE;
-                if (lane == 0) {
-                    a.dep_ant[nd] = SK::ant(wmin);
-                    a.dep_val[nd] = (float)((double)(a.w - (r + 1)) / (double)SK::len(wmin));
-                }
-                ++nd;
-            }
-            if (lane == 0) {
-                if (sh.glob_best < (int64_t)ACO_MAX_DISTANCE) {
-                    a.dep_ant[nd] = -1;
-                    a.dep_val[nd] = (float)((double)a.w / (double)sh.glob_best);
-                    ++nd;
-                }
-                st->ndep = nd;
-            }
-        }
-    }
+    for (int q = 0; q < 4; ++q)
+        if (s.IN[q]) a.dep_val[threadIdx.x + q * B] = V[q] ? (float)(1.0 / (double)s.L[q]) : 0.0f;
+    if (threadIdx.x == 0) st->ndep = a.m;
 
     if (a.aging) {
         // retireAnts: the first valid ant among the first m/2 at max_age;
