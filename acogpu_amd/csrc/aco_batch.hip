@@ -366,7 +366,22 @@ extern "C" int aco_batch_create(const aco_params* p, int32_t count, const int32_
     B.seed_lo = (uint32_t)p->seed;
     B.seed_hi = (uint32_t)(p->seed >> 32);
     b->dep_elems = dep;
-    TRYB(build_items(b, p->ants_per_block ? p->ants_per_block : 8, p->ants_per_wave ? p->ants_per_wave : gmax));
+    // Default block: 8 waves; 16 when several ants share a wave (no per-wave
+    // LDS) and the candidate table leaves room for one block per CU, so the
+    // CU still runs four waves per SIMD (256 instances of n = 1000, k = 20:
+    // 17.1 -> 11.3 ms per construction).
+    int32_t W = p->ants_per_block;
+    const int32_t G = p->ants_per_wave ? p->ants_per_wave : gmax;
+    if (!W) {
+        int max_lds = 65536, dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            max_lds = 65536;
+        (void)hipGetLastError();
+        const size_t table = aco::batch_construct_lds_bytes(b->nmax, b->k, 16, G) + 256;
+        W = (G > 1 && 2 * table > (size_t)max_lds) ? 16 : 8;
+    }
+    TRYB(build_items(b, W, G));
     *out = b;
     return ACO_OK;
 #undef TRYB
