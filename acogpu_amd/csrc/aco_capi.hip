@@ -23,6 +23,7 @@ struct aco_ctx {
     int32_t n = 0, m = 0, k = 0, P = 0, words = 0, apb = 1;
     int32_t path = 0;          // construction kernel: 0 global, 1 full row, 2 LDS-staged
     int32_t young_blocks = 0;  // staged path: blocks reserved for young aging ants
+    bool fused = false;        // whole iterations launch statistics + deposit as one kernel
     int32_t max_lds = 0;       // LDS bytes per block on this device
     int32_t nn_cols = 0;  // k (candidate) or n (full row)
     hipStream_t stream = nullptr;
@@ -192,6 +193,7 @@ extern "C" int aco_create(const aco_params* p, const int32_t* dist, const int32_
     if (c->p.variant == ACO_VARIANT_RBAS && c->p.w > c->p.n) c->p.w = c->p.n;
     c->n = p->n;
     c->m = p->m;
+    c->fused = c->p.variant == ACO_VARIANT_RBAS && aco::stats_deposit_fusable(c->p.variant, c->n, c->m, c->p.w);
     c->k = p->mode == ACO_MODE_CANDIDATE ? p->k : 0;
     c->nn_cols = p->mode == ACO_MODE_CANDIDATE ? p->k : p->n;
     c->P = (c->n + 63) & ~63;
@@ -557,14 +559,7 @@ static hipError_t enqueue_construct(aco_ctx* c, int cur, const aco::LaunchTiming
     return hipSuccess;
 }
 
-static hipError_t enqueue_stats(aco_ctx* c, int cur, bool mirror, const aco::LaunchTiming& t) {
-    hipError_t e;
-    float* tau = c->d_tau[cur];
-    float* tau_next = c->d_tau[1 - cur];
-    const bool cand = c->p.mode == ACO_MODE_CANDIDATE;
-    (void)tau;
-    (void)tau_next;
-    (void)cand;
+static aco::StatsArgs stats_args(aco_ctx* c, bool mirror, const aco::LaunchTiming& t) {
     aco::StatsArgs st{};
     st.tours = c->d_tours;
     st.lengths = c->d_len;
@@ -586,14 +581,10 @@ static hipError_t enqueue_stats(aco_ctx* c, int cur, bool mirror, const aco::Lau
     st.global_perf = c->p.global_perf;
     st.min_pop = c->p.min_pop;
     st.host_stats = mirror ? c->h_stats : nullptr;
-    if ((e = aco::launch_stats(st, c->stream, t)) != hipSuccess) return e;
-    return hipSuccess;
+    return st;
 }
 
-// Ant System: the adjacency deposit where its buffer exists, else fp32
-// atomics; rank-based: the ordered row deposit.
-static hipError_t enqueue_deposit(aco_ctx* c, int cur, const aco::LaunchTiming& t) {
-    float* tau_next = c->d_tau[1 - cur];
+static aco::DepositArgs deposit_args(aco_ctx* c, int cur, const aco::LaunchTiming& t) {
     aco::DepositArgs da{};
     da.tours = c->d_tours;
     da.best_tour = c->d_best_tour;
@@ -601,16 +592,35 @@ static hipError_t enqueue_deposit(aco_ctx* c, int cur, const aco::LaunchTiming& 
     da.dep_val = c->d_dep_val;
     da.state = c->d_state;
     da.stamp = t.stamp;
-    da.tau_next = tau_next;
+    da.tau_next = c->d_tau[1 - cur];
     da.n = c->n;
     da.P = c->P;
     da.dense = c->p.variant == ACO_VARIANT_AS;
     da.adj = c->d_adj;
     da.m = c->m;
     da.list_cap = c->m + c->p.w + 2;
+    return da;
+}
+
+static hipError_t enqueue_stats(aco_ctx* c, bool mirror, const aco::LaunchTiming& t) {
+    return aco::launch_stats(stats_args(c, mirror, t), c->stream, t);
+}
+
+// Ant System: the adjacency deposit where its buffer exists, else fp32
+// atomics; rank-based: the ordered row deposit.
+static hipError_t enqueue_deposit(aco_ctx* c, int cur, const aco::LaunchTiming& t) {
+    const aco::DepositArgs da = deposit_args(c, cur, t);
     if (da.dense && c->d_adj) return aco::launch_deposit_adj(da, c->stream, t);
     if (!da.dense) return aco::launch_deposit_rows(da, c->stream, t);
     return aco::launch_deposit(da, c->m, c->stream, t);
+}
+
+// Rank-based colonies within aco::stats_deposit_fusable: statistics and
+// deposit as one launch (stamping both phases' slots: the statistics from
+// the kernel's start to block 0's deposit list, the deposit from there).
+static hipError_t enqueue_stats_deposit(aco_ctx* c, int cur, bool mirror, const aco::LaunchTiming& t) {
+    return aco::launch_stats_deposit(stats_args(c, mirror, t), deposit_args(c, cur, aco::LaunchTiming()),
+                                     c->stream, t);
 }
 
 // mirror: the statistics kernel also writes its results to h_stats; t: the
@@ -620,7 +630,7 @@ static hipError_t enqueue_phase(aco_ctx* c, int cur, int phase, bool mirror = fa
     switch (phase) {
         case ACO_PHASE_PROBABILITIES: return enqueue_sweep(c, cur, t);
         case ACO_PHASE_TOUR: return enqueue_construct(c, cur, t);
-        case ACO_PHASE_ANT_DISTANCE: return enqueue_stats(c, cur, mirror, t);
+        case ACO_PHASE_ANT_DISTANCE: return enqueue_stats(c, mirror, t);
         case ACO_PHASE_UPDATE_PHEROMONE: return enqueue_deposit(c, cur, t);
         default: return hipSuccess;  // evaporation is part of the probability sweep
     }
@@ -633,7 +643,11 @@ static hipError_t enqueue_iteration(aco_ctx* c, int cur, hipEvent_t* ev) {
     hipError_t e;
     if (ev && (e = hipEventRecord(ev[0], c->stream)) != hipSuccess) return e;
     for (int i = 0; i < 4; ++i) {
-        if ((e = enqueue_phase(c, cur, order[i])) != hipSuccess) return e;
+        if (c->fused && order[i] == ACO_PHASE_ANT_DISTANCE)
+            e = enqueue_stats_deposit(c, cur, false, aco::LaunchTiming());
+        else if (!(c->fused && order[i] == ACO_PHASE_UPDATE_PHEROMONE))
+            e = enqueue_phase(c, cur, order[i]);
+        if (e != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[i + 1], c->stream)) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -652,7 +666,10 @@ static hipError_t enqueue_timed_iteration(aco_ctx* c, int cur) {
     for (int i = 0; i < 4 && e == hipSuccess; ++i) {
         aco::LaunchTiming t;
         t.stamp = c->d_stamp + i;
-        e = enqueue_phase(c, cur, order[i], true, t);
+        if (c->fused && order[i] == ACO_PHASE_ANT_DISTANCE)
+            e = enqueue_stats_deposit(c, cur, true, t);  // also stamps the deposit's slot
+        else if (!(c->fused && order[i] == ACO_PHASE_UPDATE_PHEROMONE))
+            e = enqueue_phase(c, cur, order[i], true, t);
     }
     if (e == hipSuccess) e = aco::launch_stamp_end(c->d_stamp, c->h_stamp, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->tev_end, c->stream);

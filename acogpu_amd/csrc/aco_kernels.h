@@ -26,6 +26,7 @@ struct DevState {
     int32_t skip;            // the current iteration is skipped (the sweep copies halted)
     unsigned long long tours;       // tours constructed (valid ants of every iteration run)
     unsigned long long fb_listed;   // fallback steps answered by the row's top list
+    uint32_t fuse_count;     // blocks of the fused statistics + deposit done reading (0 between launches)
 };
 
 // Top list of a pheromone row (large instances, alpha = 1): the TOP_K cities
@@ -311,6 +312,13 @@ hipError_t launch_nn_aux(const int32_t* nn, const int32_t* dist, const float* et
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());
 hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s,
                           const LaunchTiming& t = LaunchTiming());
+// Rank-based statistics and deposit as one launch (stats_deposit_fusable):
+// the results of launch_stats followed by launch_deposit_rows.  s.stamp, when
+// set, receives the start time in s.stamp[0] and, in s.stamp[1], the time
+// block 0 has its deposit list (the statistics phase's end).
+bool stats_deposit_fusable(int32_t variant, int32_t n, int32_t m, int32_t w);
+hipError_t launch_stats_deposit(const StatsArgs& s, const DepositArgs& d, hipStream_t st,
+                                const LaunchTiming& t = LaunchTiming());
 // Rank-based deposit (list entries in order, one writer per cell, no
 // atomics): the same fp32 sums on every launch.
 hipError_t launch_deposit_rows(const DepositArgs& a, hipStream_t s, const LaunchTiming& t = LaunchTiming());

@@ -2276,16 +2276,84 @@ struct DepositRowsShared {
     float dv[DEP_PASS];
     const int32_t* tp[DEP_PASS];
 };
+// Phase 1 of a pass (entries 0 .. ne - 1 with weights dv and tours tp in
+// LDS): wave w takes positions 64 s + lane, s = w, w + nw, ..., of every
+// entry at once (DEP_PASS loads in flight per lane) and notes the band's
+// cities' neighbours in nb.  Every thread calls it.
+template <int ROWS>
+__device__ __forceinline__ void deposit_rows_scan(int32_t n, int32_t x0, int32_t rows, int32_t ne,
+                                                  DepositRowsShared<ROWS>& sh) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int32_t nseg = (n + 63) >> 6;
+    for (int32_t s0 = wave; s0 < nseg; s0 += nw) {
+        const int32_t i = (s0 << 6) + lane;
+        const bool last = lane == 63 || i == n - 1;  // successor outside the wave's 64
+        int32_t c[DEP_PASS], pe[DEP_PASS], se[DEP_PASS];
+#pragma unroll
+        for (int e = 0; e < DEP_PASS; ++e) {
+            const bool in = e < ne && i < n && sh.dv[e] != 0.0f;
+            const int32_t* t = sh.tp[e];
+            c[e] = in ? t[i] : -1;
+            pe[e] = (in && lane == 0) ? t[i == 0 ? n - 1 : i - 1] : -1;
+            se[e] = (in && last) ? t[i + 1 == n ? 0 : i + 1] : -1;
+        }
+#pragma unroll
+        for (int e = 0; e < DEP_PASS; ++e) {
+            const int32_t up = __shfl_up(c[e], 1), down = __shfl_down(c[e], 1);
+            const int32_t r = c[e] - x0;
+            if (c[e] >= 0 && (uint32_t)r < (uint32_t)rows) {
+                sh.nb[e][r][0] = lane == 0 ? pe[e] : up;
+                sh.nb[e][r][1] = last ? se[e] : down;
+            }
+        }
+    }
+}
+
+// Phase 2 of a pass: four lanes per row; the row's 2 ne (target, deposit)
+// slots; a target's first slot loads the cell, sums every slot naming it in
+// list order and stores it; the row's k-th first slot is lane k mod 4's.
+template <int ROWS>
+__device__ __forceinline__ void deposit_rows_cells(float* tau_next, int32_t P, int32_t x0, int32_t rows, int32_t ne,
+                                                   const DepositRowsShared<ROWS>& sh) {
+    if ((int32_t)threadIdx.x >= 4 * rows) return;
+    const int32_t r = threadIdx.x >> 2, sub = threadIdx.x & 3;
+    float* row = tau_next + (size_t)(x0 + r) * P;
+    int32_t j[2 * DEP_PASS];
+    float d[2 * DEP_PASS], v[2 * DEP_PASS];
+#pragma unroll
+    for (int q = 0; q < 2 * DEP_PASS; ++q) {
+        const int e = q >> 1;
+        const bool on = e < ne && sh.dv[e] != 0.0f;
+        j[q] = on ? sh.nb[e][r][q & 1] : -1;
+        d[q] = on ? sh.dv[e] : 0.0f;
+    }
+    uint32_t mine = 0;
+    int32_t cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 2 * DEP_PASS; ++q) {
+        bool f = j[q] >= 0;
+#pragma unroll
+        for (int q2 = 0; q2 < q; ++q2) f = f && j[q2] != j[q];
+        mine |= (uint32_t)(f && (cnt & 3) == sub) << q;
+        cnt += f;
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * DEP_PASS; ++q) v[q] = ((mine >> q) & 1) ? row[j[q]] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 2 * DEP_PASS; ++q) {
+        if (!((mine >> q) & 1)) continue;
+        float s = v[q];
+#pragma unroll
+        for (int q2 = q; q2 < 2 * DEP_PASS; ++q2)
+            if (j[q2] == j[q]) s += d[q2];
+        row[j[q]] = s;
+    }
+}
+
 template <int ROWS>
 __device__ __forceinline__ void deposit_rows_band(const DepositArgs& a, int32_t x0, int32_t rows, bool check_skip,
                                                   DepositRowsShared<ROWS>& sh) {
     static_assert(4 * ROWS <= 1024, "four lanes per row");
-    int32_t(&nb)[DEP_PASS][ROWS][2] = sh.nb;
-    float(&dv)[DEP_PASS] = sh.dv;
-    const int32_t*(&tp)[DEP_PASS] = sh.tp;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int32_t n = a.n;
-    const int32_t nseg = (n + 63) >> 6;
     // the first pass's list slots are read with the list length and the skip
     // flag, in one round trip (slots past ndep are read but never used)
     int32_t ant = -1;
@@ -2303,72 +2371,13 @@ __device__ __forceinline__ void deposit_rows_band(const DepositArgs& a, int32_t 
             dval = a.dep_val[e0 + threadIdx.x];
         }
         if ((int32_t)threadIdx.x < ne) {
-            dv[threadIdx.x] = dval;
-            tp[threadIdx.x] = ant >= 0 ? a.tours + (size_t)ant * n : a.best_tour;
+            sh.dv[threadIdx.x] = dval;
+            sh.tp[threadIdx.x] = ant >= 0 ? a.tours + (size_t)ant * a.n : a.best_tour;
         }
         __syncthreads();
-        // wave w takes positions 64 s + lane, s = w, w + 16, ..., of every
-        // entry of the pass at once (DEP_PASS loads in flight per lane)
-        for (int32_t s0 = wave; s0 < nseg; s0 += nw) {
-            const int32_t i = (s0 << 6) + lane;
-            const bool last = lane == 63 || i == n - 1;  // successor outside the wave's 64
-            int32_t c[DEP_PASS], pe[DEP_PASS], se[DEP_PASS];
-#pragma unroll
-            for (int e = 0; e < DEP_PASS; ++e) {
-                const bool in = e < ne && i < n && dv[e] != 0.0f;
-                const int32_t* t = tp[e];
-                c[e] = in ? t[i] : -1;
-                pe[e] = (in && lane == 0) ? t[i == 0 ? n - 1 : i - 1] : -1;
-                se[e] = (in && last) ? t[i + 1 == n ? 0 : i + 1] : -1;
-            }
-#pragma unroll
-            for (int e = 0; e < DEP_PASS; ++e) {
-                const int32_t up = __shfl_up(c[e], 1), down = __shfl_down(c[e], 1);
-                const int32_t r = c[e] - x0;
-                if (c[e] >= 0 && (uint32_t)r < (uint32_t)rows) {
-                    nb[e][r][0] = lane == 0 ? pe[e] : up;
-                    nb[e][r][1] = last ? se[e] : down;
-                }
-            }
-        }
+        deposit_rows_scan<ROWS>(a.n, x0, rows, ne, sh);
         __syncthreads();
-        if ((int32_t)threadIdx.x < 4 * rows) {
-            // the row's 2 ne (target, deposit) slots; a target's first slot
-            // loads the cell, sums every slot naming it in list order and
-            // stores it; the row's k-th first slot is lane k mod 4's
-            const int32_t r = threadIdx.x >> 2, sub = threadIdx.x & 3;
-            float* row = a.tau_next + (size_t)(x0 + r) * a.P;
-            int32_t j[2 * DEP_PASS];
-            float d[2 * DEP_PASS], v[2 * DEP_PASS];
-#pragma unroll
-            for (int q = 0; q < 2 * DEP_PASS; ++q) {
-                const int e = q >> 1;
-                const bool on = e < ne && dv[e] != 0.0f;
-                j[q] = on ? nb[e][r][q & 1] : -1;
-                d[q] = on ? dv[e] : 0.0f;
-            }
-            uint32_t mine = 0;
-            int32_t cnt = 0;
-#pragma unroll
-            for (int q = 0; q < 2 * DEP_PASS; ++q) {
-                bool f = j[q] >= 0;
-#pragma unroll
-                for (int q2 = 0; q2 < q; ++q2) f = f && j[q2] != j[q];
-                mine |= (uint32_t)(f && (cnt & 3) == sub) << q;
-                cnt += f;
-            }
-#pragma unroll
-            for (int q = 0; q < 2 * DEP_PASS; ++q) v[q] = ((mine >> q) & 1) ? row[j[q]] : 0.0f;
-#pragma unroll
-            for (int q = 0; q < 2 * DEP_PASS; ++q) {
-                if (!((mine >> q) & 1)) continue;
-                float s = v[q];
-#pragma unroll
-                for (int q2 = q; q2 < 2 * DEP_PASS; ++q2)
-                    if (j[q2] == j[q]) s += d[q2];
-                row[j[q]] = s;
-            }
-        }
+        deposit_rows_cells<ROWS>(a.tau_next, a.P, x0, rows, ne, sh);
         __syncthreads();
     }
 }
@@ -2381,6 +2390,218 @@ __global__ __launch_bounds__(1024) void k_deposit_rows(const DepositArgs a) {
     __shared__ DepositRowsShared<ROWS> sh;
     const int32_t x0 = blockIdx.x * ROWS;
     deposit_rows_band<ROWS>(a, x0, min(ROWS, a.n - x0), true, sh);
+}
+
+// Rank-based statistics and deposit in one launch (m <= 1024, w <= DEP_PASS,
+// n <= 4096; 1024-thread blocks over 16-row bands).  Every block ranks the
+// ants itself (one key per thread, the same keys and order as
+// stats_block_ranked) and builds the deposit list in LDS; the best-so-far
+// entry is the iteration-best ant's tour when this iteration improves the
+// best, else the stored best tour.  After scanning its band's tours a block
+// counts itself done reading (fuse_count); the block that completes the
+// count, when every block has read the iteration's lengths, validity,
+// tours, best tour and state, makes the statistics' writes: tours counter,
+// per-ant stagnation, iteration best and best-so-far, the best tour's copy,
+// the deposit list, aging and the iteration number.  Every block adds its
+// band's deposits.  One launch and one list computation per block instead
+// of a one-block statistics launch ahead of the deposit's.
+struct FusedShared {
+    uint64_t red_rank[16 * RANK_MAX];  // per-wave candidates (K-wide)
+    uint32_t red32[32];                // aging candidates
+    int64_t ibest, gb;
+    int32_t ib_ant, improved_ant, gperf, ndep, last;
+    int32_t list_ant[DEP_PASS];
+};
+
+template <typename K>
+__device__ __forceinline__ void fused_rank_list(const StatsArgs& a, int64_t L, bool V, int64_t gb_old,
+                                                int32_t gperf_old, FusedShared& fs, DepositRowsShared<16>& sh) {
+    using SK = StatKey<K>;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = (int)(blockDim.x >> 6);
+    K* red_rank = reinterpret_cast<K*>(fs.red_rank);
+    const int32_t t = a.w - 1, tt = max(t, 1);
+    K k0 = V ? SK::pack(L, (int32_t)threadIdx.x) : SK::NONE;
+    for (int32_t r = 0; r < tt; ++r) {
+        const K wmin = SK::wave_min(k0);
+        k0 = k0 == wmin ? SK::NONE : k0;
+        if (lane == 0) red_rank[wave * RANK_MAX + r] = wmin;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const int32_t nc = nw * RANK_MAX;
+        K cs[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t x = lane + 64 * q;
+            cs[q] = (x < nc && (x % RANK_MAX) < tt) ? red_rank[x] : SK::NONE;
+        }
+        K mykey = SK::NONE, ib = SK::NONE;
+        int32_t found = 0;
+        for (int32_t r = 0; r < tt; ++r) {
+            const K wmin = SK::wave_min(min(min(cs[0], cs[1]), min(cs[2], cs[3])));
+            if (wmin == SK::NONE) break;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (cs[q] == wmin) cs[q] = SK::NONE;
+            mykey = lane == r ? wmin : mykey;
+            ib = r == 0 ? wmin : ib;
+            ++found;
+        }
+        const bool fnd = ib != SK::NONE && SK::len(ib) < (int64_t)ACO_MAX_DISTANCE - 1;
+        const int64_t ibest = fnd ? SK::len(ib) : (int64_t)ACO_MAX_DISTANCE - 1;
+        const bool improved = fnd && ibest < gb_old;
+        const int64_t gb = improved ? ibest : gb_old;
+        const int32_t nr = min(t, found);
+        if (lane < nr) {
+            const int32_t ant = SK::ant(mykey);
+            sh.dv[lane] = (float)((double)(a.w - (lane + 1)) / (double)SK::len(mykey));
+            sh.tp[lane] = a.tours + (size_t)ant * a.n;
+            fs.list_ant[lane] = ant;
+        }
+        if (lane == 0) {
+            int32_t nd = nr;
+            if (gb < (int64_t)ACO_MAX_DISTANCE) {
+                sh.dv[nd] = (float)((double)a.w / (double)gb);
+                sh.tp[nd] = improved ? a.tours + (size_t)SK::ant(ib) * a.n : a.best_tour;
+                fs.list_ant[nd] = -1;
+                ++nd;
+            }
+            fs.ndep = nd;
+            fs.ibest = ibest;
+            fs.ib_ant = fnd ? SK::ant(ib) : -1;
+            fs.improved_ant = improved ? SK::ant(ib) : -1;
+            fs.gb = gb;
+            fs.gperf = improved ? 0 : gperf_old + 1;
+        }
+    }
+    __syncthreads();
+}
+
+// The statistics' writes (src/Colony.cu:996-1043 order), by the last block
+// of k_stats_deposit; L, V: this thread's ant as read at the start.
+// Per-thread values every block loads with its ant's length (only the last
+// block uses them): no memory round trip before the statistics' writes.
+struct FusedPre {
+    int64_t prev_len;
+    int32_t perf, age, num_valid, tour_word;  // tour_word: entry tid of the improved ant's tour
+};
+
+__device__ __forceinline__ void fused_stats_writes(const StatsArgs& a, int64_t L, bool V, const FusedPre& pre,
+                                                   FusedShared& fs, const DepositRowsShared<16>& sh) {
+    DevState* st = a.state;
+    const int32_t tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, nw = (int)(blockDim.x >> 6);
+    const bool in = tid < a.m;
+    // the ants that built a tour this iteration: the valid ones, before aging
+    if (tid == 0) st->tours += a.aging ? (unsigned long long)pre.num_valid : (unsigned long long)a.m;
+    if (a.aging && V) {  // Colony::antperformance
+        a.perf[tid] = L == pre.prev_len ? pre.perf + 1 : 0;
+        a.prev_len[tid] = L;
+    }
+    if (tid == 0) {
+        st->iter_best = fs.ibest;
+        st->iter_best_ant = fs.ib_ant;
+        st->glob_best = fs.gb;
+        st->gperf = fs.gperf;
+        st->ndep = fs.ndep;
+    }
+    if (tid < fs.ndep) {
+        a.dep_ant[tid] = fs.list_ant[tid];
+        a.dep_val[tid] = sh.dv[tid];
+    }
+    const int32_t imp = fs.improved_ant;
+    if (imp >= 0) {
+        if (tid < a.n) a.best_tour[tid] = pre.tour_word;
+        for (int32_t i = tid + blockDim.x; i < a.n; i += blockDim.x) a.best_tour[i] = a.tours[(size_t)imp * a.n + i];
+    }
+    if (a.aging) {  // retireAnts / recruiteAnts, as stats_block_ranked
+        const int32_t ag = pre.age;
+        uint32_t cand = (tid < a.m / 2 && V && ag >= a.max_age) ? (uint32_t)tid : ~0u;
+        uint32_t inv = (in && !V) ? (uint32_t)tid : ~0u;
+        cand = wave_min_u32(cand);
+        inv = wave_min_u32(inv);
+        if (lane == 0) {
+            fs.red32[wave] = cand;
+            fs.red32[nw + wave] = inv;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t ret = fs.red32[0], first_inv = fs.red32[nw];
+            for (int w = 1; w < nw; ++w) {
+                ret = min(ret, fs.red32[w]);
+                first_inv = min(first_inv, fs.red32[nw + w]);
+            }
+            int32_t num_valid = pre.num_valid;
+            if (ret != ~0u) {
+                a.valid[ret] = 0;
+                num_valid -= 1;
+                st->converged = 1;
+                first_inv = min(first_inv, ret);
+            }
+            if ((fs.gperf >= a.global_perf || num_valid < a.min_pop) && first_inv != ~0u) {
+                a.valid[first_inv] = 1;
+                a.age[first_inv] = 0;
+                num_valid += 1;
+            }
+            st->num_valid = num_valid;
+        }
+    }
+    if (tid == 0) {
+        st->iter += 1;
+        // TSP::terminationCondition's optimum test (src/TSP.cpp:168-171)
+        if (st->stop_at >= 0 && fs.ibest <= st->stop_at) st->halted = 1;
+        if (a.host_stats != nullptr) {
+            a.host_stats[0] = fs.ibest;
+            a.host_stats[1] = fs.gb;
+            a.host_stats[2] = a.aging ? (int64_t)st->num_valid : (int64_t)a.m;
+            __threadfence_system();
+        }
+        st->fuse_count = 0;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_stats_deposit(const StatsArgs a, const DepositArgs d) {
+    if (a.stamp != nullptr && blockIdx.x == 0 && threadIdx.x == 0) a.stamp[0] = wall_clock64();
+    __shared__ DepositRowsShared<16> sh;
+    __shared__ FusedShared fs;
+    DevState* st = a.state;
+    const int32_t tid = threadIdx.x;
+    const bool in = tid < a.m;
+    FusedPre pre{};
+    const int64_t L = in ? a.lengths[tid] : 0;
+    const bool vl = in && (!a.aging || a.valid[tid]);
+    if (a.aging && in) {
+        pre.prev_len = a.prev_len[tid];
+        pre.perf = a.perf[tid];
+        pre.age = a.age[tid];
+    }
+    const int64_t gb_old = st->glob_best;
+    const int32_t gperf_old = st->gperf;
+    pre.num_valid = st->num_valid;
+    if (st->skip) return;
+    const bool V = vl;
+    const bool wide = __syncthreads_or((V && (L < 0 || L >= StatKey<uint32_t>::LEN_LIMIT)) ? 1 : 0) != 0;
+    if (wide)
+        fused_rank_list<uint64_t>(a, L, V, gb_old, gperf_old, fs, sh);
+    else
+        fused_rank_list<uint32_t>(a, L, V, gb_old, gperf_old, fs, sh);
+    // the deposit phase's time from block 0 holding its list
+    if (a.stamp != nullptr && blockIdx.x == 0 && tid == 0) a.stamp[1] = wall_clock64();
+    // the improved ant's tour, requested now for the last block's copy
+    if (fs.improved_ant >= 0 && tid < a.n) pre.tour_word = a.tours[(size_t)fs.improved_ant * a.n + tid];
+    const int32_t ne = fs.ndep;
+    const int32_t x0 = blockIdx.x * 16, rows = min(16, a.n - x0);
+    deposit_rows_scan<16>(a.n, x0, rows, ne, sh);
+    __syncthreads();  // this block's reads of the iteration's inputs are over
+    if (tid == 0) {
+        __threadfence();
+        fs.last = atomicAdd(&st->fuse_count, 1u) == gridDim.x - 1;
+        __threadfence();
+    }
+    __syncthreads();
+    deposit_rows_cells<16>(d.tau_next, d.P, x0, rows, ne, sh);
+    if (fs.last) fused_stats_writes(a, L, V, pre, fs, sh);
 }
 
 // Ant System deposit in two passes with no global atomics; the same sum as
@@ -3343,6 +3564,17 @@ hipError_t launch_stats(const StatsArgs& a, hipStream_t s, const LaunchTiming& t
 
 hipError_t launch_deposit(const DepositArgs& a, int32_t grid, hipStream_t s, const LaunchTiming& t) {
     launch_k(k_deposit, dim3(grid), dim3(256), 0, s, t.start, t.stop, a);
+    return hipGetLastError();
+}
+
+bool stats_deposit_fusable(int32_t variant, int32_t n, int32_t m, int32_t w) {
+    return variant != 0 && n >= 2 && n <= 4096 && m >= 1 && m <= 1024 && w >= 1 && w <= DEP_PASS &&
+           w - 1 <= RANK_MAX;
+}
+
+hipError_t launch_stats_deposit(const StatsArgs& s, const DepositArgs& d, hipStream_t st, const LaunchTiming& t) {
+    if (!stats_deposit_fusable(s.variant, s.n, s.m, s.w) || d.dense) return hipErrorInvalidValue;
+    launch_k(k_stats_deposit, dim3(cdiv(s.n, 16)), dim3(1024), 0, st, t.start, t.stop, s, d);
     return hipGetLastError();
 }
 
