@@ -2414,7 +2414,7 @@ struct FusedShared {
 };
 
 template <typename K>
-__device__ __forceinline__ void fused_rank_list(const StatsArgs& a, int64_t L, bool V, int64_t gb_old,
+__device__ __forceinline__ void fused_rank_list(const StatsArgs& a, int64_t L, bool V, int32_t age, int64_t gb_old,
                                                 int32_t gperf_old, FusedShared& fs, DepositRowsShared<16>& sh) {
     using SK = StatKey<K>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2426,6 +2426,18 @@ __device__ __forceinline__ void fused_rank_list(const StatsArgs& a, int64_t L, b
         const K wmin = SK::wave_min(k0);
         k0 = k0 == wmin ? SK::NONE : k0;
         if (lane == 0) red_rank[wave * RANK_MAX + r] = wmin;
+    }
+    if (a.aging) {  // retireAnts / recruiteAnts candidates, for the last block
+        const int32_t tid = threadIdx.x;
+        const bool in = tid < a.m;
+        uint32_t cand = (tid < a.m / 2 && V && age >= a.max_age) ? (uint32_t)tid : ~0u;
+        uint32_t inv = (in && !V) ? (uint32_t)tid : ~0u;
+        cand = wave_min_u32(cand);
+        inv = wave_min_u32(inv);
+        if (lane == 0) {
+            fs.red32[wave] = cand;
+            fs.red32[nw + wave] = inv;
+        }
     }
     __syncthreads();
     if (wave == 0) {
@@ -2483,7 +2495,9 @@ __device__ __forceinline__ void fused_rank_list(const StatsArgs& a, int64_t L, b
 // Per-thread values every block loads with its ant's length (only the last
 // block uses them): no memory round trip before the statistics' writes.
 struct FusedPre {
-    int64_t prev_len;
+    int64_t prev_len, stop_at;
+    unsigned long long tours;
+    uint32_t iter;
     int32_t perf, age, num_valid, tour_word;  // tour_word: entry tid of the improved ant's tour
 };
 
@@ -2491,10 +2505,9 @@ __device__ __forceinline__ void fused_stats_writes(const StatsArgs& a, int64_t L
                                                    FusedShared& fs, const DepositRowsShared<16>& sh) {
     DevState* st = a.state;
     const int32_t tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63, nw = (int)(blockDim.x >> 6);
-    const bool in = tid < a.m;
+    const int nw = (int)(blockDim.x >> 6);
     // the ants that built a tour this iteration: the valid ones, before aging
-    if (tid == 0) st->tours += a.aging ? (unsigned long long)pre.num_valid : (unsigned long long)a.m;
+    if (tid == 0) st->tours = pre.tours + (a.aging ? (unsigned long long)pre.num_valid : (unsigned long long)a.m);
     if (a.aging && V) {  // Colony::antperformance
         a.perf[tid] = L == pre.prev_len ? pre.perf + 1 : 0;
         a.prev_len[tid] = L;
@@ -2515,17 +2528,7 @@ __device__ __forceinline__ void fused_stats_writes(const StatsArgs& a, int64_t L
         if (tid < a.n) a.best_tour[tid] = pre.tour_word;
         for (int32_t i = tid + blockDim.x; i < a.n; i += blockDim.x) a.best_tour[i] = a.tours[(size_t)imp * a.n + i];
     }
-    if (a.aging) {  // retireAnts / recruiteAnts, as stats_block_ranked
-        const int32_t ag = pre.age;
-        uint32_t cand = (tid < a.m / 2 && V && ag >= a.max_age) ? (uint32_t)tid : ~0u;
-        uint32_t inv = (in && !V) ? (uint32_t)tid : ~0u;
-        cand = wave_min_u32(cand);
-        inv = wave_min_u32(inv);
-        if (lane == 0) {
-            fs.red32[wave] = cand;
-            fs.red32[nw + wave] = inv;
-        }
-        __syncthreads();
+    if (a.aging) {  // retireAnts / recruiteAnts (candidates from fused_rank_list), as stats_block_ranked
         if (tid == 0) {
             uint32_t ret = fs.red32[0], first_inv = fs.red32[nw];
             for (int w = 1; w < nw; ++w) {
@@ -2548,9 +2551,9 @@ __device__ __forceinline__ void fused_stats_writes(const StatsArgs& a, int64_t L
         }
     }
     if (tid == 0) {
-        st->iter += 1;
+        st->iter = pre.iter + 1;
         // TSP::terminationCondition's optimum test (src/TSP.cpp:168-171)
-        if (st->stop_at >= 0 && fs.ibest <= st->stop_at) st->halted = 1;
+        if (pre.stop_at >= 0 && fs.ibest <= pre.stop_at) st->halted = 1;
         if (a.host_stats != nullptr) {
             a.host_stats[0] = fs.ibest;
             a.host_stats[1] = fs.gb;
@@ -2579,13 +2582,16 @@ __global__ __launch_bounds__(1024) void k_stats_deposit(const StatsArgs a, const
     const int64_t gb_old = st->glob_best;
     const int32_t gperf_old = st->gperf;
     pre.num_valid = st->num_valid;
+    pre.tours = st->tours;
+    pre.stop_at = st->stop_at;
+    pre.iter = st->iter;
     if (st->skip) return;
     const bool V = vl;
     const bool wide = __syncthreads_or((V && (L < 0 || L >= StatKey<uint32_t>::LEN_LIMIT)) ? 1 : 0) != 0;
     if (wide)
-        fused_rank_list<uint64_t>(a, L, V, gb_old, gperf_old, fs, sh);
+        fused_rank_list<uint64_t>(a, L, V, pre.age, gb_old, gperf_old, fs, sh);
     else
-        fused_rank_list<uint32_t>(a, L, V, gb_old, gperf_old, fs, sh);
+        fused_rank_list<uint32_t>(a, L, V, pre.age, gb_old, gperf_old, fs, sh);
     // the deposit phase's time from block 0 holding its list
     if (a.stamp != nullptr && blockIdx.x == 0 && tid == 0) a.stamp[1] = wall_clock64();
     // the improved ant's tour, requested now for the last block's copy
@@ -2593,14 +2599,19 @@ __global__ __launch_bounds__(1024) void k_stats_deposit(const StatsArgs a, const
     const int32_t ne = fs.ndep;
     const int32_t x0 = blockIdx.x * 16, rows = min(16, a.n - x0);
     deposit_rows_scan<16>(a.n, x0, rows, ne, sh);
-    __syncthreads();  // this block's reads of the iteration's inputs are over
-    if (tid == 0) {
-        __threadfence();
-        fs.last = atomicAdd(&st->fuse_count, 1u) == gridDim.x - 1;
-        __threadfence();
-    }
+    // This block's reads of the iteration's inputs are over (their values
+    // sit in LDS and registers).  The count needs no fence: no store of the
+    // last block can reach a read that has returned.  The block's last
+    // thread counts (its wave adds no deposits: the 64 lanes of wave 0 do),
+    // so the atomic's round trip overlaps the deposits instead of queueing
+    // ahead of their loads.
     __syncthreads();
+    const bool counter = tid == (int32_t)blockDim.x - 1;
+    uint32_t ticket = 0;
+    if (counter) ticket = atomicAdd(&st->fuse_count, 1u);
     deposit_rows_cells<16>(d.tau_next, d.P, x0, rows, ne, sh);
+    if (counter) fs.last = ticket == gridDim.x - 1;
+    __syncthreads();
     if (fs.last) fused_stats_writes(a, L, V, pre, fs, sh);
 }
 
