@@ -562,6 +562,31 @@ def test_ranked_run_matches_oracle_run(mode, path, aging, w):
     np.testing.assert_array_equal(best, obest)
 
 
+@pytest.mark.parametrize("n,m,w,aging", [(2, 3, 2, 0), (3, 4, 3, 1), (5, 6, 5, 0), (17, 9, 8, 1), (40, 1, 4, 0),
+                                         (64, 70, 8, 1)])
+def test_ranked_run_small_matches_oracle_run(n, m, w, aging):
+    """Small rank-based colonies through whole graph-replayed iterations:
+    one deposit band (n < 16), lists of w = 8 (the most one fused statistics
+    and deposit launch takes), a single ant, aging; equal to the oracle's
+    fp32 run bit for bit."""
+    iters = 30
+    k = min(12, n - 1)
+    say(f"case: RBAS n={n} m={m} w={w} aging={aging} k={k}, {iters} iterations vs oracle run")
+    xy = instances.uniform(n, 1000, seed=60 + n)
+    dist = orc.dist_from_int_coords(xy)
+    ag = dict(min_age=3, max_age=6, global_perf=2, min_pop=max(1, m // 2))
+    p = capi.default_params(n=n, m=m, k=k, w=w, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS, aging=aging,
+                            seed=13, tau0=0.05, **ag)
+    with make(p, xy=xy) as col:
+        col.run(iters)
+        ib, gb, _ = col.stats()
+        best = col.best_tour()
+    _, ogb, oib, obest = orc.run(dist, m, variant=orc.RBAS, mode=orc.MODE_CANDIDATE, k=k, w=w, aging=bool(aging),
+                                 iterations=iters, alpha=1.0, beta=2.0, rho=0.2, tau0=0.05, seed=13, **ag)
+    assert (gb, ib) == (ogb, oib)
+    np.testing.assert_array_equal(best, obest)
+
+
 def test_ranked_run_large_n_matches_oracle_run():
     """n > 4096 cities: the ordered deposit in 256-row bands; a short whole
     run equals the oracle's fp32 run, with one iteration also replayed
