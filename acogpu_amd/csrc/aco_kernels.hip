@@ -2215,9 +2215,12 @@ __global__ __launch_bounds__(1024) void k_stats(const StatsArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Deposit: one block per list entry, fp32 atomics on both (i,j) and (j,i) of
-// every tour edge (dev_update_pheromones src/AntSystem.cu:105-128, which only
-// updated one direction; the CPU branch src/AntSystem.cu:296-316 is symmetric).
+// Ant System deposit with fp32 atomics (the path the n = 20000 workload
+// names, and AS beyond the adjacency deposit's size): one block per list
+// entry, atomics on both (i,j) and (j,i) of every tour edge
+// (dev_update_pheromones src/AntSystem.cu:105-128, which only updated one
+// direction; the CPU branch src/AntSystem.cu:296-316 is symmetric); the
+// rank-based deposit adds its tours in list order instead (k_deposit_rows).
 // One depositing tour per block.  The iteration state and the block's
 // deposit entry are read together (the entry arrays hold m + w + 2 slots per
 // colony, so every block's slot exists), and each thread loads the ends of
