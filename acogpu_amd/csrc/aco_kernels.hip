@@ -416,7 +416,10 @@ __device__ __forceinline__ void sweep_row_end(const SweepArgs& a, int32_t i, int
 
 // Columns g0 .. g0 + 1023 of row i, any n (columns >= n are skipped): the
 // batch's sweep and the last chunk of a single colony's rows.
-__device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int32_t i, int32_t g0, int lane, SweepRow& r) {
+// halt: the iteration is skipped; tested after the chunk's loads are issued
+// and before anything is stored (false: nothing stored).
+__device__ __forceinline__ bool sweep_chunk(const SweepArgs& a, int32_t i, int32_t g0, int lane, SweepRow& r,
+                                            bool halt = false) {
     // Distances from the (L2-resident) coordinates instead of the matrix
     // row: 4n bytes of HBM reads become 8n bytes of L2 reads.
     const bool from_xy = r.erow == nullptr && a.xy != nullptr;
@@ -470,6 +473,7 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int32_t i, int32
             }
         }
     }
+    if (halt) return false;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const int32_t j0 = g0 + 256 * s + 4 * lane;
@@ -501,6 +505,7 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int32_t i, int32
             }
         }
     }
+    return true;
 }
 
 __device__ __forceinline__ void sweep_row(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf) {
@@ -546,7 +551,8 @@ __device__ __forceinline__ float eta_pow_fast(int32_t d, int32_t eta_len) {
 // a table gather (PW 0).  Same values, stores and row-sum order as
 // sweep_chunk.
 template <int SRC, int PW, bool ALPHA1>
-__device__ __forceinline__ void sweep_chunk_full(const SweepArgs& a, int32_t i, int32_t g0, int lane, SweepRow& r) {
+__device__ __forceinline__ bool sweep_chunk_full(const SweepArgs& a, int32_t i, int32_t g0, int lane, SweepRow& r,
+                                                 bool halt = false) {
     // Non-temporal loads and stores for the row streams (each read or
     // written once per sweep); the coordinates are re-read by every row and
     // stay cached.
@@ -572,6 +578,7 @@ __device__ __forceinline__ void sweep_chunk_full(const SweepArgs& a, int32_t i, 
             p1[s] = *reinterpret_cast<const int4*>(a.xy + 2 * j0 + 4);
         }
     }
+    if (halt) return false;  // as sweep_chunk
     float w[4][4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -631,17 +638,19 @@ __device__ __forceinline__ void sweep_chunk_full(const SweepArgs& a, int32_t i, 
                 for (int q = 0; q < 4; ++q) top_offer(r.top, w[s][q], g0 + 256 * s + 4 * lane + q, true, lane);
         }
     }
+    return true;
 }
 
 template <int SRC, int PW, bool ALPHA1>
 __device__ __forceinline__ void sweep_row_fast(const SweepArgs& a, int32_t i, int lane, uint64_t* topbuf,
-                                               int32_t nnj, float nne) {
+                                               int32_t nnj, float nne, bool halt) {
     SweepRow r = sweep_row_begin(a, i, topbuf);
     r.nnj = nnj;
     r.nne = nne;
     const int32_t full_end = a.n & ~1023;
-    for (int32_t g0 = 0; g0 < full_end; g0 += 1024) sweep_chunk_full<SRC, PW, ALPHA1>(a, i, g0, lane, r);
-    if (full_end < a.n) sweep_chunk(a, i, full_end, lane, r);
+    for (int32_t g0 = 0; g0 < full_end; g0 += 1024)
+        if (!sweep_chunk_full<SRC, PW, ALPHA1>(a, i, g0, lane, r, halt)) return;
+    if (full_end < a.n && !sweep_chunk(a, i, full_end, lane, r, halt)) return;
     sweep_row_end(a, i, lane, r);
 }
 
@@ -660,18 +669,19 @@ __global__ __launch_bounds__(256) void k_sweep(const SweepArgs a) {
         nne = a.eta_nn[(size_t)i * a.k + lane];
     }
     // A halted run skips whole iterations: the sweep, first kernel of an
-    // iteration, publishes the decision for the other three.
-    if (a.state != nullptr) {
-        const int32_t h = a.state->halted;
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.state->skip = h;
-        if (h) return;
-    }
+    // iteration, publishes the decision for the other three.  The flag is
+    // tested after the row's loads are issued and before its first store,
+    // so its round trip overlaps theirs.
+    const int32_t h = a.state != nullptr ? a.state->halted : 0;
+    if (i < a.n)
+        sweep_row_fast<SRC, PW, ALPHA1>(a, i, lane, topbuf + (size_t)(threadIdx.x >> 6) * TOP_BUF, nnj, nne,
+                                        h != 0);
+    if (a.state != nullptr && blockIdx.x == 0 && threadIdx.x == 0) a.state->skip = h;
+    if (h) return;
     // updateAntAges (src/Colony.cu:353-364): every ant ages by one at the
     // start of an iteration, before construction reads the ages.
     if (a.age != nullptr)
         for (int32_t k = blockIdx.x * 256 + threadIdx.x; k < a.m; k += gridDim.x * 256) a.age[k] += 1;
-    if (i >= a.n) return;
-    sweep_row_fast<SRC, PW, ALPHA1>(a, i, lane, topbuf + (size_t)(threadIdx.x >> 6) * TOP_BUF, nnj, nne);
 }
 
 // Bit-for-bit check of eta_pow_fast<PW> against the table of
