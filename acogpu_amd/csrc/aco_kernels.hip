@@ -1443,7 +1443,9 @@ __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a)
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ int32_t s_ant;
     stamp_start(a.stamp);
-    if (a.state->skip) return;
+    // a skipped iteration builds nothing; normal blocks stage first (LDS
+    // only) and test the flag after, so its round trip overlaps the staging
+    const int32_t skip = a.state->skip;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
@@ -1456,13 +1458,14 @@ __global__ __launch_bounds__(512) void k_construct_staged(const ConstructArgs a)
     const int32_t converged = a.state->converged;
     const int32_t normal_blocks = gridDim.x - a.young_blocks;
     const bool young_block = (int32_t)blockIdx.x >= normal_blocks;
-    if (young_block && (!a.aging || !converged)) return;
+    if (young_block && (skip || !a.aging || !converged)) return;
 
     // Normal blocks run one round (W ants).
     if (!young_block) {
         stage_lists_weights(nl, a.nn16, wl, a.choice_nn, nk);
         staged_zero_entry(L, nk);
         __syncthreads();
+        if (skip) return;
         const int32_t ant = blockIdx.x * nwaves + wave;
         if (ant < a.m && !(a.aging && (!a.valid[ant] || ant_is_young(a, ant, converged))))
             construct_staged_dispatch(a, ant, false, 0.0f, wl, nl, vis, iter, lane);
