@@ -143,15 +143,17 @@ def test_async_phases_equal_iterate_with_one_wait():
             b.phase_end()
 
 
-def test_iterate_stats_equals_phases():
+@pytest.mark.parametrize("w", [1, 6])
+def test_iterate_stats_equals_phases(w):
     """aco_iterate_stats (the iteration's kernels launched back to back with
     their own dispatch timestamps, statistics written to host memory by the
-    statistics kernel, one wait) gives the iteration of the phase-by-phase
-    path, with its statistics, over both pheromone buffers; the timers of the
+    statistics kernel, one wait; rank-based: statistics and deposit as one
+    launch) gives the iteration of the phase-by-phase path (two launches),
+    with its statistics, over both pheromone buffers; the timers of the
     phases that run kernels are positive."""
-    say("case: aco_iterate_stats vs waiting phases, RBAS w=1 aging n=200")
+    say(f"case: aco_iterate_stats vs waiting phases, RBAS w={w} aging n=200")
     xy = instances.uniform(200, 1000, seed=37)
-    p = capi.default_params(n=200, m=64, k=16, w=1, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS, aging=1,
+    p = capi.default_params(n=200, m=64, k=16, w=w, beta=2.0, rho=0.2, variant=capi.VARIANT_RBAS, aging=1,
                             min_age=2, max_age=4, global_perf=2, seed=9)
     with capi.Colony(p, xy=xy) as a, capi.Colony(p, xy=xy) as b:
         a.initialize()
